@@ -1,0 +1,205 @@
+"""Headline benchmark: MC samples/s and SR-step seconds of the VMC hot path
+on the 10x10 J1-J2 model (J2/J1 = 0.5, RBM alpha=4, C4v x spin-parity
+projection: 16 Pfaffian copies of 100x100, 4096 walkers, 2x10^5 samples per
+SR step), mixed-precision S (FP32 products, FP64 centring / solve) against
+the plain-FP64 S on the same samples.
+
+One "step" = make_sample (30 thermalisation sweeps + 49 stored sweeps per
+walker) + measurement (E_loc, O_k) + SR step, i.e. one SR iteration of
+main.f90:311-455.  Inputs are synthetic (random W, b ~ N(0, 0.01^2), f ~ N(0,1)
+renormalised) and resident on the device.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import multiprocessing as mp
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    sys.stderr.write(f"[bench] {msg}\n")
+    sys.stderr.flush()
+
+
+def _cpu_chain(args):
+    L, alphar, J2, nsweeps, seed = args
+    import oracle
+    from rbm_pp_amd import lattice as lt
+    spec = lt.j1j2_model(L, alphar=alphar, J2=J2)
+    p = lt.random_params(spec)
+    orc = oracle.Oracle(spec, p)
+    rng = oracle.Rng("mt", seed)
+    t0 = time.perf_counter()
+    orc.make_sample(rng, True, nsample=nsweeps, nwarmup=0, fixed_k=32)
+    return nsweeps, time.perf_counter() - t0
+
+
+def cpu_baseline(L, alphar, J2, nsweeps, procs):
+    """The oracle's restatement of make_sample (one single-threaded chain per
+    process, the reference's one-rank-per-core layout) on the host cores."""
+    ctx = mp.get_context("fork")
+    t0 = time.perf_counter()
+    with ctx.Pool(procs) as pool:
+        res = pool.map(_cpu_chain, [(L, alphar, J2, nsweeps, 1000 + r) for r in range(procs)])
+    wall = time.perf_counter() - t0
+    total = sum(r[0] for r in res)
+    slowest = max(r[1] for r in res)
+    return total / slowest, wall
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--L", type=int, default=10)
+    ap.add_argument("--alphar", type=int, default=4)
+    ap.add_argument("--J2", type=float, default=0.5)
+    ap.add_argument("--walkers", type=int, default=4096)
+    ap.add_argument("--samples", type=int, default=200000, help="stored samples per SR step")
+    ap.add_argument("--therm", type=int, default=200, help="thermalisation sweeps of the first call")
+    ap.add_argument("--recompute-k", type=int, default=32)
+    ap.add_argument("--cpu-sweeps", type=int, default=250)
+    ap.add_argument("--cpu-procs", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64-compare", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    # CPU baseline first: worker processes are forked before the GPU is touched
+    cpu = None
+    if not args.no_cpu:
+        procs = args.cpu_procs or min(16, os.cpu_count() or 1)
+        log(f"cpu baseline: {procs} chains x {args.cpu_sweeps} sweeps (oracle make_sample)")
+        rate, wall = cpu_baseline(args.L, args.alphar, args.J2, args.cpu_sweeps, procs)
+        cpu = {"value": rate, "unit": "samples/s", "cores": procs, "kind": "port",
+               "sample": f"{procs} independent chains x {args.cpu_sweeps} stored sweeps each "
+                         f"(N={args.L * args.L} proposals/sweep, recompute every 32 accepts), "
+                         f"{wall:.1f} s wall"}
+        log(f"cpu baseline: {rate:.1f} samples/s")
+
+    from rbm_pp_amd import api
+    from rbm_pp_amd import lattice as lt
+
+    spec = lt.j1j2_model(args.L, alphar=args.alphar, J2=args.J2)
+    p = lt.random_params(spec, seed_w=1, seed_f=2, sigma_w=0.01)
+    nspw = math.ceil(args.samples / args.walkers)
+    eng = api.Engine(spec, n_walkers=args.walkers, n_samples=nspw, seed=1, n_warmup=-1,
+                     n_warmup_first=args.therm, recompute_k=args.recompute_k,
+                     s_precision=api.S_FP32, params=p)
+    N, Cc = spec.N, spec.C
+    log(f"model L={args.L} N={N} copies={Cc} nv={spec.nv} walkers={args.walkers} samples/walker={nspw}")
+    t0 = time.perf_counter()
+    eng.make_sample(True, shift=0.0)
+    log(f"thermalisation ({args.therm} sweeps/walker): {time.perf_counter() - t0:.1f} s")
+
+    wf_list = [0.0] * 10
+    shift = 0.0
+    gam = eng.gamma()
+
+    def one_step(rec):
+        nonlocal shift, gam
+        t0 = time.perf_counter()
+        nacc = eng.make_sample(False, shift=shift)
+        tt = eng.timings()
+        t1 = time.perf_counter()
+        r = eng.calc_phys_quantity_opt(shift=shift)
+        t2 = time.perf_counter()
+        gam, rep = eng.sr_step(gam, s_factor0=1e-4, delta_tau=0.05, refine_iters=1)
+        t3 = time.perf_counter()
+        tt2 = eng.timings()
+        # main.f90:441-451: reweighting shift from the running mean of wf_av
+        wf_list.pop(0); wf_list.append(r["wf_av"])
+        shift = sum(wf_list) / 10.0 * 1e-3 / N / N
+        if rec is not None:
+            rec.append(dict(wall=t3 - t0, sample=t1 - t0, measure=t2 - t1, sr=t3 - t2,
+                            sweep_kernel=tt["sample"], s_build=tt2["s_build"], solve=tt2["solve"],
+                            nacc=nacc, E=r["Etot"], lam=rep["lambda_"]))
+        return nacc, r
+
+    for i in range(args.warmup):
+        nacc, r = one_step(None)
+        log(f"warmup step {i}: E/N={r['Etot'] / 4 / N:.6f}")
+    ns_step = args.walkers * nspw
+    recs = []
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        one_step(recs)
+        x = recs[-1]
+        log(f"step {i}: {x['wall']:.2f} s (sample {x['sample']:.2f}, measure {x['measure']:.2f}, "
+            f"sr {x['sr']:.3f}) E/N={x['E'] / 4 / N:.6f} accept={x['nacc'] / ns_step / N:.3f}")
+    t_total = time.perf_counter() - t_start
+
+    # FP64 S on the last samples, for the precision comparison
+    fp64 = None
+    if not args.no_fp64_compare:
+        eng.set_s_precision(api.S_FP64)
+        S64 = eng.smat()
+        t64 = eng.timings()["s_build"]
+        eng.set_s_precision(api.S_FP32)
+        S32 = eng.smat()
+        t32 = eng.timings()["s_build"]
+        fp64 = dict(s_build_fp64_s=t64, s_build_fp32_s=t32,
+                    s_rel_err_fp32=float(np.linalg.norm(S32 - S64) / np.linalg.norm(S64)))
+
+    samples = ns_step * args.steps
+    acc = sum(r["nacc"] for r in recs) / samples / N
+    nwarm = max(30, nspw // 100)
+    sweeps_per_launch = args.walkers * (nwarm + 1 + nspw)
+    t_sweep = sum(r["sweep_kernel"] for r in recs) / len(recs)
+    # SURVEY 8(d) per-sample bytes of the sampler: 8 C N^3 (1 + 3A)
+    b_unit = 8.0 * Cc * N ** 3 * (1.0 + 3.0 * acc)
+    achieved = b_unit * sweeps_per_launch / t_sweep / 1e9
+    # this kernel's own minimum: one read of every X^-1 per proposal, one
+    # read+write per accepted move
+    b_min = 8.0 * Cc * N ** 3 * (1.0 + 2.0 * acc)
+    out = {
+        "metric": "MC samples/s + SR-step s, 10x10 J1-J2; mixed-prec vs FP64 at equal energy",
+        "value": samples / t_total,
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_total / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"j1j2_{args.L}x{args.L}_J2_{args.J2}_alphar{args.alphar}_c4v_parity",
+                   "walkers": args.walkers, "samples_per_sr_step": ns_step,
+                   "sweeps_per_walker_per_step": nwarm + 1 + nspw, "recompute_k": args.recompute_k,
+                   "s_precision": "fp32 products, fp64 centring/shift/solve", "parallelism": "single_gpu"},
+        "sampler_samples_per_s": samples / sum(r["sample"] for r in recs),
+        "sr_step_s": sum(r["sr"] for r in recs) / len(recs),
+        "measure_s": sum(r["measure"] for r in recs) / len(recs),
+        "sample_s": sum(r["sample"] for r in recs) / len(recs),
+        "acceptance": acc,
+        "energy_per_site": recs[-1]["E"] / 4 / N,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_sweep", "launch_s": t_sweep,
+                     "achieved_min_traffic_GBs": b_min * sweeps_per_launch / t_sweep / 1e9},
+        "cpu_baseline": cpu,
+    }
+    if fp64:
+        out["s_build"] = fp64
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

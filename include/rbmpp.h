@@ -1,0 +1,172 @@
+/*
+ * rbmpp.h -- C ABI of the MI355X VMC hot path for the RBM x pair-product
+ * wave function of the J1-J2 Heisenberg model.
+ *
+ * The entry points replace the reference's sampler / estimator / SR-step
+ * seam (yusukenomura/RBM_PP, code/normal_code) one for one and are meant to
+ * be bound from a Fortran driver through iso_c_binding (see
+ * rbm_pp_amd/fortran/rbmpp_mod.f90).  All arrays cross the boundary in the
+ * reference's own Fortran layouts (column-major, 1-based index values where
+ * the reference files hold 1-based values), as plain pointers.  The handle
+ * owns all device memory and one HIP stream; calls are synchronous on
+ * return and return 0 on success or a negative RBMPP_E_* status (never
+ * abort).  One host thread per handle.
+ */
+#ifndef RBMPP_H
+#define RBMPP_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rbmpp_handle rbmpp_handle;
+
+/* status codes */
+#define RBMPP_OK 0
+#define RBMPP_E_ARG (-1)          /* bad argument / size */
+#define RBMPP_E_HIP (-2)          /* HIP runtime error */
+#define RBMPP_E_NOMEM (-3)        /* device allocation failed */
+#define RBMPP_E_SINGULAR (-10)    /* X singular (pivot == 0) */
+#define RBMPP_E_THETA_DRIFT (-40) /* incremental theta off by >1e-5 (make_sample.f90:84-89) */
+#define RBMPP_E_SIGN_DRIFT (-42)  /* incremental sign /= recomputed (make_sample.f90:90-92) */
+#define RBMPP_E_CANDIDATE (-30)   /* no exchange candidate found (make_sample.f90:290) */
+#define RBMPP_E_INIT (-21)        /* no initial configuration with |psi|>1e-300 (make_sample.f90:359) */
+#define RBMPP_E_SDIAG (-70)       /* S diagonal checks of SR_pdposv.f90:32-33,78-100 */
+#define RBMPP_E_NOT_SPD (-75)     /* Cholesky failed (pdposv info /= 0, SR_pdposv.f90:233) */
+#define RBMPP_E_SLATER (-5)       /* Slater_Elm not skew-symmetric (slater.f90:51-56) */
+#define RBMPP_E_STATE (-6)        /* call out of order (e.g. sr_step before measure) */
+
+/* S-matrix precision (the sampler, X^-1 chain, centring, shift and solve
+   are FP64 in every mode) */
+#define RBMPP_S_FP64 0
+#define RBMPP_S_FP32 1
+#define RBMPP_S_BF16X3 2
+
+typedef struct {
+  int32_t L;                 /* linear size; N = L*L sites (main.f90:77,85) */
+  int32_t n_sym;             /* N_MP_Trans, number of symmetry operations (main.f90:91) */
+  int32_t alphac, alphar;    /* complex / real hidden-unit densities (main.f90:79) */
+  int32_t n_slater;          /* N_Slater (main.f90:128) */
+  int32_t spin_parity;       /* +1 / -1 (main.f90:77) */
+  double J1, J2;             /* couplings (main.f90:83) */
+  int32_t n_walkers;         /* independent Markov chains (one MPI rank each in the reference) */
+  int32_t n_samples;         /* stored samples per walker per make_sample call */
+  int32_t n_warmup;          /* sweeps before storing; <0: max(30, n_samples/100) (make_sample.f90:66) */
+  int32_t n_warmup_first;    /* sweeps before storing on the first call; <0: as n_warmup */
+  int32_t n_update;          /* proposals per sweep; 0: N (main.f90:139) */
+  int32_t recompute_k;       /* >0: full Pf/X^-1 recompute after recompute_k accepts;
+                                0: adaptive Nfastup in [1,100] (make_sample.f90:74-128) */
+  int32_t s_precision;       /* RBMPP_S_* */
+  int32_t reserved;
+  uint64_t seed;             /* Philox key; walker w uses stream w */
+  double p_restart;          /* reference 0.05 (make_sample.f90:61) */
+  double p_global;           /* reference 3e-4 (make_sample.f90:135) */
+} rbmpp_config;
+
+typedef struct {
+  double s_factor0;          /* relative diagonal shift (SR_pdposv.f90:167) */
+  double delta_tau;          /* step (main.f90:79) */
+  double max_change;         /* 0.04 (main.f90:50) */
+  double slater_abs_av;      /* 0.25 (module.f90:66) */
+  int32_t refine_iters;      /* FP64 iterative-refinement sweeps after the Cholesky solve */
+  int32_t reserved;
+} rbmpp_sr_config;
+
+typedef struct {
+  double lambda;             /* sqrt(x^T S x) * 5/L (SR_pdposv.f90:309) */
+  double delw_max;           /* max |delta_tau * x_k| (SR_pdposv.f90:329-340) */
+  double scaling;            /* PP block scale (SR_pdposv.f90:40) */
+  double sdiag_av;           /* mean positive diagonal (SR_pdposv.f90:133) */
+  double s_factor;           /* max(1e-4, s_factor0) */
+  double residual;           /* ||S x - g|| / ||g|| after refinement */
+  double mean_abs_f;         /* mean |f| before renormalisation */
+  int32_t k_nonzero;         /* SR_pdposv.f90:337-341 */
+  int32_t reserved;
+} rbmpp_sr_report;
+
+/* Creates the device state.  Tables in the reference's layouts:
+ *   w_map[(g-1)*N + (i-1)]        = W_map(i,1,g), 1-based site (main.f90:106)
+ *   para_mp_trans[2*(g-1)+{0,1}]  = re/im of para_mp_trans(g)  (main.f90:98)
+ *   orbital_idx[(j-1)*N + (i-1)]  = Orbital_Idx(i,j), 1-based  (parameter.f90:93-95)
+ * Replaces the set-up of main.f90:85-259 (sizes, allocation, InitSPWeight,
+ * define_sublattice) for the GPU path. */
+int rbmpp_create(const rbmpp_config* cfg, const int32_t* w_map, const double* para_mp_trans,
+                 const int32_t* orbital_idx, rbmpp_handle** out);
+void rbmpp_destroy(rbmpp_handle* h);
+const char* rbmpp_last_error(const rbmpp_handle* h);
+int rbmpp_nv(const rbmpp_handle* h);
+
+/* Parameters (the BCAST payload of main.f90:251-259) followed by
+ * update_slater_elements (slater.f90:7-64) on the device.
+ *   wcirr[2*((f-1)*N+(iw-1))+{0,1}] = Wcirr(iw,f);  bcirr[2*(f-1)+{0,1}]
+ *   wrirr[(f-1)*N+(iw-1)] = Wrirr(iw,f);  brirr[f-1];  slater[k-1] = Slater(k) */
+int rbmpp_set_params(rbmpp_handle* h, const double* wcirr, const double* bcirr,
+                     const double* wrirr, const double* brirr, const double* slater);
+/* gamma vector in the order of main.f90:263-294 (SR_pdposv.f90:385-412) */
+int rbmpp_set_gamma(rbmpp_handle* h, const double* gamma);
+int rbmpp_get_gamma(const rbmpp_handle* h, double* gamma);
+
+/* make_sample (make_sample.f90:4-240) for all walkers; the samples stay on
+ * the device.  naccept = accepted exchanges in stored sweeps, summed over
+ * walkers (main.f90:348). */
+int rbmpp_make_sample(rbmpp_handle* h, int32_t first_iteration, double shift, int64_t* naccept);
+
+/* calc_phys_quantity_opt (calc_phys_quantity.f90:20-246) over the stored
+ * samples followed by the reductions of main.f90:347-367.  Outputs are the
+ * normalised values main.f90 holds after the reduction: etot = sum E / norm,
+ * etot2, norm (un-normalised), wf_av = sum|psi|/Ns, gvec, ovec (Nv each).  O
+ * and S stay on the device for rbmpp_sr_step. */
+int rbmpp_measure(rbmpp_handle* h, double shift, double* etot, double* etot2, double* norm,
+                  double* wf_av, double* gvec, double* ovec);
+
+/* The SR block (SR_pdposv.f90:1-414): S build in the configured precision
+ * with FP64 centring, PP block scaling, relative diagonal shift, FP64
+ * Cholesky + iterative refinement (replaces pdposv), trust region, max-change
+ * clip, f renormalisation.  gamma is updated in place and loaded as the new
+ * parameters. */
+int rbmpp_sr_step(rbmpp_handle* h, const rbmpp_sr_config* c, double* gamma,
+                  rbmpp_sr_report* rep);
+
+/* Switches the S-build precision (RBMPP_S_*) for the next SR step; no
+ * counterpart in the reference (it builds S in FP64 only). */
+int rbmpp_set_s_precision(rbmpp_handle* h, int32_t s_precision);
+
+/* calc_phys_quantity_after_opt (calc_phys_quantity.f90:250-302) +
+ * main.f90:528-542 reductions. */
+int rbmpp_measure_energy(rbmpp_handle* h, double shift, double* etot, double* etot2,
+                         double* norm, double* wf_av);
+
+/* ---- parity / test entry points on caller-given configurations ---- */
+/* Sample store: x[(s)*N + i] = +-1, s < n (walker-major: s = w*n_samples + k). */
+int rbmpp_set_samples(rbmpp_handle* h, int32_t n, const int32_t* x);
+int rbmpp_get_samples(const rbmpp_handle* h, int32_t* x, double* psi);
+int64_t rbmpp_n_samples(const rbmpp_handle* h);
+/* Walker state after make_sample: x[w*N+i], pf[w*C+q], psi[2w+{0,1}] and
+ * (optional) InvM in the reference's layout invm[w][q][(j)*N+(i)] = InvM(i+1,j+1,q+1). */
+int rbmpp_get_walkers(const rbmpp_handle* h, int32_t* x, double* pf, double* psi, double* invm,
+                      int64_t* naccept_per_walker);
+/* Full evaluation (calc_theta, calculate_MAll, sign_PP, calc_amplitude_RBM_PP)
+ * of n configurations: psi[s][0..C] complex, pf[s][C], invm optional. */
+int rbmpp_eval(rbmpp_handle* h, int32_t n, const int32_t* x, double* psi, double* pf,
+               double* invm);
+/* Pf ratios (calculate_new_PfM_two) for one exchange per configuration:
+ * i1[s], i2[s] 0-based sites with opposite spins; outputs pf_new[s][C] and
+ * psi_new[s] (the amplitude of the exchanged configuration). */
+int rbmpp_ratio_exchange(rbmpp_handle* h, int32_t n, const int32_t* x, const int32_t* i1,
+                         const int32_t* i2, double* pf_new, double* psi_new);
+/* calc_local_energy_correlation + O_k of calc_phys_quantity.f90:114-222 for n
+ * configurations, not reweighted: eloc[s] complex (sum_x' H psi(x')), psi[s]
+ * complex, oloc[s][Nv] complex (d psi / d gamma_k); oloc may be NULL. */
+int rbmpp_local(rbmpp_handle* h, int32_t n, const int32_t* x, double* eloc, double* psi,
+                double* oloc);
+/* The centred, scaled-free S = <O O^T>/norm - <O><O>^T of the last
+ * rbmpp_measure, Nv x Nv column-major, in the configured precision. */
+int rbmpp_get_smat(rbmpp_handle* h, double* s);
+/* phase timings of the last calls in seconds:
+ * {slater, sample, measure, s_build, solve, sr_total, recompute_count, reserved} */
+int rbmpp_timings(const rbmpp_handle* h, double* t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,823 @@
+// Host side of the C ABI (include/rbmpp.h): device state of one model, the
+// make_sample / measure / SR-step phases, and the parity entry points.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/rbmpp.h"
+#include "rbmpp_kernels.h"
+#include "rbmpp_sr.h"
+
+namespace {
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+};
+
+}  // namespace
+
+struct rbmpp_handle {
+  rbmpp_config cfg;
+  int L, N, P, C, ac, ar, nsl, nv, nv_rbm, mc, mr, npair, W, nspw;
+  int real_only;
+  hipStream_t st = nullptr;
+  std::string err;
+  std::vector<int> wmap, invmap, orb;
+  std::vector<unsigned char> subB;
+  std::vector<double> chi, gamma;
+  double sp_cs[2], sp_cc[2], sp_ss[2], sp_w[2];
+  std::vector<void*> allocs;
+  // tables and parameters
+  int *d_wmap = nullptr, *d_invmap = nullptr, *d_orb = nullptr;
+  unsigned char* d_subB = nullptr;
+  double2* d_chi = nullptr;
+  double* d_se = nullptr;
+  double *d_Wr = nullptr, *d_br = nullptr, *d_f = nullptr;
+  double2 *d_Wc = nullptr, *d_bc = nullptr;
+  // walkers and samples
+  WalkerState ws;
+  signed char* d_xs = nullptr;
+  double2* d_psis = nullptr;
+  int64_t n_stored = 0;
+  int walkers_ready = 0;
+  // measurement
+  size_t meas_cap = 0;  // configurations the measurement buffers hold
+  double2 *d_mpsi = nullptr, *d_eloc = nullptr;
+  double *d_rw = nullptr, *d_orows = nullptr, *d_kappa = nullptr, *d_part = nullptr;
+  double* d_scr = nullptr;  // recompute scratch of the local kernel (N > RBMPP_NLDS)
+  int* d_status = nullptr;
+  int local_grid = 0;
+  size_t part_cap = 0;
+  // SR
+  double *d_S = nullptr, *d_A = nullptr, *d_L = nullptr, *d_vec = nullptr;
+  int* d_info = nullptr;
+  int measured = 0;
+  int s_valid = 0;
+  double m_norm = 0, m_etot = 0, m_etot2 = 0, m_wfav = 0;
+  std::vector<double> m_g, m_o;
+  double t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+
+static int fail(rbmpp_handle* h, int code, const std::string& msg) {
+  if (h) h->err = msg;
+  return code;
+}
+
+#define HIPCHK(h, x)                                                          \
+  do {                                                                        \
+    hipError_t e_ = (x);                                                      \
+    if (e_ != hipSuccess)                                                     \
+      return fail(h, RBMPP_E_HIP, std::string(#x ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+static int dalloc(rbmpp_handle* h, T** p, size_t n) {
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(h, RBMPP_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  }
+  h->allocs.push_back((void*)*p);
+  return 0;
+}
+
+static DevModel devmodel(const rbmpp_handle* h) {
+  DevModel m;
+  m.L = h->L; m.N = h->N; m.P = h->P; m.C = h->C; m.ac = h->ac; m.ar = h->ar;
+  m.npair = h->npair; m.nsl = h->nsl; m.nv = h->nv; m.nv_rbm = h->nv_rbm; m.mc = h->mc; m.mr = h->mr;
+  m.real_only = h->real_only;
+  m.J1 = h->cfg.J1; m.J2 = h->cfg.J2;
+  for (int s = 0; s < 2; s++) {
+    m.sp_cs[s] = h->sp_cs[s]; m.sp_cc[s] = h->sp_cc[s]; m.sp_ss[s] = h->sp_ss[s]; m.sp_w[s] = h->sp_w[s];
+  }
+  m.wmap = h->d_wmap; m.invmap = h->d_invmap; m.orb = h->d_orb; m.subB = h->d_subB; m.chi = h->d_chi;
+  m.se = h->d_se; m.Wr = h->d_Wr; m.br = h->d_br; m.Wc = h->d_Wc; m.bc = h->d_bc;
+  return m;
+}
+
+static void tic(rbmpp_handle* h) { hipEventRecord(h->e0, h->st); }
+static double toc(rbmpp_handle* h) {
+  hipEventRecord(h->e1, h->st);
+  hipEventSynchronize(h->e1);
+  float ms = 0;
+  hipEventElapsedTime(&ms, h->e0, h->e1);
+  return ms * 1e-3;
+}
+
+extern "C" {
+
+const char* rbmpp_last_error(const rbmpp_handle* h) { return h ? h->err.c_str() : "null handle"; }
+int rbmpp_nv(const rbmpp_handle* h) { return h ? h->nv : 0; }
+
+void rbmpp_destroy(rbmpp_handle* h) {
+  if (!h) return;
+  if (h->st) hipStreamSynchronize(h->st);
+  for (void* p : h->allocs) hipFree(p);
+  if (h->e0) hipEventDestroy(h->e0);
+  if (h->e1) hipEventDestroy(h->e1);
+  if (h->st) hipStreamDestroy(h->st);
+  delete h;
+}
+
+int rbmpp_create(const rbmpp_config* cfg, const int32_t* w_map, const double* para_mp_trans,
+                 const int32_t* orbital_idx, rbmpp_handle** out) {
+  if (!cfg || !w_map || !para_mp_trans || !orbital_idx || !out) return RBMPP_E_ARG;
+  *out = nullptr;
+  rbmpp_handle* h = new rbmpp_handle();
+  h->cfg = *cfg;
+  const int L = cfg->L, N = L * L, P = cfg->n_sym;
+  h->L = L; h->N = N; h->P = P; h->C = 2 * P; h->ac = cfg->alphac; h->ar = cfg->alphar;
+  h->nsl = cfg->n_slater; h->npair = N / 2;
+  h->mc = h->ac * P * 2; h->mr = h->ar * P * 2;
+  h->nv = h->ac * 2 * (N + 1) + h->ar * (N + 1) + h->nsl;
+  h->nv_rbm = h->nv - h->nsl;
+  h->W = cfg->n_walkers; h->nspw = cfg->n_samples;
+  auto bad = [&](const char* m) { int r = fail(h, RBMPP_E_ARG, m); *out = h; return r; };
+  if (L < 2 || N % 2) return bad("L must give an even number of sites");
+  if (P < 1 || h->C > RBMPP_CMAX) return bad("n_sym must be in [1, 8]");
+  if (h->ac < 0 || h->ar < 0 || h->mr > RBMPP_MRMAX || h->mc > RBMPP_MCMAX) return bad("alpha out of range");
+  if (h->nsl < 1 || h->nsl > 4 * N) return bad("n_slater must be in [1, 4N]");
+  if (N > 256) return bad("N > 256 not supported");
+  if (h->W < 1 || h->nspw < 0) return bad("n_walkers / n_samples");
+  if (cfg->spin_parity != 1 && cfg->spin_parity != -1) return bad("spin_parity must be +-1");
+  if (cfg->s_precision < 0 || cfg->s_precision > 2) return bad("s_precision");
+  // tables: 1-based Fortran layouts -> 0-based
+  h->wmap.resize((size_t)P * N);
+  h->invmap.resize((size_t)P * N);
+  for (int g = 0; g < P; g++) {
+    std::vector<int> seen(N, 0);
+    for (int i = 0; i < N; i++) {
+      const int v = w_map[(size_t)g * N + i] - 1;
+      if (v < 0 || v >= N || seen[v]) return bad("W_map is not a permutation (main.f90:110-119)");
+      seen[v] = 1;
+      h->wmap[(size_t)g * N + i] = v;
+      h->invmap[(size_t)g * N + v] = i;
+    }
+  }
+  h->orb.resize((size_t)N * N);
+  for (int j = 0; j < N; j++)
+    for (int i = 0; i < N; i++) {
+      const int v = orbital_idx[(size_t)j * N + i] - 1;
+      if (v < 0 || v >= h->nsl) return bad("Orbital_Idx out of range (parameter.f90:98)");
+      h->orb[(size_t)i * N + j] = v;
+    }
+  h->chi.assign(para_mp_trans, para_mp_trans + 2 * P);
+  h->real_only = (h->ac == 0);
+  for (int g = 0; g < P; g++)
+    if (h->chi[2 * g + 1] != 0.0) h->real_only = 0;
+  h->subB.resize(N);
+  for (int s = 0; s < N; s++) h->subB[s] = (unsigned char)(((s / L) + (s % L)) & 1);
+  // InitSPWeight (spin_projection.f90:5-79) with beta = {0, pi}
+  const double pi = std::acos(-1.0);
+  const double beta[2] = {0.0, pi};
+  for (int s = 0; s < 2; s++) {
+    const double c = std::cos(0.5 * beta[s]), sn = std::sin(0.5 * beta[s]);
+    h->sp_cc[s] = c * c; h->sp_cs[s] = c * sn; h->sp_ss[s] = sn * sn;
+  }
+  h->sp_w[0] = 1.0;
+  h->sp_w[1] = cfg->spin_parity == 1 ? 1.0 : -1.0;
+  h->gamma.assign(h->nv, 0.0);
+  *out = h;
+
+  HIPCHK(h, hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
+  HIPCHK(h, hipEventCreate(&h->e0));
+  HIPCHK(h, hipEventCreate(&h->e1));
+  int rc = 0;
+  const size_t W = h->W, C = h->C, NN = (size_t)N * N;
+  if ((rc = dalloc(h, &h->d_wmap, (size_t)P * N)) || (rc = dalloc(h, &h->d_invmap, (size_t)P * N)) ||
+      (rc = dalloc(h, &h->d_orb, NN)) || (rc = dalloc(h, &h->d_subB, N)) || (rc = dalloc(h, &h->d_chi, P)) ||
+      (rc = dalloc(h, &h->d_se, C * 4 * NN)) || (rc = dalloc(h, &h->d_Wr, (size_t)h->ar * N)) ||
+      (rc = dalloc(h, &h->d_br, h->ar)) || (rc = dalloc(h, &h->d_Wc, (size_t)h->ac * N)) ||
+      (rc = dalloc(h, &h->d_bc, h->ac)) || (rc = dalloc(h, &h->d_f, h->nsl)))
+    return rc;
+  WalkerState& ws = h->ws;
+  if ((rc = dalloc(h, &ws.x, W * N)) || (rc = dalloc(h, &ws.eidx, W * N)) || (rc = dalloc(h, &ws.ecfg, W * N)) ||
+      (rc = dalloc(h, &ws.thr, W * std::max(h->mr, 1))) || (rc = dalloc(h, &ws.thc, W * std::max(h->mc, 1))) ||
+      (rc = dalloc(h, &ws.pf, W * C)) || (rc = dalloc(h, &ws.sgn, W * P)) || (rc = dalloc(h, &ws.psi, W)) ||
+      (rc = dalloc(h, &ws.minv, W * C * NN)) || (rc = dalloc(h, &ws.nacc_tmp, W)) ||
+      (rc = dalloc(h, &ws.nfastup, W)) || (rc = dalloc(h, &ws.ctr, W)) || (rc = dalloc(h, &ws.status, W)) ||
+      (rc = dalloc(h, &ws.naccept, W)) || (rc = dalloc(h, &ws.nrecompute, W)))
+    return rc;
+  ws.scratch = nullptr;
+  if (N > RBMPP_NLDS && (rc = dalloc(h, &ws.scratch, W * NN))) return rc;
+  const size_t ns = (size_t)W * std::max(h->nspw, 1);
+  if ((rc = dalloc(h, &h->d_xs, ns * N)) || (rc = dalloc(h, &h->d_psis, ns))) return rc;
+  HIPCHK(h, hipMemcpyAsync(h->d_wmap, h->wmap.data(), sizeof(int) * P * N, hipMemcpyHostToDevice, h->st));
+  HIPCHK(h, hipMemcpyAsync(h->d_invmap, h->invmap.data(), sizeof(int) * P * N, hipMemcpyHostToDevice, h->st));
+  HIPCHK(h, hipMemcpyAsync(h->d_orb, h->orb.data(), sizeof(int) * NN, hipMemcpyHostToDevice, h->st));
+  HIPCHK(h, hipMemcpyAsync(h->d_subB, h->subB.data(), N, hipMemcpyHostToDevice, h->st));
+  HIPCHK(h, hipMemcpyAsync(h->d_chi, h->chi.data(), sizeof(double) * 2 * P, hipMemcpyHostToDevice, h->st));
+  // walker bookkeeping: Nfastup = min(100, 2N) (main.f90:309), counters 0
+  std::vector<int> nf(W, std::min(100, 2 * N)), zi(W, 0);
+  std::vector<unsigned long long> zc(W, 0ull);
+  HIPCHK(h, hipMemcpyAsync(ws.nfastup, nf.data(), sizeof(int) * W, hipMemcpyHostToDevice, h->st));
+  HIPCHK(h, hipMemcpyAsync(ws.nacc_tmp, zi.data(), sizeof(int) * W, hipMemcpyHostToDevice, h->st));
+  HIPCHK(h, hipMemcpyAsync(ws.ctr, zc.data(), sizeof(unsigned long long) * W, hipMemcpyHostToDevice, h->st));
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  return RBMPP_OK;
+}
+
+int rbmpp_set_params(rbmpp_handle* h, const double* wcirr, const double* bcirr, const double* wrirr,
+                     const double* brirr, const double* slater) {
+  if (!h || !slater) return RBMPP_E_ARG;
+  const int N = h->N;
+  std::vector<double> g(h->nv);
+  int k = 0;
+  for (int f = 0; f < h->ac; f++) { g[k++] = bcirr[2 * f]; g[k++] = bcirr[2 * f + 1]; }
+  for (int f = 0; f < h->ac; f++)
+    for (int iw = 0; iw < N; iw++) { g[k++] = wcirr[2 * (f * N + iw)]; g[k++] = wcirr[2 * (f * N + iw) + 1]; }
+  for (int f = 0; f < h->ar; f++) g[k++] = brirr[f];
+  for (int f = 0; f < h->ar; f++)
+    for (int iw = 0; iw < N; iw++) g[k++] = wrirr[f * N + iw];
+  for (int o = 0; o < h->nsl; o++) g[k++] = slater[o];
+  return rbmpp_set_gamma(h, g.data());
+}
+
+int rbmpp_set_gamma(rbmpp_handle* h, const double* gamma) {
+  if (!h || !gamma) return RBMPP_E_ARG;
+  const int N = h->N;
+  h->gamma.assign(gamma, gamma + h->nv);
+  std::vector<double> Wr((size_t)h->ar * N), br(h->ar), Wc((size_t)2 * h->ac * N), bc(2 * h->ac), f(h->nsl);
+  int k = 0;
+  for (int q = 0; q < h->ac; q++) { bc[2 * q] = gamma[k++]; bc[2 * q + 1] = gamma[k++]; }
+  for (int q = 0; q < h->ac; q++)
+    for (int iw = 0; iw < N; iw++) { Wc[2 * (q * N + iw)] = gamma[k++]; Wc[2 * (q * N + iw) + 1] = gamma[k++]; }
+  for (int q = 0; q < h->ar; q++) br[q] = gamma[k++];
+  for (int q = 0; q < h->ar; q++)
+    for (int iw = 0; iw < N; iw++) Wr[q * N + iw] = gamma[k++];
+  for (int o = 0; o < h->nsl; o++) f[o] = gamma[k++];
+  if (h->ar) {
+    HIPCHK(h, hipMemcpyAsync(h->d_Wr, Wr.data(), sizeof(double) * Wr.size(), hipMemcpyHostToDevice, h->st));
+    HIPCHK(h, hipMemcpyAsync(h->d_br, br.data(), sizeof(double) * br.size(), hipMemcpyHostToDevice, h->st));
+  }
+  if (h->ac) {
+    HIPCHK(h, hipMemcpyAsync(h->d_Wc, Wc.data(), sizeof(double) * Wc.size(), hipMemcpyHostToDevice, h->st));
+    HIPCHK(h, hipMemcpyAsync(h->d_bc, bc.data(), sizeof(double) * bc.size(), hipMemcpyHostToDevice, h->st));
+  }
+  HIPCHK(h, hipMemcpyAsync(h->d_f, f.data(), sizeof(double) * f.size(), hipMemcpyHostToDevice, h->st));
+  tic(h);
+  launch_slater(devmodel(h), h->d_f, h->d_se, nullptr, h->st);
+  HIPCHK(h, hipGetLastError());
+  h->t[0] = toc(h);
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  h->s_valid = 0;
+  return RBMPP_OK;
+}
+
+int rbmpp_get_gamma(const rbmpp_handle* h, double* gamma) {
+  if (!h || !gamma) return RBMPP_E_ARG;
+  std::copy(h->gamma.begin(), h->gamma.end(), gamma);
+  return RBMPP_OK;
+}
+
+int rbmpp_make_sample(rbmpp_handle* h, int32_t first_iteration, double shift, int64_t* naccept) {
+  if (!h) return RBMPP_E_ARG;
+  if (!first_iteration && !h->walkers_ready) return fail(h, RBMPP_E_STATE, "make_sample: walkers not initialised");
+  SweepArgs a;
+  a.m = devmodel(h);
+  a.ws = h->ws;
+  a.nwalk = h->W;
+  a.nsamp = h->nspw;
+  int nwarm = h->cfg.n_warmup;
+  if (first_iteration && h->cfg.n_warmup_first >= 0) nwarm = h->cfg.n_warmup_first;
+  if (nwarm < 0) nwarm = std::max(30, h->nspw / 100);  // make_sample.f90:66
+  a.nwarm = nwarm;
+  a.nupdate = h->cfg.n_update > 0 ? h->cfg.n_update : h->N;
+  a.fixed_k = h->cfg.recompute_k;
+  a.first = first_iteration ? 1 : 0;
+  a.shift = shift;
+  a.p_restart = h->cfg.p_restart;
+  a.p_global = h->cfg.p_global;
+  a.key0 = (uint32_t)h->cfg.seed;
+  a.key1 = (uint32_t)(h->cfg.seed >> 32);
+  a.xs = h->d_xs;
+  a.psis = h->d_psis;
+  tic(h);
+  if (launch_sweep(a, h->st) != 0) return fail(h, RBMPP_E_ARG, "sweep: unsupported size");
+  HIPCHK(h, hipGetLastError());
+  h->t[1] = toc(h);
+  std::vector<int> stat(h->W);
+  std::vector<long long> nacc(h->W);
+  std::vector<int> nrec(h->W);
+  HIPCHK(h, hipMemcpy(stat.data(), h->ws.status, sizeof(int) * h->W, hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(nacc.data(), h->ws.naccept, sizeof(long long) * h->W, hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(nrec.data(), h->ws.nrecompute, sizeof(int) * h->W, hipMemcpyDeviceToHost));
+  long long tot = 0, trec = 0;
+  for (int w = 0; w < h->W; w++) {
+    if (stat[w] != 0) {
+      char buf[128];
+      snprintf(buf, sizeof buf, "make_sample: walker %d status %d", w, stat[w]);
+      return fail(h, stat[w], buf);
+    }
+    tot += nacc[w];
+    trec += nrec[w];
+  }
+  h->t[6] = (double)trec;
+  if (naccept) *naccept = tot;
+  h->walkers_ready = 1;
+  h->n_stored = (int64_t)h->W * h->nspw;
+  h->measured = 0;
+  return RBMPP_OK;
+}
+
+// (re)allocate the measurement buffers for n configurations
+static int ensure_measure(rbmpp_handle* h, size_t n) {
+  if (n <= h->meas_cap && h->d_mpsi) return 0;
+  int rc;
+  const size_t rows = h->real_only ? n : 2 * n;
+  if ((rc = dalloc(h, &h->d_mpsi, n)) || (rc = dalloc(h, &h->d_eloc, n)) || (rc = dalloc(h, &h->d_rw, n)) ||
+      (rc = dalloc(h, &h->d_orows, rows * h->nv)) || (rc = dalloc(h, &h->d_kappa, rows)) ||
+      (rc = dalloc(h, &h->d_status, n)))
+    return rc;
+  h->meas_cap = n;
+  // the local kernel is grid-strided; N > RBMPP_NLDS needs one N x N
+  // recompute scratch per block, sized once for the largest grid
+  h->local_grid = h->N > RBMPP_NLDS ? 1024 : 4096;
+  if (h->N > RBMPP_NLDS && !h->d_scr &&
+      (rc = dalloc(h, &h->d_scr, (size_t)h->local_grid * h->N * h->N)))
+    return rc;
+  return 0;
+}
+
+static int ensure_part(rbmpp_handle* h, size_t n) {
+  if (n <= h->part_cap) return 0;
+  int rc = dalloc(h, &h->d_part, n);
+  if (rc) return rc;
+  h->part_cap = n;
+  return 0;
+}
+
+static int run_local(rbmpp_handle* h, int n, const signed char* d_x, int mode, double shift, const int* ex1,
+                     const int* ex2, double* d_invm, double* d_pf, double2* d_psiq, double2* d_expsi,
+                     double* d_expf) {
+  int rc = ensure_measure(h, (size_t)n);
+  if (rc) return rc;
+  LocalArgs a;
+  memset(&a, 0, sizeof a);
+  a.m = devmodel(h);
+  a.nconf = n;
+  a.xs = d_x;
+  a.mode = mode;
+  a.shift = shift;
+  a.ex_i1 = ex1;
+  a.ex_i2 = ex2;
+  a.scratch = h->d_scr;
+  a.psi = h->d_mpsi;
+  a.pf = d_pf;
+  a.psi_q = d_psiq;
+  a.invm = d_invm;
+  a.eloc = h->d_eloc;
+  a.orows = h->d_orows;
+  a.rw = h->d_rw;
+  a.ex_psi = d_expsi;
+  a.ex_pf = d_expf;
+  a.status = h->d_status;
+  const int grid = std::min(n, h->local_grid);
+  if (launch_local(a, grid, h->st) != 0) return fail(h, RBMPP_E_ARG, "local: unsupported size");
+  HIPCHK(h, hipGetLastError());
+  std::vector<int> stat(n);
+  HIPCHK(h, hipMemcpyAsync(stat.data(), h->d_status, sizeof(int) * n, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  for (int s = 0; s < n; s++)
+    if (stat[s] != 0) {
+      char buf[128];
+      snprintf(buf, sizeof buf, "configuration %d: status %d", s, stat[s]);
+      return fail(h, stat[s], buf);
+    }
+  return 0;
+}
+
+// sums over the measured configurations (main.f90:351-367)
+static int reduce_measure(rbmpp_handle* h, int n, bool with_o) {
+  const int nblk = 256;
+  const int nchunk = std::max(1, std::min(256, n / 64));
+  int rc = ensure_part(h, std::max<size_t>((size_t)nblk * 4, (size_t)(nchunk + 1) * 2 * h->nv));
+  if (rc) return rc;
+  launch_scalar_partials(n, h->d_mpsi, h->d_rw, h->d_eloc, h->d_part, nblk, h->st);
+  std::vector<double> part((size_t)nblk * 4);
+  HIPCHK(h, hipMemcpyAsync(part.data(), h->d_part, sizeof(double) * part.size(), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  double s[4] = {0, 0, 0, 0};
+  for (int b = 0; b < nblk; b++)
+    for (int k = 0; k < 4; k++) s[k] += part[(size_t)b * 4 + k];
+  h->m_norm = s[2];
+  h->m_etot = s[0] / s[2];
+  h->m_etot2 = s[1] / s[2];
+  h->m_wfav = s[3] / (double)n;
+  if (with_o) {
+    launch_gvec_partials(n, h->nv, h->real_only ? 0 : 1, h->d_orows, h->d_mpsi, h->d_rw, h->d_eloc, h->d_part,
+                         nchunk, h->st);
+    HIPCHK(h, hipGetLastError());
+    std::vector<double> go(2 * h->nv);
+    HIPCHK(h, hipMemcpyAsync(go.data(), h->d_part + (size_t)nchunk * 2 * h->nv, sizeof(double) * 2 * h->nv,
+                             hipMemcpyDeviceToHost, h->st));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    h->m_g.assign(go.begin(), go.begin() + h->nv);
+    h->m_o.assign(go.begin() + h->nv, go.end());
+    for (int k = 0; k < h->nv; k++) { h->m_g[k] /= h->m_norm; h->m_o[k] /= h->m_norm; }
+  }
+  return 0;
+}
+
+int rbmpp_measure(rbmpp_handle* h, double shift, double* etot, double* etot2, double* norm, double* wf_av,
+                  double* gvec, double* ovec) {
+  if (!h) return RBMPP_E_ARG;
+  if (h->n_stored <= 0) return fail(h, RBMPP_E_STATE, "measure: no samples");
+  const int n = (int)h->n_stored;
+  tic(h);
+  int rc = run_local(h, n, h->d_xs, 1 | 2, shift, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+  if (rc) return rc;
+  rc = reduce_measure(h, n, true);
+  if (rc) return rc;
+  h->t[2] = toc(h);
+  if (etot) *etot = h->m_etot;
+  if (etot2) *etot2 = h->m_etot2;
+  if (norm) *norm = h->m_norm;
+  if (wf_av) *wf_av = h->m_wfav;
+  if (gvec) std::copy(h->m_g.begin(), h->m_g.end(), gvec);
+  if (ovec) std::copy(h->m_o.begin(), h->m_o.end(), ovec);
+  h->measured = 1;
+  h->s_valid = 0;
+  return RBMPP_OK;
+}
+
+int rbmpp_measure_energy(rbmpp_handle* h, double shift, double* etot, double* etot2, double* norm,
+                         double* wf_av) {
+  if (!h) return RBMPP_E_ARG;
+  if (h->n_stored <= 0) return fail(h, RBMPP_E_STATE, "measure_energy: no samples");
+  const int n = (int)h->n_stored;
+  tic(h);
+  int rc = run_local(h, n, h->d_xs, 1, shift, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+  if (rc) return rc;
+  rc = reduce_measure(h, n, false);
+  if (rc) return rc;
+  h->t[2] = toc(h);
+  if (etot) *etot = h->m_etot;
+  if (etot2) *etot2 = h->m_etot2;
+  if (norm) *norm = h->m_norm;
+  if (wf_av) *wf_av = h->m_wfav;
+  return RBMPP_OK;
+}
+
+// centred S = <O O^T>/norm - <O><O>^T in the configured precision
+static int build_s(rbmpp_handle* h) {
+  const int nv = h->nv, n = (int)h->n_stored;
+  int rc;
+  if (!h->d_S && ((rc = dalloc(h, &h->d_S, (size_t)nv * nv)) || (rc = dalloc(h, &h->d_A, (size_t)nv * nv)) ||
+                  (rc = dalloc(h, &h->d_L, (size_t)nv * nv)) || (rc = dalloc(h, &h->d_vec, (size_t)4 * nv)) ||
+                  (rc = dalloc(h, &h->d_info, 1))))
+    return rc;
+  const int rows = h->real_only ? n : 2 * n;
+  const int nt = syrk_tiles(nv), np = nt * (nt + 1) / 2;
+  int nsplit = std::max(1, std::min(64, (2048 + np - 1) / np));
+  nsplit = std::max(1, std::min(nsplit, rows / 64));
+  if ((rc = ensure_part(h, (size_t)nsplit * np * 64 * 64))) return rc;
+  // obar to device
+  HIPCHK(h, hipMemcpyAsync(h->d_vec, h->m_o.data(), sizeof(double) * nv, hipMemcpyHostToDevice, h->st));
+  launch_row_kappa(n, h->real_only ? 0 : 1, h->d_mpsi, h->d_rw, h->d_kappa, h->st);
+  const int prec = h->cfg.s_precision == RBMPP_S_FP32 ? 1 : 0;
+  launch_syrk(prec, rows, nv, h->d_orows, h->d_kappa, h->d_vec, nsplit, h->d_part, 1.0 / h->m_norm, h->d_S,
+              h->st);
+  HIPCHK(h, hipGetLastError());
+  h->s_valid = 1;
+  return 0;
+}
+
+int rbmpp_get_smat(rbmpp_handle* h, double* s) {
+  if (!h || !s) return RBMPP_E_ARG;
+  if (!h->measured) return fail(h, RBMPP_E_STATE, "get_smat: no measurement");
+  if (!h->s_valid) {
+    int rc = build_s(h);
+    if (rc) return rc;
+  }
+  HIPCHK(h, hipMemcpyAsync(s, h->d_S, sizeof(double) * h->nv * h->nv, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  return RBMPP_OK;
+}
+
+int rbmpp_sr_step(rbmpp_handle* h, const rbmpp_sr_config* c, double* gamma, rbmpp_sr_report* rep) {
+  if (!h || !c || !gamma) return RBMPP_E_ARG;
+  if (!h->measured) return fail(h, RBMPP_E_STATE, "sr_step before measure");
+  const int nv = h->nv, nvr = h->nv_rbm, N = h->N, nsl = h->nsl;
+  tic(h);
+  int rc = build_s(h);
+  if (rc) return rc;
+  h->t[3] = toc(h);
+  hipEvent_t ea, eb;
+  hipEventCreate(&ea);
+  hipEventCreate(&eb);
+  hipEventRecord(ea, h->st);
+  // SR_pdposv.f90:4-6 : g <- g - 2 E <O>
+  std::vector<double> g(nv);
+  for (int k = 0; k < nv; k++) g[k] = h->m_g[k] - 2.0 * h->m_etot * h->m_o[k];
+  // SR_pdposv.f90:18-40 : block scaling factor from mean positive diagonals
+  std::vector<double> d(nv);
+  launch_diag(nv, h->d_S, h->d_vec + nv, h->st);
+  HIPCHK(h, hipMemcpyAsync(d.data(), h->d_vec + nv, sizeof(double) * nv, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  double avR = 0, avP = 0;
+  int i1 = 0, i2 = 0;
+  for (int k = 0; k < nv; k++)
+    if (d[k] > 0.0) {
+      if (k < nvr) { i1++; avR += d[k]; } else { i2++; avP += d[k]; }
+    }
+  if (i1 != nvr) return fail(h, RBMPP_E_SDIAG, "i1/=Nv_RBM (SR_pdposv.f90:32)");
+  if (i2 != nsl - nsl / N) return fail(h, RBMPP_E_SDIAG, "i2 /= N_Slater - N_Slater/N (SR_pdposv.f90:33)");
+  avR = nvr > 0 ? avR / i1 : 1.0;
+  avP = avP / i2;
+  const double scal = std::sqrt(avR / avP);
+  std::vector<int> fii(nv, 0);
+  for (int i = 0; i < N; i++) fii[h->orb[(size_t)i * N + i] + nv - nsl] = 1;
+  double sav = 0;
+  int cnt = 0;
+  for (int k = 0; k < nv; k++) {
+    const double dk = d[k] * (k >= nvr ? scal * scal : 1.0);
+    if (dk < 0.0) return fail(h, RBMPP_E_SDIAG, "diagonal element of S matrix is not positive");
+    if (dk == 0.0 && !fii[k]) return fail(h, RBMPP_E_SDIAG, "zero S diagonal for other than f_ii");
+    if (dk > 0.0 && fii[k]) return fail(h, RBMPP_E_SDIAG, "positive S diagonal for f_ii");
+    if (dk > 0.0) { sav += dk; cnt++; }
+  }
+  sav /= cnt;
+  const double sf = std::max(1e-4, c->s_factor0);
+  for (int k = nvr; k < nv; k++) g[k] *= scal;
+  // A = D S D + Sdiag_av * s_factor I ; L = chol(A)
+  launch_scale_shift(nv, nvr, scal, sav * sf, h->d_S, h->d_A, h->st);
+  HIPCHK(h, hipMemcpyAsync(h->d_L, h->d_A, sizeof(double) * nv * nv, hipMemcpyDeviceToDevice, h->st));
+  HIPCHK(h, hipMemsetAsync(h->d_info, 0, sizeof(int), h->st));
+  launch_cholesky(nv, h->d_L, h->d_info, h->st);
+  HIPCHK(h, hipGetLastError());
+  int info = 0;
+  HIPCHK(h, hipMemcpyAsync(&info, h->d_info, sizeof(int), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  if (info != 0) return fail(h, RBMPP_E_NOT_SPD, "Cholesky: matrix not positive definite");
+  // x = A^-1 g with FP64 iterative refinement
+  double* dx = h->d_vec;           // solution
+  double* dr = h->d_vec + nv;      // residual / correction
+  double* dy = h->d_vec + 2 * nv;  // A x
+  HIPCHK(h, hipMemcpyAsync(dx, g.data(), sizeof(double) * nv, hipMemcpyHostToDevice, h->st));
+  launch_chol_solve(nv, h->d_L, dx, h->st);
+  std::vector<double> x(nv), ax(nv);
+  double resid = 0;
+  for (int it = 0; it <= c->refine_iters; it++) {
+    launch_gemv(nv, h->d_A, dx, dy, h->st);
+    HIPCHK(h, hipMemcpyAsync(x.data(), dx, sizeof(double) * nv, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(h, hipMemcpyAsync(ax.data(), dy, sizeof(double) * nv, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    double rn = 0, gn = 0;
+    std::vector<double> r(nv);
+    for (int k = 0; k < nv; k++) { r[k] = g[k] - ax[k]; rn += r[k] * r[k]; gn += g[k] * g[k]; }
+    resid = std::sqrt(rn / std::max(gn, 1e-300));
+    if (it == c->refine_iters) break;
+    HIPCHK(h, hipMemcpyAsync(dr, r.data(), sizeof(double) * nv, hipMemcpyHostToDevice, h->st));
+    launch_chol_solve(nv, h->d_L, dr, h->st);
+    std::vector<double> corr(nv);
+    HIPCHK(h, hipMemcpyAsync(corr.data(), dr, sizeof(double) * nv, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    for (int k = 0; k < nv; k++) x[k] += corr[k];
+    HIPCHK(h, hipMemcpyAsync(dx, x.data(), sizeof(double) * nv, hipMemcpyHostToDevice, h->st));
+  }
+  // SR_pdposv.f90:251-311 : lambda = sqrt(x^T A x), trust region 5/L
+  double lam = 0;
+  for (int k = 0; k < nv; k++) lam += x[k] * ax[k];
+  lam = std::sqrt(lam) * 5.0 / (double)h->L;
+  if (lam > 1.0)
+    for (int k = 0; k < nv; k++) x[k] /= lam;
+  for (int k = nvr; k < nv; k++) x[k] *= scal;
+  // SR_pdposv.f90:329-366 : max-change clip, update, f renormalisation
+  double dmax = 0;
+  int knz = 0;
+  for (int k = 0; k < nv; k++) {
+    if (std::fabs(c->delta_tau * x[k]) > dmax) dmax = std::fabs(c->delta_tau * x[k]);
+    if (std::fabs(x[k]) > 1e-12) knz++;
+  }
+  for (int k = 0; k < nv; k++) {
+    if (dmax > c->max_change) gamma[k] = gamma[k] - c->delta_tau * x[k] * (c->max_change / dmax);
+    else gamma[k] = gamma[k] - c->delta_tau * x[k];
+  }
+  double r1 = 0;
+  for (int k = nv - nsl; k < nv; k++) r1 += std::fabs(gamma[k]);
+  r1 /= nsl;
+  for (int k = nv - nsl; k < nv; k++) gamma[k] = gamma[k] * c->slater_abs_av / r1;
+  hipEventRecord(eb, h->st);
+  hipEventSynchronize(eb);
+  float ms = 0;
+  hipEventElapsedTime(&ms, ea, eb);
+  h->t[4] = ms * 1e-3;
+  hipEventDestroy(ea);
+  hipEventDestroy(eb);
+  if (rep) {
+    rep->lambda = lam; rep->delw_max = dmax; rep->scaling = scal; rep->sdiag_av = sav; rep->s_factor = sf;
+    rep->residual = resid; rep->mean_abs_f = r1; rep->k_nonzero = knz; rep->reserved = 0;
+  }
+  rc = rbmpp_set_gamma(h, gamma);
+  h->t[5] = h->t[3] + h->t[4];
+  return rc;
+}
+
+// ---------------------------------------------------------- test entries
+static int upload_x(rbmpp_handle* h, int n, const int32_t* x, signed char** d_out, std::vector<void*>& tmp) {
+  std::vector<signed char> xb((size_t)n * h->N);
+  for (size_t k = 0; k < xb.size(); k++) {
+    if (x[k] != 1 && x[k] != -1) return fail(h, RBMPP_E_ARG, "x must be +-1");
+    xb[k] = (signed char)x[k];
+  }
+  for (int s = 0; s < n; s++) {
+    int up = 0;
+    for (int i = 0; i < h->N; i++) up += x[(size_t)s * h->N + i] == 1;
+    if (up != h->npair) return fail(h, RBMPP_E_ARG, "configuration not in the Sz=0 sector");
+  }
+  signed char* d = nullptr;
+  HIPCHK(h, hipMalloc((void**)&d, xb.size()));
+  tmp.push_back(d);
+  HIPCHK(h, hipMemcpyAsync(d, xb.data(), xb.size(), hipMemcpyHostToDevice, h->st));
+  *d_out = d;
+  return 0;
+}
+
+static void free_tmp(std::vector<void*>& tmp) {
+  for (void* p : tmp) hipFree(p);
+  tmp.clear();
+}
+
+int rbmpp_set_samples(rbmpp_handle* h, int32_t n, const int32_t* x) {
+  if (!h || !x || n <= 0) return RBMPP_E_ARG;
+  if ((int64_t)n > (int64_t)h->W * std::max(h->nspw, 1)) return fail(h, RBMPP_E_ARG, "too many samples");
+  std::vector<signed char> xb((size_t)n * h->N);
+  for (size_t k = 0; k < xb.size(); k++) {
+    if (x[k] != 1 && x[k] != -1) return fail(h, RBMPP_E_ARG, "x must be +-1");
+    xb[k] = (signed char)x[k];
+  }
+  HIPCHK(h, hipMemcpyAsync(h->d_xs, xb.data(), xb.size(), hipMemcpyHostToDevice, h->st));
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  h->n_stored = n;
+  h->measured = 0;
+  return RBMPP_OK;
+}
+
+int64_t rbmpp_n_samples(const rbmpp_handle* h) { return h ? h->n_stored : 0; }
+
+int rbmpp_get_samples(const rbmpp_handle* h0, int32_t* x, double* psi) {
+  rbmpp_handle* h = const_cast<rbmpp_handle*>(h0);
+  if (!h) return RBMPP_E_ARG;
+  const size_t n = (size_t)h->n_stored;
+  if (x) {
+    std::vector<signed char> xb(n * h->N);
+    HIPCHK(h, hipMemcpy(xb.data(), h->d_xs, xb.size(), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < xb.size(); k++) x[k] = xb[k];
+  }
+  if (psi) HIPCHK(h, hipMemcpy(psi, h->d_psis, sizeof(double2) * n, hipMemcpyDeviceToHost));
+  return RBMPP_OK;
+}
+
+int rbmpp_get_walkers(const rbmpp_handle* h0, int32_t* x, double* pf, double* psi, double* invm,
+                      int64_t* naccept_per_walker) {
+  rbmpp_handle* h = const_cast<rbmpp_handle*>(h0);
+  if (!h) return RBMPP_E_ARG;
+  const size_t W = h->W, N = h->N, C = h->C;
+  if (x) {
+    std::vector<signed char> xb(W * N);
+    HIPCHK(h, hipMemcpy(xb.data(), h->ws.x, xb.size(), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < xb.size(); k++) x[k] = xb[k];
+  }
+  if (pf) HIPCHK(h, hipMemcpy(pf, h->ws.pf, sizeof(double) * W * C, hipMemcpyDeviceToHost));
+  if (psi) HIPCHK(h, hipMemcpy(psi, h->ws.psi, sizeof(double2) * W, hipMemcpyDeviceToHost));
+  if (invm) {
+    // device row-major M[r][c] = InvM(r,c) -> reference layout InvM(i,j) at j*N+i
+    std::vector<double> m(W * C * N * N);
+    HIPCHK(h, hipMemcpy(m.data(), h->ws.minv, sizeof(double) * m.size(), hipMemcpyDeviceToHost));
+    for (size_t b = 0; b < W * C; b++)
+      for (size_t r = 0; r < N; r++)
+        for (size_t c2 = 0; c2 < N; c2++) invm[b * N * N + c2 * N + r] = m[b * N * N + r * N + c2];
+  }
+  if (naccept_per_walker) {
+    std::vector<long long> v(W);
+    HIPCHK(h, hipMemcpy(v.data(), h->ws.naccept, sizeof(long long) * W, hipMemcpyDeviceToHost));
+    for (size_t w = 0; w < W; w++) naccept_per_walker[w] = v[w];
+  }
+  return RBMPP_OK;
+}
+
+int rbmpp_eval(rbmpp_handle* h, int32_t n, const int32_t* x, double* psi, double* pf, double* invm) {
+  if (!h || !x || n <= 0) return RBMPP_E_ARG;
+  std::vector<void*> tmp;
+  signed char* d_x = nullptr;
+  int rc = upload_x(h, n, x, &d_x, tmp);
+  double *d_pf = nullptr, *d_inv = nullptr;
+  double2* d_pq = nullptr;
+  const size_t C = h->C, NN = (size_t)h->N * h->N;
+  if (!rc && hipMalloc((void**)&d_pf, sizeof(double) * n * C) == hipSuccess) tmp.push_back(d_pf);
+  if (!rc && hipMalloc((void**)&d_pq, sizeof(double2) * n * C) == hipSuccess) tmp.push_back(d_pq);
+  if (!rc && invm && hipMalloc((void**)&d_inv, sizeof(double) * n * C * NN) == hipSuccess) tmp.push_back(d_inv);
+  if (!rc) rc = run_local(h, n, d_x, invm ? 4 : 0, -1.0, nullptr, nullptr, d_inv, d_pf, d_pq, nullptr, nullptr);
+  if (!rc) {
+    std::vector<double2> p0(n), pq(n * C);
+    hipMemcpy(p0.data(), h->d_mpsi, sizeof(double2) * n, hipMemcpyDeviceToHost);
+    hipMemcpy(pq.data(), d_pq, sizeof(double2) * n * C, hipMemcpyDeviceToHost);
+    if (psi)
+      for (int s = 0; s < n; s++) {
+        psi[(size_t)s * 2 * (C + 1)] = p0[s].x;
+        psi[(size_t)s * 2 * (C + 1) + 1] = p0[s].y;
+        for (size_t q = 0; q < C; q++) {
+          psi[(size_t)s * 2 * (C + 1) + 2 * (q + 1)] = pq[s * C + q].x;
+          psi[(size_t)s * 2 * (C + 1) + 2 * (q + 1) + 1] = pq[s * C + q].y;
+        }
+      }
+    if (pf) hipMemcpy(pf, d_pf, sizeof(double) * n * C, hipMemcpyDeviceToHost);
+    if (invm) hipMemcpy(invm, d_inv, sizeof(double) * n * C * NN, hipMemcpyDeviceToHost);
+  }
+  free_tmp(tmp);
+  return rc;
+}
+
+int rbmpp_ratio_exchange(rbmpp_handle* h, int32_t n, const int32_t* x, const int32_t* i1, const int32_t* i2,
+                         double* pf_new, double* psi_new) {
+  if (!h || !x || !i1 || !i2 || n <= 0) return RBMPP_E_ARG;
+  for (int s = 0; s < n; s++) {
+    if (i1[s] < 0 || i1[s] >= h->N || i2[s] < 0 || i2[s] >= h->N)
+      return fail(h, RBMPP_E_ARG, "exchange site out of range");
+    if (x[(size_t)s * h->N + i1[s]] == x[(size_t)s * h->N + i2[s]])
+      return fail(h, RBMPP_E_ARG, "exchange sites must carry opposite spins");
+  }
+  std::vector<void*> tmp;
+  signed char* d_x = nullptr;
+  int rc = upload_x(h, n, x, &d_x, tmp);
+  int *d1 = nullptr, *d2 = nullptr;
+  double* d_pf = nullptr;
+  double2* d_ps = nullptr;
+  const size_t C = h->C;
+  if (!rc) {
+    hipMalloc((void**)&d1, sizeof(int) * n); tmp.push_back(d1);
+    hipMalloc((void**)&d2, sizeof(int) * n); tmp.push_back(d2);
+    hipMalloc((void**)&d_pf, sizeof(double) * n * C); tmp.push_back(d_pf);
+    hipMalloc((void**)&d_ps, sizeof(double2) * n); tmp.push_back(d_ps);
+    hipMemcpyAsync(d1, i1, sizeof(int) * n, hipMemcpyHostToDevice, h->st);
+    hipMemcpyAsync(d2, i2, sizeof(int) * n, hipMemcpyHostToDevice, h->st);
+    rc = run_local(h, n, d_x, 8, -1.0, d1, d2, nullptr, nullptr, nullptr, d_ps, d_pf);
+  }
+  if (!rc) {
+    if (pf_new) hipMemcpy(pf_new, d_pf, sizeof(double) * n * C, hipMemcpyDeviceToHost);
+    if (psi_new) hipMemcpy(psi_new, d_ps, sizeof(double2) * n, hipMemcpyDeviceToHost);
+  }
+  free_tmp(tmp);
+  return rc;
+}
+
+int rbmpp_local(rbmpp_handle* h, int32_t n, const int32_t* x, double* eloc, double* psi, double* oloc) {
+  if (!h || !x || n <= 0) return RBMPP_E_ARG;
+  // the complex O layout is needed for the API even for real models
+  const int saved = h->real_only;
+  std::vector<void*> tmp;
+  signed char* d_x = nullptr;
+  int rc = upload_x(h, n, x, &d_x, tmp);
+  if (!rc && oloc && saved && h->meas_cap < (size_t)2 * n) {
+    // make room for 2 rows per configuration
+    rc = ensure_measure(h, (size_t)2 * n);
+  }
+  if (!rc) {
+    h->real_only = oloc ? 0 : saved;
+    rc = run_local(h, n, d_x, 1 | (oloc ? 2 : 0), -1.0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                   nullptr);
+    h->real_only = saved;
+  }
+  if (!rc) {
+    const size_t nv = h->nv;
+    if (eloc) hipMemcpy(eloc, h->d_eloc, sizeof(double2) * n, hipMemcpyDeviceToHost);
+    if (psi) hipMemcpy(psi, h->d_mpsi, sizeof(double2) * n, hipMemcpyDeviceToHost);
+    if (oloc) {
+      std::vector<double> rows(2 * nv * n);
+      hipMemcpy(rows.data(), h->d_orows, sizeof(double) * rows.size(), hipMemcpyDeviceToHost);
+      for (int s = 0; s < n; s++)
+        for (size_t k = 0; k < nv; k++) {
+          oloc[((size_t)s * nv + k) * 2] = rows[(2 * (size_t)s) * nv + k];
+          oloc[((size_t)s * nv + k) * 2 + 1] = rows[(2 * (size_t)s + 1) * nv + k];
+        }
+    }
+  }
+  h->measured = 0;
+  free_tmp(tmp);
+  return rc;
+}
+
+int rbmpp_set_s_precision(rbmpp_handle* h, int32_t s_precision) {
+  if (!h || s_precision < 0 || s_precision > 2) return RBMPP_E_ARG;
+  h->cfg.s_precision = s_precision;
+  h->s_valid = 0;
+  return RBMPP_OK;
+}
+
+int rbmpp_timings(const rbmpp_handle* h, double* t) {
+  if (!h || !t) return RBMPP_E_ARG;
+  for (int k = 0; k < 8; k++) t[k] = h->t[k];
+  return RBMPP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,267 @@
+// Device-side building blocks shared by the sampler, estimator and SR
+// kernels: model descriptor, Philox stream, amplitude of one projection
+// copy, and the workgroup-cooperative Pfaffian + inverse of a skew-symmetric
+// matrix.  gfx950 (CDNA4), wave64.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define RBMPP_WAVE 64
+
+struct DevModel {
+  int L, N, P, C, ac, ar, npair, nsl, nv, nv_rbm, mc, mr;
+  int real_only;  // alphac == 0 and all para_mp_trans real: psi and O are real
+  double J1, J2;
+  double sp_cs[2], sp_cc[2], sp_ss[2], sp_w[2];
+  const int* wmap;            // [P][N]   image of site i under operation g
+  const int* invmap;          // [P][N]   inverse permutation
+  const int* orb;             // [N][N]   orbital index of f_ij
+  const unsigned char* subB;  // [N]
+  const double2* chi;         // [P]     para_mp_trans
+  const double* se;           // [C][2N][2N]  se[q][c][r] = Slater_Elm(r,c,q)
+  const double* Wr;           // [ar][N]
+  const double* br;           // [ar]
+  const double2* Wc;          // [ac][N]
+  const double2* bc;          // [ac]
+};
+
+// ---------------------------------------------------------------- complex
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double cabs2(double2 a) { return hypot(a.x, a.y); }
+
+// ------------------------------------------------------------- Philox
+// Philox4x32-10 (Salmon et al., SC'11): counter (ctr_lo, ctr_hi, stream, 0),
+// key (seed_lo, seed_hi).  One double per counter value, 53 random bits.
+__device__ __forceinline__ double philox_u01(uint32_t k0, uint32_t k1, uint32_t stream, uint64_t ctr) {
+  uint32_t c0 = (uint32_t)ctr, c1 = (uint32_t)(ctr >> 32), c2 = stream, c3 = 0u;
+#pragma unroll
+  for (int r = 0; r < 10; r++) {
+    if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+    uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+    uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    c1 = (uint32_t)p1; c3 = (uint32_t)p0; c0 = n0; c2 = n2;
+  }
+  uint64_t z = ((uint64_t)c1 << 32) | c0;
+  return (double)(z >> 11) * 0x1.0p-53;
+}
+
+struct Rng {
+  uint32_t k0, k1, stream;
+  uint64_t ctr;
+  __device__ __forceinline__ double u01() { return philox_u01(k0, k1, stream, ctr++); }
+  // make_sample.f90:254  nran(i) = mod(int(dble(i)*grnd()),i)+1, here 0-based
+  __device__ __forceinline__ int nran0(int n) { return ((int)((double)n * u01())) % n; }
+};
+
+// ----------------------------------------------------------- reductions
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, RBMPP_WAVE);
+  return v;
+}
+
+// amplitude of projection copy q without the Pfaffian (sub.f90:30-48), from
+// angle arrays thc[q*ac+f], thr[q*ar+f] and the sign of operation g.
+__device__ __forceinline__ double2 psi_wo_pf(const DevModel& m, int q, const double2* thc,
+                                             const double* thr, int sgn) {
+  int g = q >> 1, sidx = q & 1;
+  double2 c = make_double2(1.0, 0.0);
+  for (int k = 0; k < m.ac; k++) c = cscale(c, 0.5);
+  for (int f = 0; f < m.ac; f++) {
+    double2 t = thc[q * m.ac + f];
+    double sn, cs, sh = sinh(t.x), ch = cosh(t.x);
+    sincos(t.y, &sn, &cs);
+    c = cmul(c, make_double2(ch * cs, sh * sn));
+  }
+  double r1 = 1.0;
+  for (int f = 0; f < m.ar; f++) r1 = r1 * cosh(thr[q * m.ar + f]);
+  c = cscale(c, r1);
+  c = cscale(c, (double)sgn);
+  c = cmul(c, m.chi[g]);
+  c = cscale(c, m.sp_w[sidx]);
+  return c;
+}
+
+// Same, for the configuration after exchanging the spins at sites i1, i2
+// (update_theta, RBM.f90:95-148): angles are updated on the fly.
+__device__ __forceinline__ double2 psi_wo_pf_exchanged(const DevModel& m, int q, const double2* thc,
+                                                       const double* thr, int sgn_new, int i1, int i2,
+                                                       int x1, int x2) {
+  int g = q >> 1, sidx = q & 1;
+  double s = sidx ? -1.0 : 1.0;
+  int iw1 = m.wmap[g * m.N + i1], iw2 = m.wmap[g * m.N + i2];
+  double2 c = make_double2(1.0, 0.0);
+  for (int k = 0; k < m.ac; k++) c = cscale(c, 0.5);
+  for (int f = 0; f < m.ac; f++) {
+    double2 t = thc[q * m.ac + f];
+    double2 w1 = m.Wc[f * m.N + iw1], w2 = m.Wc[f * m.N + iw2];
+    double a1 = 2.0 * (double)x1 * s, a2 = 2.0 * (double)x2 * s;
+    t.x = t.x - w1.x * a1; t.y = t.y - w1.y * a1;
+    t.x = t.x - w2.x * a2; t.y = t.y - w2.y * a2;
+    double sn, cs, sh = sinh(t.x), ch = cosh(t.x);
+    sincos(t.y, &sn, &cs);
+    c = cmul(c, make_double2(ch * cs, sh * sn));
+  }
+  double r1 = 1.0;
+  for (int f = 0; f < m.ar; f++) {
+    double t = thr[q * m.ar + f];
+    t = t - 2.0 * m.Wr[f * m.N + iw1] * (double)x1 * s;
+    t = t - 2.0 * m.Wr[f * m.N + iw2] * (double)x2 * s;
+    r1 = r1 * cosh(t);
+  }
+  c = cscale(c, r1);
+  c = cscale(c, (double)sgn_new);
+  c = cmul(c, m.chi[g]);
+  c = cscale(c, m.sp_w[sidx]);
+  return c;
+}
+
+// --------------------------------------------------------------------
+// Workgroup-cooperative Pfaffian and inverse of a skew-symmetric n x n
+// matrix A (row-major, leading dimension ld; LDS or global).  Gauss-Jordan
+// sweeps over 2x2 pivot blocks with symmetric pivoting on the largest
+// |A(k,j)|, j>k (the Parlett-Reid pivot rule of dskpfa('U','P'),
+// matrix.f90:133-151).  The Schur-complement update of each sweep is the
+// Parlett-Reid step, so Pf = (-1)^{#swaps} prod_k A(k,k+1); the completed
+// sweeps leave the inverse of the permuted matrix in A.
+//   perm[n] : returns the symmetric permutation (row r of A <-> original perm[r])
+//   buf[4*ld] : workspace; red[2*nwaves]: reduction scratch
+// Returns 0 when a pivot block is exactly singular (Pf = 0).
+// Must be called by all threads of the workgroup.
+__device__ __forceinline__ double wg_skew_inverse(double* A, int n, int ld, int* perm, double* buf,
+                                                  double* red_v, int* red_i, int* s_piv) {
+  const int tid = threadIdx.x, nth = blockDim.x;
+  const int lane = tid & (RBMPP_WAVE - 1), wid = tid / RBMPP_WAVE;
+  double* R0 = buf;
+  double* R1 = buf + ld;
+  double* C0 = buf + 2 * ld;
+  double* C1 = buf + 3 * ld;
+  for (int i = tid; i < n; i += nth) perm[i] = i;
+  double pf = 1.0;
+  __syncthreads();
+  for (int k = 0; k < n - 1; k += 2) {
+    // pivot search over row k, columns k+1..n-1 (first index of the maximum)
+    if (wid == 0) {
+      double bv = -1.0;
+      int bi = n;
+      for (int j = k + 1 + lane; j < n; j += RBMPP_WAVE) {
+        double v = fabs(A[(size_t)k * ld + j]);
+        if (v > bv) { bv = v; bi = j; }
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        double ov = __shfl_xor(bv, o, RBMPP_WAVE);
+        int oi = __shfl_xor(bi, o, RBMPP_WAVE);
+        if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+      }
+      if (lane == 0) *s_piv = bi;
+    }
+    __syncthreads();
+    const int p = *s_piv;
+    if (p != k + 1) {
+      for (int j = tid; j < n; j += nth) {
+        double t = A[(size_t)(k + 1) * ld + j];
+        A[(size_t)(k + 1) * ld + j] = A[(size_t)p * ld + j];
+        A[(size_t)p * ld + j] = t;
+      }
+      __syncthreads();
+      for (int i = tid; i < n; i += nth) {
+        double t = A[(size_t)i * ld + k + 1];
+        A[(size_t)i * ld + k + 1] = A[(size_t)i * ld + p];
+        A[(size_t)i * ld + p] = t;
+      }
+      if (tid == 0) { int t = perm[k + 1]; perm[k + 1] = perm[p]; perm[p] = t; }
+      pf = -pf;
+    }
+    __syncthreads();
+    const double e01 = A[(size_t)k * ld + k + 1];
+    const double e10 = A[(size_t)(k + 1) * ld + k];
+    if (e01 == 0.0 || e10 == 0.0) return 0.0;
+    pf *= e01;
+    const double i01 = 1.0 / e01, i10 = 1.0 / e10;
+    for (int j = tid; j < n; j += nth) {
+      R0[j] = A[(size_t)k * ld + j];
+      R1[j] = A[(size_t)(k + 1) * ld + j];
+      C0[j] = A[(size_t)j * ld + k];
+      C1[j] = A[(size_t)j * ld + k + 1];
+    }
+    __syncthreads();
+    // new pivot rows: row k <- R1/e10, row k+1 <- R0/e01; new pivot columns:
+    // col k <- -C1/e01, col k+1 <- -C0/e10; pivot block <- E^-1; the rest
+    // A_ij -= C0_i * R1_j / e10 + C1_i * R0_j / e01.
+    const int nn = n * n;
+    for (int e = tid; e < nn; e += nth) {
+      int i = e / n, j = e - i * n;
+      bool ik = (i == k) || (i == k + 1), jk = (j == k) || (j == k + 1);
+      double v;
+      if (ik && jk) {
+        v = (i == j) ? 0.0 : ((i == k) ? i10 : i01);
+      } else if (ik) {
+        v = (i == k) ? R1[j] * i10 : R0[j] * i01;
+      } else if (jk) {
+        v = (j == k) ? -C1[i] * i01 : -C0[i] * i10;
+      } else {
+        v = A[(size_t)i * ld + j] - C0[i] * (R1[j] * i10) - C1[i] * (R0[j] * i01);
+      }
+      A[(size_t)i * ld + j] = v;
+    }
+    __syncthreads();
+  }
+  (void)red_v; (void)red_i;
+  return pf;
+}
+
+// calc_theta (RBM.f90:31-91) and sign_PP (interface_for_PP.f90:71-106) of the
+// configuration s.x with electron order s.eidx.  With check=1 the results are
+// compared with the incrementally maintained values (make_sample.f90:84-92)
+// and s.status is set on a mismatch.  S is any shared-memory block with
+// members x, eidx, thr, thc, sgn, sgnbits, status.
+template <class S>
+__device__ void wg_theta_sign(const DevModel& m, S& s, int check) {
+  const int tid = threadIdx.x, nth = blockDim.x;
+  const int N = m.N, C = m.C, P = m.P;
+  for (int idx = tid; idx < C * m.ar; idx += nth) {
+    const int q = idx / m.ar, f = idx - q * m.ar, g = q >> 1;
+    const double sp = (q & 1) ? -1.0 : 1.0;
+    double t = m.br[f];
+    for (int i = 0; i < N; i++) t = t + m.Wr[f * N + m.wmap[g * N + i]] * (double)s.x[i] * sp;
+    if (check && fabs(t - s.thr[idx]) > 1.0e-5) s.status = -40;
+    s.thr[idx] = t;
+  }
+  for (int idx = tid; idx < C * m.ac; idx += nth) {
+    const int q = idx / m.ac, f = idx - q * m.ac, g = q >> 1;
+    const double sp = (q & 1) ? -1.0 : 1.0;
+    double2 t = m.bc[f];
+    for (int i = 0; i < N; i++) {
+      const double2 w = m.Wc[f * N + m.wmap[g * N + i]];
+      const double a = (double)s.x[i] * sp;
+      t.x = t.x + w.x * a;
+      t.y = t.y + w.y * a;
+    }
+    if (check && cabs2(make_double2(t.x - s.thc[idx].x, t.y - s.thc[idx].y)) > 1.0e-5) s.status = -40;
+    s.thc[idx] = t;
+  }
+  for (int g = tid; g < P; g += nth) s.sgnbits[g] = 0u;
+  __syncthreads();
+  for (int e = tid; e < P * N; e += nth) {
+    const int g = e / N, i1 = e - g * N;
+    if (i1 == 0) continue;
+    const int t1 = m.wmap[g * N + s.eidx[i1]];
+    unsigned par = (i1 >= m.npair && m.subB[t1]) ? 1u : 0u;
+    for (int i2 = 0; i2 < i1; i2++) par ^= (m.wmap[g * N + s.eidx[i2]] > t1) ? 1u : 0u;
+    if (par) atomicXor(&s.sgnbits[g], 1u);
+  }
+  __syncthreads();
+  for (int g = tid; g < P; g += nth) {
+    const int sg = (s.sgnbits[g] & 1u) ? -1 : 1;
+    if (check && sg != s.sgn[g]) s.status = -42;
+    s.sgn[g] = sg;
+  }
+  __syncthreads();
+}

@@ -1,0 +1,76 @@
+// Kernel argument blocks and launch helpers shared by the host C-ABI and the
+// kernel translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rbmpp_device.h"
+
+#define RBMPP_CMAX 16   // projection copies per walker held in LDS (2 x 8 operations)
+#define RBMPP_MRMAX 512 // real RBM angles per walker
+#define RBMPP_MCMAX 256 // complex RBM angles per walker
+#define RBMPP_NLDS 112  // largest N whose N x N recompute matrix is staged in LDS
+
+// walker status codes (mirrors RBMPP_E_* of rbmpp.h)
+#define WS_OK 0
+#define WS_SINGULAR (-10)
+#define WS_CANDIDATE (-30)
+#define WS_INIT (-21)
+#define WS_THETA (-40)
+#define WS_SIGN (-42)
+
+struct WalkerState {
+  signed char* x;          // [W][N]
+  int* eidx;               // [W][N]  Ele_Idx (0-based sites)
+  int* ecfg;               // [W][N]  Ele_Cfg
+  double* thr;             // [W][Mr]
+  double2* thc;            // [W][Mc]
+  double* pf;              // [W][C]
+  int* sgn;                // [W][P]
+  double2* psi;            // [W]
+  double* minv;            // [W][C][N][N]  minv[w][q][r*N+c] = InvM(r,c) of copy q
+  double* scratch;         // [W][N][N]     recompute matrix when N > RBMPP_NLDS
+  int* nacc_tmp;           // [W] accepts since the last recompute
+  int* nfastup;            // [W] adaptive recompute period
+  unsigned long long* ctr; // [W] Philox counter
+  int* status;             // [W]
+  long long* naccept;      // [W]
+  int* nrecompute;         // [W]
+};
+
+struct SweepArgs {
+  DevModel m;
+  WalkerState ws;
+  int nwalk, nsamp, nwarm, nupdate, fixed_k, first;
+  double shift, p_restart, p_global;
+  uint32_t key0, key1;
+  signed char* xs;  // [W*nsamp][N] stored samples
+  double2* psis;    // [W*nsamp]
+};
+
+// per-configuration evaluation (eval / local energy / O_k / measurement)
+struct LocalArgs {
+  DevModel m;
+  int nconf;
+  const signed char* xs;  // [nconf][N]
+  int mode;               // bit0: eloc, bit1: O_k, bit2: write invm, bit3: custom exchanges
+  double shift;           // reweighting offset; <0: no reweighting (raw d psi, sum H psi)
+  const int* ex_i1;       // [nconf] custom exchange (mode bit3)
+  const int* ex_i2;
+  double* scratch;        // [grid][N][N] recompute matrix when N > RBMPP_NLDS
+  // outputs
+  double2* psi;           // [nconf]
+  double* pf;             // [nconf][C]
+  double2* psi_q;         // [nconf][C] (optional)
+  double* invm;           // [nconf][C][N][N] reference layout InvM(i,j) at j*N+i (optional)
+  double2* eloc;          // [nconf]
+  double* orows;          // O rows: real models [nconf][nv]; complex [2*nconf][nv] (re, im rows)
+  double* rw;             // [nconf] reweight factor 1/(|psi|+shift)
+  double2* ex_psi;        // [nconf] amplitude after the custom exchange
+  double* ex_pf;          // [nconf][C]
+  int* status;            // [nconf]
+};
+
+void launch_slater(const DevModel& m, const double* f, double* se, int* bad, hipStream_t s);
+int launch_sweep(const SweepArgs& a, hipStream_t s);
+int launch_local(const LocalArgs& a, int grid, hipStream_t s);

@@ -1,0 +1,198 @@
+"""Parity of the HIP path (librbmpp.so through the C ABI) with the CPU oracle
+on identical seeded inputs.  FP64 paths: relative 1e-10 (normwise where a
+whole matrix is compared, scaled by the condition of the copy where noted);
+sampler trajectories: bit-exact configurations."""
+import numpy as np
+import pytest
+
+from rbm_pp_amd import lattice as lt
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-10
+
+
+def _api():
+    from rbm_pp_amd import api
+    return api
+
+
+def _model(L, alphar, alphac=0, J2=0.5, seed=1, sigma_w=0.05):
+    spec = lt.j1j2_model(L, alphar=alphar, alphac=alphac, J2=J2)
+    return spec, lt.random_params(spec, seed_w=seed, seed_f=seed + 100, sigma_w=sigma_w)
+
+
+def _configs(N, n, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        x = np.array([1] * (N // 2) + [-1] * (N // 2), np.int32)
+        rng.shuffle(x)
+        out.append(x)
+    return np.array(out)
+
+
+def _relerr(a, b):
+    return np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-300)
+
+
+CASES = [(4, 1, 0), (4, 2, 1), (6, 2, 0), (10, 4, 0)]
+
+
+@pytest.mark.parametrize("L,ar,ac", CASES)
+def test_eval_psi_pf_invm(oracle_lib, L, ar, ac):
+    api = _api()
+    spec, p = _model(L, ar, ac)
+    orc = oracle_lib.Oracle(spec, p)
+    eng = api.Engine(spec, n_walkers=1, n_samples=1, params=p)
+    xs = _configs(spec.N, 6, seed=L)
+    got = eng.eval(xs, with_invm=True)
+    for s, x in enumerate(xs):
+        st = orc.eval(x)
+        assert _relerr(got["psi"][s], st["psi"]) < RTOL
+        assert _relerr(got["pf"][s], st["pf"]) < RTOL
+        for q in range(spec.C):
+            inv_ref = st["invm"][q]
+            cond = np.linalg.cond(inv_ref)
+            assert _relerr(got["invm"][s, q], inv_ref) < max(RTOL, 1e-15 * cond), (s, q, cond)
+
+
+@pytest.mark.parametrize("L,ar,ac", CASES)
+def test_ratio_exchange(oracle_lib, L, ar, ac):
+    api = _api()
+    spec, p = _model(L, ar, ac)
+    orc = oracle_lib.Oracle(spec, p)
+    eng = api.Engine(spec, n_walkers=1, n_samples=1, params=p)
+    xs = _configs(spec.N, 8, seed=100 + L)
+    rng = np.random.default_rng(L)
+    i1 = np.array([int(rng.integers(spec.N)) for _ in xs], np.int32)
+    i2 = np.array([int(rng.choice(np.nonzero(x != x[a])[0])) for x, a in zip(xs, i1)], np.int32)
+    pf, psi = eng.ratio_exchange(xs, i1, i2)
+    for s, x in enumerate(xs):
+        orc.eval(x)
+        _, pfn, psin = orc.ratio_exchange(int(i1[s]), int(i2[s]))
+        assert _relerr(pf[s], pfn) < RTOL
+        assert abs(psi[s] - psin[0]) <= RTOL * abs(psin[0])
+
+
+@pytest.mark.parametrize("L,ar,ac", CASES)
+def test_local_energy_and_ok(oracle_lib, L, ar, ac):
+    api = _api()
+    spec, p = _model(L, ar, ac)
+    orc = oracle_lib.Oracle(spec, p)
+    eng = api.Engine(spec, n_walkers=1, n_samples=1, params=p)
+    xs = _configs(spec.N, 6, seed=200 + L)
+    got = eng.local(xs, want_o=True)
+    for s, x in enumerate(xs):
+        orc.eval(x)
+        e = orc.eloc()
+        o = orc.oloc()
+        assert abs(got["eloc"][s] - e) <= RTOL * abs(e)
+        assert _relerr(got["oloc"][s], o) < RTOL
+
+
+@pytest.mark.parametrize("L,ar,ac,W,ns,K", [(4, 1, 0, 4, 6, 8), (6, 2, 1, 3, 4, 20), (10, 4, 0, 2, 3, 32)])
+def test_sampler_trajectory_bit_exact(oracle_lib, L, ar, ac, W, ns, K):
+    """GPU walkers and the oracle chain driven by the same Philox streams
+    visit the same configurations; amplitudes agree to FP64 rounding."""
+    api = _api()
+    spec, p = _model(L, ar, ac, sigma_w=0.1)
+    seed = 1234
+    nwarm = 4
+    eng = api.Engine(spec, n_walkers=W, n_samples=ns, seed=seed, n_warmup=nwarm, recompute_k=K,
+                     params=p)
+    eng.make_sample(True, shift=0.0)
+    x_gpu, psi_gpu = eng.get_samples()
+    wk = eng.get_walkers()
+    orc = oracle_lib.Oracle(spec, p)
+    states = []
+    for w in range(W):
+        rng = oracle_lib.Rng("philox", seed, stream=w)
+        st = orc.new_state()
+        out = orc.make_sample(rng, True, nsample=ns, nwarmup=nwarm, fixed_k=K, ref_draws=False,
+                              state=st)
+        states.append((rng, st))
+        assert np.array_equal(x_gpu[w * ns:(w + 1) * ns], out["x"]), f"walker {w}"
+        np.testing.assert_allclose(psi_gpu[w * ns:(w + 1) * ns], out["psi"], rtol=1e-9)
+        np.testing.assert_allclose(wk["pf"][w], out["pf"][-1], rtol=1e-8)
+        assert wk["naccept"][w] == out["naccept"]
+    # a second call continues every chain from its last sample
+    eng.make_sample(False, shift=1e-3)
+    x2, _ = eng.get_samples()
+    for w in range(W):
+        rng, st = states[w]
+        out = orc.make_sample(rng, False, nsample=ns, nwarmup=nwarm, fixed_k=K, ref_draws=False,
+                              shift=1e-3, state=st)
+        assert np.array_equal(x2[w * ns:(w + 1) * ns], out["x"]), f"walker {w} (second call)"
+
+
+@pytest.mark.parametrize("L,ar,ac", [(4, 2, 1), (6, 2, 0), (10, 4, 0)])
+def test_measure_s_and_sr_step(oracle_lib, L, ar, ac):
+    api = _api()
+    spec, p = _model(L, ar, ac)
+    orc = oracle_lib.Oracle(spec, p)
+    n = 96
+    xs = _configs(spec.N, n, seed=300 + L)
+    eng = api.Engine(spec, n_walkers=1, n_samples=n, params=p)
+    eng.set_samples(xs)
+    shift = 2e-3
+    got = eng.calc_phys_quantity_opt(shift=shift)
+    ref = orc.measure(xs, shift=shift, want_s=True)
+    norm = ref["norm"]
+    assert got["norm"] == pytest.approx(norm, rel=RTOL)
+    assert got["Etot"] == pytest.approx(ref["Etot"] / norm, rel=RTOL)
+    assert got["Etot2"] == pytest.approx(ref["Etot2"] / norm, rel=RTOL)
+    assert got["wf_av"] == pytest.approx(ref["wf_av"] / n, rel=RTOL)
+    assert _relerr(got["gvec"], ref["gvec"] / norm) < RTOL
+    assert _relerr(got["ovec"], ref["ovec"] / norm) < RTOL
+    ob = ref["ovec"] / norm
+    S_ref = ref["smat"] / norm - np.outer(ob, ob)
+    S = eng.smat()
+    assert _relerr(S, S_ref) < 1e-9
+    gam0 = orc.gamma()
+    rc, gam_ref, rep_ref, _ = orc.sr_step(ref["Etot"] / norm, norm, ref["gvec"], ref["ovec"], ref["smat"],
+                                          gam0, s_factor0=1e-3)
+    assert rc == 0
+    gam, rep = eng.sr_step(gam0, s_factor0=1e-3, refine_iters=1)
+    assert rep["residual"] < 1e-12
+    assert rep["lambda_"] == pytest.approx(rep_ref[0], rel=1e-8)
+    assert _relerr(gam - gam0, gam_ref - gam0) < 1e-8
+    np.testing.assert_allclose(eng.gamma(), gam, rtol=0, atol=0)
+
+
+def test_s_build_fp32_error_bound(oracle_lib):
+    """Mixed S: centred in FP64, products in FP32; stated bound 1e-5
+    normwise relative to the FP64 S on this workload."""
+    api = _api()
+    spec, p = _model(6, 2, 0)
+    n = 256
+    xs = _configs(spec.N, n, seed=7)
+    e64 = api.Engine(spec, n_walkers=1, n_samples=n, params=p, s_precision=api.S_FP64)
+    e32 = api.Engine(spec, n_walkers=1, n_samples=n, params=p, s_precision=api.S_FP32)
+    for e in (e64, e32):
+        e.set_samples(xs)
+        e.calc_phys_quantity_opt(shift=1e-3)
+    S64, S32 = e64.smat(), e32.smat()
+    err = np.linalg.norm(S32 - S64) / np.linalg.norm(S64)
+    assert err < 1e-5, err
+
+
+def test_vmc_energy_4x4_against_enumeration(oracle_lib):
+    """GPU sampler + estimator: VMC energy of a fixed 4x4 wave function within
+    5 error bars of its exact expectation (12870-state enumeration)."""
+    import ed
+    api = _api()
+    spec, p = _model(4, 1, 0, sigma_w=0.1)
+    orc = oracle_lib.Oracle(spec, p)
+    states, Hg = ed.hamiltonian(4, 1.0, 0.5, gauge=True)
+    m = orc.measure(states.astype(np.int32), shift=0.0, want_s=False)
+    psi = m["psi"]
+    e_exact = (np.conj(psi) @ (Hg @ psi)).real / (np.abs(psi) ** 2).sum()
+    W, ns = 64, 100
+    eng = api.Engine(spec, n_walkers=W, n_samples=ns, seed=99, n_warmup=20, recompute_k=32, params=p)
+    eng.make_sample(True)
+    r = eng.calc_phys_quantity_opt(shift=0.0)
+    loc = eng.local(eng.get_samples()[0], want_o=False)
+    el = (loc["eloc"] / loc["psi"]).real
+    err = el.reshape(W, ns).mean(1).std() / np.sqrt(W)
+    assert abs(r["Etot"] - e_exact) < 5 * err + 1e-12
