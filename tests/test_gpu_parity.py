@@ -114,7 +114,8 @@ def test_sampler_trajectory_bit_exact(oracle_lib, L, ar, ac, W, ns, K):
         states.append((rng, st))
         assert np.array_equal(x_gpu[w * ns:(w + 1) * ns], out["x"]), f"walker {w}"
         np.testing.assert_allclose(psi_gpu[w * ns:(w + 1) * ns], out["psi"], rtol=1e-9)
-        np.testing.assert_allclose(wk["pf"][w], out["pf"][-1], rtol=1e-8)
+        # after many fast updates: normwise over the copies of the walker
+        assert _relerr(wk["pf"][w], out["pf"][-1]) < 1e-8
         assert wk["naccept"][w] == out["naccept"]
     # a second call continues every chain from its last sample
     eng.make_sample(False, shift=1e-3)
