@@ -70,10 +70,15 @@ def main():
     ap.add_argument("--samples", type=int, default=200000, help="stored samples per SR step")
     ap.add_argument("--therm", type=int, default=200, help="thermalisation sweeps of the first call")
     ap.add_argument("--recompute-k", type=int, default=32)
-    ap.add_argument("--cpu-sweeps", type=int, default=250)
+    ap.add_argument("--cpu-sweeps", type=int, default=600)
     ap.add_argument("--cpu-procs", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64-compare", action="store_true")
+    ap.add_argument("--fused-measure", action="store_true",
+                    help="measure inside the sampler with the fast-updated X^-1 instead of "
+                         "recomputing Pf and X^-1 per sample as the reference does")
+    ap.add_argument("--cond-guard", type=float, default=1e4,
+                    help="recompute after an accept whose Pf ratio of a copy leaves (1/g, g); 0: off")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)",
@@ -100,7 +105,8 @@ def main():
     nspw = math.ceil(args.samples / args.walkers)
     eng = api.Engine(spec, n_walkers=args.walkers, n_samples=nspw, seed=1, n_warmup=-1,
                      n_warmup_first=args.therm, recompute_k=args.recompute_k,
-                     s_precision=api.S_FP32, params=p)
+                     s_precision=api.S_FP32, params=p, fused_measure=args.fused_measure,
+                     cond_guard=args.cond_guard)
     N, Cc = spec.N, spec.C
     log(f"model L={args.L} N={N} copies={Cc} nv={spec.nv} walkers={args.walkers} samples/walker={nspw}")
     t0 = time.perf_counter()
@@ -128,6 +134,7 @@ def main():
         if rec is not None:
             rec.append(dict(wall=t3 - t0, sample=t1 - t0, measure=t2 - t1, sr=t3 - t2,
                             sweep_kernel=tt["sample"], s_build=tt2["s_build"], solve=tt2["solve"],
+                            recomputes=tt["recompute_count"],
                             nacc=nacc, E=r["Etot"], lam=rep["lambda_"]))
         return nacc, r
 
@@ -141,7 +148,8 @@ def main():
         one_step(recs)
         x = recs[-1]
         log(f"step {i}: {x['wall']:.2f} s (sample {x['sample']:.2f}, measure {x['measure']:.2f}, "
-            f"sr {x['sr']:.3f}) E/N={x['E'] / 4 / N:.6f} accept={x['nacc'] / ns_step / N:.3f}")
+            f"sr {x['sr']:.3f}) E/N={x['E'] / 4 / N:.6f} accept={x['nacc'] / ns_step / N:.3f} "
+            f"recomputes/walker={x['recomputes'] / args.walkers:.1f}")
     t_total = time.perf_counter() - t_start
 
     # FP64 S on the last samples, for the precision comparison
@@ -153,6 +161,7 @@ def main():
         eng.set_s_precision(api.S_FP32)
         S32 = eng.smat()
         t32 = eng.timings()["s_build"]
+        log(f"S build: fp64 {t64 * 1e3:.3f} ms, fp32 {t32 * 1e3:.3f} ms")
         fp64 = dict(s_build_fp64_s=t64, s_build_fp32_s=t32,
                     s_rel_err_fp32=float(np.linalg.norm(S32 - S64) / np.linalg.norm(S64)))
 
@@ -161,11 +170,15 @@ def main():
     nwarm = max(30, nspw // 100)
     sweeps_per_launch = args.walkers * (nwarm + 1 + nspw)
     t_sweep = sum(r["sweep_kernel"] for r in recs) / len(recs)
-    # SURVEY 8(d) per-sample bytes of the sampler: 8 C N^3 (1 + 3A)
+    # algorithmic HBM bytes per sweep of one walker: each of the N proposals
+    # reads every copy's X^-1 (8 C N^2 bytes); an accepted one (fraction A)
+    # adds the three passes of calculate_new_InvM_two (pfupdate_two.f90: the
+    # read forming its update vectors, then the read and write of the update):
+    # 8 C N^3 (1 + 3A)
     b_unit = 8.0 * Cc * N ** 3 * (1.0 + 3.0 * acc)
     achieved = b_unit * sweeps_per_launch / t_sweep / 1e9
-    # this kernel's own minimum: one read of every X^-1 per proposal, one
-    # read+write per accepted move
+    # this kernel's own minimum: the update vectors come from the ratio pass,
+    # so an accepted move adds only the read and write of the update
     b_min = 8.0 * Cc * N ** 3 * (1.0 + 2.0 * acc)
     out = {
         "metric": "MC samples/s + SR-step s, 10x10 J1-J2; mixed-prec vs FP64 at equal energy",
@@ -183,7 +196,10 @@ def main():
         "config": {"workload": f"j1j2_{args.L}x{args.L}_J2_{args.J2}_alphar{args.alphar}_c4v_parity",
                    "walkers": args.walkers, "samples_per_sr_step": ns_step,
                    "sweeps_per_walker_per_step": nwarm + 1 + nspw, "recompute_k": args.recompute_k,
-                   "s_precision": "fp32 products, fp64 centring/shift/solve", "parallelism": "single_gpu"},
+                   "s_precision": "fp32 products, fp64 centring/shift/solve",
+                   "measurement": "fused into sampler" if args.fused_measure else "recompute per sample",
+                   "cond_guard": args.cond_guard,
+                   "parallelism": "single_gpu"},
         "sampler_samples_per_s": samples / sum(r["sample"] for r in recs),
         "sr_step_s": sum(r["sr"] for r in recs) / len(recs),
         "measure_s": sum(r["measure"] for r in recs) / len(recs),

@@ -57,10 +57,20 @@ typedef struct {
   int32_t recompute_k;       /* >0: full Pf/X^-1 recompute after recompute_k accepts;
                                 0: adaptive Nfastup in [1,100] (make_sample.f90:74-128) */
   int32_t s_precision;       /* RBMPP_S_* */
-  int32_t reserved;
+  int32_t fused_measure;     /* 1: measure every stored sample inside the sampler with the
+                                walker's fast-updated X^-1 (refreshed every recompute_k
+                                accepts); 0: recompute Pf and X^-1 per sample in
+                                rbmpp_measure, as calc_phys_quantity.f90:85 does.  N <= 104 */
   uint64_t seed;             /* Philox key; walker w uses stream w */
   double p_restart;          /* reference 0.05 (make_sample.f90:61) */
   double p_global;           /* reference 3e-4 (make_sample.f90:135) */
+  double cond_guard;         /* > 0: recompute Pf and X^-1 of the walker right after an
+                                accepted move whose Pf ratio of any projection copy lies
+                                outside (1/cond_guard, cond_guard); 0: never (the
+                                reference).  A copy that passes close to a singular
+                                configuration otherwise leaves the fast-updated Pf and
+                                X^-1 inaccurate (up to overflow) until the next periodic
+                                recompute; not in the reference. */
 } rbmpp_config;
 
 typedef struct {
@@ -141,6 +151,12 @@ int rbmpp_measure_energy(rbmpp_handle* h, double shift, double* etot, double* et
 int rbmpp_set_samples(rbmpp_handle* h, int32_t n, const int32_t* x);
 int rbmpp_get_samples(const rbmpp_handle* h, int32_t* x, double* psi);
 int64_t rbmpp_n_samples(const rbmpp_handle* h);
+/* Per-sample values of the last measurement (calc_phys_quantity_opt,
+ * main.f90:351-367, before the sums): psi[2s+{0,1}], the reweighted local
+ * energy eloc[2s+{0,1}] = E_loc * psi-weight, rw[s] = 1/(|psi|^2 + shift).
+ * Any pointer may be NULL.  Valid after rbmpp_measure / rbmpp_make_sample
+ * with fused measurement. */
+int rbmpp_get_measured(const rbmpp_handle* h, double* psi, double* eloc, double* rw);
 /* Walker state after make_sample: x[w*N+i], pf[w*C+q], psi[2w+{0,1}] and
  * (optional) InvM in the reference's layout invm[w][q][(j)*N+(i)] = InvM(i+1,j+1,q+1). */
 int rbmpp_get_walkers(const rbmpp_handle* h, int32_t* x, double* pf, double* psi, double* invm,

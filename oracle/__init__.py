@@ -122,7 +122,8 @@ def pfaffian(A: np.ndarray) -> float:
 class _SampleCfg(C.Structure):
     _fields_ = [("nsample", C.c_int), ("nwarmup", C.c_int), ("nupdate", C.c_int),
                 ("fixed_k", C.c_int), ("ref_draws", C.c_int), ("p_restart", C.c_double),
-                ("p_global", C.c_double), ("shift", C.c_double)]
+                ("p_global", C.c_double), ("shift", C.c_double),
+                ("cond_guard", C.c_double)]
 
 
 class _SrCfg(C.Structure):
@@ -275,9 +276,9 @@ class Oracle:
     def make_sample(self, rng: Rng, first: bool, nsample: int, nwarmup: int,
                     nupdate: int | None = None, fixed_k: int = 0, ref_draws: bool = True,
                     p_restart: float = 0.05, p_global: float = 3e-4, shift: float = 0.0,
-                    nfastup: int | None = None, state=None):
+                    nfastup: int | None = None, state=None, cond_guard: float = 0.0):
         cfg = _SampleCfg(nsample, nwarmup, self.N if nupdate is None else nupdate, fixed_k,
-                         int(ref_draws), p_restart, p_global, shift)
+                         int(ref_draws), p_restart, p_global, shift, cond_guard)
         nf = C.c_int(min(100, 2 * self.N) if nfastup is None else nfastup)
         xs = np.zeros((nsample, self.N), np.int32)
         psi = np.zeros(2 * nsample); pf = np.zeros(nsample * self.C)

@@ -1080,6 +1080,7 @@ int om_make_sample(const om_model* m, om_state* s, om_rng* r, int first,
   int N = m->N, C = m->C, rc;
   om_rng* rd = c->ref_draws ? r : NULL; /* stream for the amplitude self-check */
   long nacc = 0, nrec = 0;
+  double rt[C]; /* Pf ratio of every copy for the current proposal */
   if (first) {
     rc = make_initial_x(m, s, r, rd);
     if (rc) return rc;
@@ -1139,12 +1140,26 @@ int om_make_sample(const om_model* m, om_state* s, om_rng* r, int first,
       int i1, i2;
       rc = spin_flip_candidate(m, s->x, r, &i1, &i2);
       if (rc) return rc;
-      propose_exchange(m, s, i1, i2, NULL, rd);
+      propose_exchange(m, s, i1, i2, rt, rd);
       double r1 = (cabs(s->psi2[0]) + c->shift) / (cabs(s->psi[0]) + c->shift);
       if (om_rng_u01(r) < r1 * r1) {
         if (i0 > 0) nacc++;
         nacc_tmp++;
         accept_exchange(m, s, i1, i2);
+        if (c->cond_guard > 0.0) {
+          int hit = 0;
+          for (int q = 0; q < C; q++) {
+            const double a = fabs(rt[q]);
+            if (!(a * c->cond_guard > 1.0 && a < c->cond_guard)) hit = 1;
+          }
+          if (hit) {
+            double dpf, dinv;
+            rc = recompute(m, s, NULL, &dpf, &dinv);
+            if (rc) return rc;
+            nrec++;
+            nacc_tmp = 0;
+          }
+        }
       }
     }
     if (i0 > 0) {

@@ -94,6 +94,9 @@ typedef struct {
   double p_restart; /* reference 0.05 */
   double p_global;  /* reference 3e-4 */
   double shift;
+  double cond_guard; /* > 0: recompute right after an accepted move whose Pf
+                        ratio of any copy lies outside (1/cond_guard,
+                        cond_guard) (not in the reference: 0 restates it) */
 } om_sample_cfg;
 int om_make_sample(const om_model* m, om_state* s, om_rng* r, int first,
                    const om_sample_cfg* c, int* nfastup, int* x_out, double* psi_out,

@@ -36,8 +36,9 @@ class Config(C.Structure):
                 ("n_slater", C.c_int32), ("spin_parity", C.c_int32), ("J1", C.c_double),
                 ("J2", C.c_double), ("n_walkers", C.c_int32), ("n_samples", C.c_int32),
                 ("n_warmup", C.c_int32), ("n_warmup_first", C.c_int32), ("n_update", C.c_int32),
-                ("recompute_k", C.c_int32), ("s_precision", C.c_int32), ("reserved", C.c_int32),
-                ("seed", C.c_uint64), ("p_restart", C.c_double), ("p_global", C.c_double)]
+                ("recompute_k", C.c_int32), ("s_precision", C.c_int32), ("fused_measure", C.c_int32),
+                ("seed", C.c_uint64), ("p_restart", C.c_double), ("p_global", C.c_double),
+                ("cond_guard", C.c_double)]
 
 
 class SrConfig(C.Structure):
@@ -74,6 +75,7 @@ def lib():
         L.rbmpp_measure_energy.argtypes = [vp, C.c_double] + [dp] * 4
         L.rbmpp_set_samples.argtypes = [vp, C.c_int32, ip]
         L.rbmpp_get_samples.argtypes = [vp, ip, dp]
+        L.rbmpp_get_measured.argtypes = [vp, dp, dp, dp]
         L.rbmpp_n_samples.restype = C.c_int64; L.rbmpp_n_samples.argtypes = [vp]
         L.rbmpp_get_walkers.argtypes = [vp, ip, dp, dp, dp, C.POINTER(C.c_int64)]
         L.rbmpp_eval.argtypes = [vp, C.c_int32, ip, dp, dp, dp]
@@ -100,12 +102,14 @@ class Engine:
     def __init__(self, spec: lt.ModelSpec, n_walkers: int, n_samples: int, seed: int = 1,
                  n_warmup: int = -1, n_warmup_first: int = -1, n_update: int = 0,
                  recompute_k: int = 0, s_precision: int = S_FP64, p_restart: float = 0.05,
-                 p_global: float = 3e-4, params: lt.Params | None = None):
+                 p_global: float = 3e-4, params: lt.Params | None = None,
+                 fused_measure: bool = False, cond_guard: float = 1e4):
         self.spec = spec
         N, P = spec.N, spec.P
         self.cfg = Config(spec.L, P, spec.alphac, spec.alphar, spec.n_slater, spec.spin_parity,
                           spec.J1, spec.J2, n_walkers, n_samples, n_warmup, n_warmup_first, n_update,
-                          recompute_k, s_precision, 0, seed, p_restart, p_global)
+                          recompute_k, s_precision, int(fused_measure), seed, p_restart, p_global,
+                          cond_guard)
         # reference layouts: W_map(i,1,g) 1-based; Orbital_Idx(i,j) column-major 1-based
         wmap = np.ascontiguousarray(spec.w_map.astype(np.int32) + 1)
         chi = np.ascontiguousarray(np.stack([spec.chi.real, spec.chi.imag], -1).reshape(-1))
@@ -195,6 +199,14 @@ class Engine:
         x = np.zeros((n, self.N), np.int32); psi = np.zeros(2 * n)
         self._check(lib().rbmpp_get_samples(self.h, _i(x), _d(psi)))
         return x, psi.view(np.complex128)
+
+    def get_measured(self):
+        """Per-sample psi, reweighted E_loc and reweight factor of the last
+        measurement."""
+        n = lib().rbmpp_n_samples(self.h)
+        psi = np.zeros(2 * n); el = np.zeros(2 * n); rw = np.zeros(n)
+        self._check(lib().rbmpp_get_measured(self.h, _d(psi), _d(el), _d(rw)))
+        return psi.view(np.complex128), el.view(np.complex128), rw
 
     def get_walkers(self, with_invm: bool = False):
         W, N, Cc = self.W, self.N, self.C

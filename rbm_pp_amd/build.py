@@ -27,7 +27,8 @@ def build(verbose: bool = False) -> str:
     for src, obj, p in procs:
         out, _ = p.communicate()
         text = out.decode(errors="replace")
-        lines = [l for l in text.splitlines() if "packed-fp32-ops" not in l and l.strip()]
+        lines = [l for l in text.splitlines() if "packed-fp32-ops" not in l and l.strip()
+                 and "nodiscard" not in l]
         if p.returncode != 0:
             sys.stderr.write(text)
             raise RuntimeError(f"hipcc failed on {src}")
@@ -41,4 +42,4 @@ def build(verbose: bool = False) -> str:
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    print(build(verbose="-v" in sys.argv))

@@ -47,6 +47,8 @@ struct rbmpp_handle {
   double2* d_psis = nullptr;
   int64_t n_stored = 0;
   int walkers_ready = 0;
+  int fused_valid = 0;     // measurement buffers hold the fused results of the stored samples
+  double fused_shift = 0;  // reweighting offset they were made with
   // measurement
   size_t meas_cap = 0;  // configurations the measurement buffers hold
   double2 *d_mpsi = nullptr, *d_eloc = nullptr;
@@ -58,6 +60,7 @@ struct rbmpp_handle {
   // SR
   double *d_S = nullptr, *d_A = nullptr, *d_L = nullptr, *d_vec = nullptr;
   int* d_info = nullptr;
+  DevModel* d_model = nullptr;  // device copy of devmodel(h)
   int measured = 0;
   int s_valid = 0;
   double m_norm = 0, m_etot = 0, m_etot2 = 0, m_wfav = 0;
@@ -112,6 +115,8 @@ static double toc(rbmpp_handle* h) {
   hipEventElapsedTime(&ms, h->e0, h->e1);
   return ms * 1e-3;
 }
+
+static int ensure_measure(rbmpp_handle* h, size_t n);
 
 extern "C" {
 
@@ -211,6 +216,11 @@ int rbmpp_create(const rbmpp_config* cfg, const int32_t* w_map, const double* pa
   if (N > RBMPP_NLDS && (rc = dalloc(h, &ws.scratch, W * NN))) return rc;
   const size_t ns = (size_t)W * std::max(h->nspw, 1);
   if ((rc = dalloc(h, &h->d_xs, ns * N)) || (rc = dalloc(h, &h->d_psis, ns))) return rc;
+  if ((rc = dalloc(h, &h->d_model, 1))) return rc;
+  {
+    const DevModel hm = devmodel(h);
+    HIPCHK(h, hipMemcpyAsync(h->d_model, &hm, sizeof(DevModel), hipMemcpyHostToDevice, h->st));
+  }
   HIPCHK(h, hipMemcpyAsync(h->d_wmap, h->wmap.data(), sizeof(int) * P * N, hipMemcpyHostToDevice, h->st));
   HIPCHK(h, hipMemcpyAsync(h->d_invmap, h->invmap.data(), sizeof(int) * P * N, hipMemcpyHostToDevice, h->st));
   HIPCHK(h, hipMemcpyAsync(h->d_orb, h->orb.data(), sizeof(int) * NN, hipMemcpyHostToDevice, h->st));
@@ -270,6 +280,7 @@ int rbmpp_set_gamma(rbmpp_handle* h, const double* gamma) {
   h->t[0] = toc(h);
   HIPCHK(h, hipStreamSynchronize(h->st));
   h->s_valid = 0;
+  h->fused_valid = 0;  // the fused values belong to the previous parameters
   return RBMPP_OK;
 }
 
@@ -283,7 +294,7 @@ int rbmpp_make_sample(rbmpp_handle* h, int32_t first_iteration, double shift, in
   if (!h) return RBMPP_E_ARG;
   if (!first_iteration && !h->walkers_ready) return fail(h, RBMPP_E_STATE, "make_sample: walkers not initialised");
   SweepArgs a;
-  a.m = devmodel(h);
+  a.m = h->d_model;
   a.ws = h->ws;
   a.nwalk = h->W;
   a.nsamp = h->nspw;
@@ -297,12 +308,23 @@ int rbmpp_make_sample(rbmpp_handle* h, int32_t first_iteration, double shift, in
   a.shift = shift;
   a.p_restart = h->cfg.p_restart;
   a.p_global = h->cfg.p_global;
+  a.cond_guard = h->cfg.cond_guard;
   a.key0 = (uint32_t)h->cfg.seed;
   a.key1 = (uint32_t)(h->cfg.seed >> 32);
   a.xs = h->d_xs;
   a.psis = h->d_psis;
+  a.measure = 0;
+  a.m_psi = nullptr; a.m_eloc = nullptr; a.m_rw = nullptr; a.m_orows = nullptr;
+  h->fused_valid = 0;
+  const bool fused = h->cfg.fused_measure && h->N <= RBMPP_NLDS && h->nspw > 0;
+  if (fused) {
+    int rc = ensure_measure(h, (size_t)h->W * h->nspw);
+    if (rc) return rc;
+    a.measure = 1;
+    a.m_psi = h->d_mpsi; a.m_eloc = h->d_eloc; a.m_rw = h->d_rw; a.m_orows = h->d_orows;
+  }
   tic(h);
-  if (launch_sweep(a, h->st) != 0) return fail(h, RBMPP_E_ARG, "sweep: unsupported size");
+  if (launch_sweep(devmodel(h), a, h->st) != 0) return fail(h, RBMPP_E_ARG, "sweep: unsupported size");
   HIPCHK(h, hipGetLastError());
   h->t[1] = toc(h);
   std::vector<int> stat(h->W);
@@ -324,6 +346,8 @@ int rbmpp_make_sample(rbmpp_handle* h, int32_t first_iteration, double shift, in
   h->t[6] = (double)trec;
   if (naccept) *naccept = tot;
   h->walkers_ready = 1;
+  h->fused_valid = fused ? 1 : 0;
+  h->fused_shift = shift;
   h->n_stored = (int64_t)h->W * h->nspw;
   h->measured = 0;
   return RBMPP_OK;
@@ -361,9 +385,10 @@ static int run_local(rbmpp_handle* h, int n, const signed char* d_x, int mode, d
                      double* d_expf) {
   int rc = ensure_measure(h, (size_t)n);
   if (rc) return rc;
+  h->fused_valid = 0;  // the buffers are about to be overwritten
   LocalArgs a;
   memset(&a, 0, sizeof a);
-  a.m = devmodel(h);
+  a.m = h->d_model;
   a.nconf = n;
   a.xs = d_x;
   a.mode = mode;
@@ -382,7 +407,13 @@ static int run_local(rbmpp_handle* h, int n, const signed char* d_x, int mode, d
   a.ex_pf = d_expf;
   a.status = h->d_status;
   const int grid = std::min(n, h->local_grid);
-  if (launch_local(a, grid, h->st) != 0) return fail(h, RBMPP_E_ARG, "local: unsupported size");
+  // the kernel reads the descriptor from device memory; real_only may have
+  // been switched for this call
+  {
+    const DevModel hm = devmodel(h);
+    HIPCHK(h, hipMemcpyAsync(h->d_model, &hm, sizeof(DevModel), hipMemcpyHostToDevice, h->st));
+  }
+  if (launch_local(devmodel(h), a, grid, h->st) != 0) return fail(h, RBMPP_E_ARG, "local: unsupported size");
   HIPCHK(h, hipGetLastError());
   std::vector<int> stat(n);
   HIPCHK(h, hipMemcpyAsync(stat.data(), h->d_status, sizeof(int) * n, hipMemcpyDeviceToHost, h->st));
@@ -434,7 +465,9 @@ int rbmpp_measure(rbmpp_handle* h, double shift, double* etot, double* etot2, do
   if (h->n_stored <= 0) return fail(h, RBMPP_E_STATE, "measure: no samples");
   const int n = (int)h->n_stored;
   tic(h);
-  int rc = run_local(h, n, h->d_xs, 1 | 2, shift, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+  int rc = 0;
+  if (!(h->fused_valid && h->fused_shift == shift))
+    rc = run_local(h, n, h->d_xs, 1 | 2, shift, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
   if (rc) return rc;
   rc = reduce_measure(h, n, true);
   if (rc) return rc;
@@ -456,7 +489,9 @@ int rbmpp_measure_energy(rbmpp_handle* h, double shift, double* etot, double* et
   if (h->n_stored <= 0) return fail(h, RBMPP_E_STATE, "measure_energy: no samples");
   const int n = (int)h->n_stored;
   tic(h);
-  int rc = run_local(h, n, h->d_xs, 1, shift, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+  int rc = 0;
+  if (!(h->fused_valid && h->fused_shift == shift))
+    rc = run_local(h, n, h->d_xs, 1, shift, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
   if (rc) return rc;
   rc = reduce_measure(h, n, false);
   if (rc) return rc;
@@ -485,9 +520,11 @@ static int build_s(rbmpp_handle* h) {
   HIPCHK(h, hipMemcpyAsync(h->d_vec, h->m_o.data(), sizeof(double) * nv, hipMemcpyHostToDevice, h->st));
   launch_row_kappa(n, h->real_only ? 0 : 1, h->d_mpsi, h->d_rw, h->d_kappa, h->st);
   const int prec = h->cfg.s_precision == RBMPP_S_FP32 ? 1 : 0;
+  tic(h);
   launch_syrk(prec, rows, nv, h->d_orows, h->d_kappa, h->d_vec, nsplit, h->d_part, 1.0 / h->m_norm, h->d_S,
               h->st);
   HIPCHK(h, hipGetLastError());
+  h->t[3] = toc(h);
   h->s_valid = 1;
   return 0;
 }
@@ -508,10 +545,8 @@ int rbmpp_sr_step(rbmpp_handle* h, const rbmpp_sr_config* c, double* gamma, rbmp
   if (!h || !c || !gamma) return RBMPP_E_ARG;
   if (!h->measured) return fail(h, RBMPP_E_STATE, "sr_step before measure");
   const int nv = h->nv, nvr = h->nv_rbm, N = h->N, nsl = h->nsl;
-  tic(h);
   int rc = build_s(h);
   if (rc) return rc;
-  h->t[3] = toc(h);
   hipEvent_t ea, eb;
   hipEventCreate(&ea);
   hipEventCreate(&eb);
@@ -530,8 +565,14 @@ int rbmpp_sr_step(rbmpp_handle* h, const rbmpp_sr_config* c, double* gamma, rbmp
     if (d[k] > 0.0) {
       if (k < nvr) { i1++; avR += d[k]; } else { i2++; avP += d[k]; }
     }
-  if (i1 != nvr) return fail(h, RBMPP_E_SDIAG, "i1/=Nv_RBM (SR_pdposv.f90:32)");
-  if (i2 != nsl - nsl / N) return fail(h, RBMPP_E_SDIAG, "i2 /= N_Slater - N_Slater/N (SR_pdposv.f90:33)");
+  if (i1 != nvr || i2 != nsl - nsl / N) {
+    int kbad = -1;
+    for (int k = 0; k < nv && kbad < 0; k++)
+      if (!(d[k] > 0.0) && (k < nvr || !std::isfinite(d[k]) || d[k] < 0.0)) kbad = k;
+    const std::string at = kbad >= 0 ? " (first: S[" + std::to_string(kbad) + "] = " + std::to_string(d[kbad]) + ")" : "";
+    if (i1 != nvr) return fail(h, RBMPP_E_SDIAG, "i1/=Nv_RBM (SR_pdposv.f90:32)" + at);
+    return fail(h, RBMPP_E_SDIAG, "i2 /= N_Slater - N_Slater/N (SR_pdposv.f90:33): " + std::to_string(i2) + " positive" + at);
+  }
   avR = nvr > 0 ? avR / i1 : 1.0;
   avP = avP / i2;
   const double scal = std::sqrt(avR / avP);
@@ -659,6 +700,7 @@ int rbmpp_set_samples(rbmpp_handle* h, int32_t n, const int32_t* x) {
   HIPCHK(h, hipMemcpyAsync(h->d_xs, xb.data(), xb.size(), hipMemcpyHostToDevice, h->st));
   HIPCHK(h, hipStreamSynchronize(h->st));
   h->n_stored = n;
+  h->fused_valid = 0;
   h->measured = 0;
   return RBMPP_OK;
 }
@@ -675,6 +717,17 @@ int rbmpp_get_samples(const rbmpp_handle* h0, int32_t* x, double* psi) {
     for (size_t k = 0; k < xb.size(); k++) x[k] = xb[k];
   }
   if (psi) HIPCHK(h, hipMemcpy(psi, h->d_psis, sizeof(double2) * n, hipMemcpyDeviceToHost));
+  return RBMPP_OK;
+}
+
+int rbmpp_get_measured(const rbmpp_handle* h0, double* psi, double* eloc, double* rw) {
+  rbmpp_handle* h = const_cast<rbmpp_handle*>(h0);
+  if (!h) return RBMPP_E_ARG;
+  const size_t n = (size_t)h->n_stored;
+  if (n > h->meas_cap || !h->d_mpsi) return fail(h, RBMPP_E_STATE, "get_measured: no measurement");
+  if (psi) HIPCHK(h, hipMemcpy(psi, h->d_mpsi, sizeof(double2) * n, hipMemcpyDeviceToHost));
+  if (eloc) HIPCHK(h, hipMemcpy(eloc, h->d_eloc, sizeof(double2) * n, hipMemcpyDeviceToHost));
+  if (rw) HIPCHK(h, hipMemcpy(rw, h->d_rw, sizeof(double) * n, hipMemcpyDeviceToHost));
   return RBMPP_OK;
 }
 

@@ -65,6 +65,11 @@ __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, RBMPP_WAVE);
   return v;
 }
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, RBMPP_WAVE));
+  return v;
+}
 
 // amplitude of projection copy q without the Pfaffian (sub.f90:30-48), from
 // angle arrays thc[q*ac+f], thr[q*ar+f] and the sign of operation g.

@@ -9,7 +9,7 @@
 #define RBMPP_CMAX 16   // projection copies per walker held in LDS (2 x 8 operations)
 #define RBMPP_MRMAX 512 // real RBM angles per walker
 #define RBMPP_MCMAX 256 // complex RBM angles per walker
-#define RBMPP_NLDS 112  // largest N whose N x N recompute matrix is staged in LDS
+#define RBMPP_NLDS 104  // largest N whose N x N matrices are staged in LDS
 
 // walker status codes (mirrors RBMPP_E_* of rbmpp.h)
 #define WS_OK 0
@@ -39,18 +39,27 @@ struct WalkerState {
 };
 
 struct SweepArgs {
-  DevModel m;
+  const DevModel* m;  // device copy of the model descriptor
   WalkerState ws;
   int nwalk, nsamp, nwarm, nupdate, fixed_k, first;
   double shift, p_restart, p_global;
+  // > 0: recompute Pf and X^-1 right after an accepted move whose Pf ratio
+  // of any copy lies outside (1/cond_guard, cond_guard) (0: never, as the
+  // reference)
+  double cond_guard;
   uint32_t key0, key1;
   signed char* xs;  // [W*nsamp][N] stored samples
   double2* psis;    // [W*nsamp]
+  // measurement fused into the sampler (at every stored sweep): reweighted
+  // local energy, amplitude, reweight factor and O row(s) per sample
+  int measure;
+  double2 *m_psi, *m_eloc;
+  double *m_rw, *m_orows;
 };
 
 // per-configuration evaluation (eval / local energy / O_k / measurement)
 struct LocalArgs {
-  DevModel m;
+  const DevModel* m;  // device copy of the model descriptor
   int nconf;
   const signed char* xs;  // [nconf][N]
   int mode;               // bit0: eloc, bit1: O_k, bit2: write invm, bit3: custom exchanges
@@ -72,5 +81,5 @@ struct LocalArgs {
 };
 
 void launch_slater(const DevModel& m, const double* f, double* se, int* bad, hipStream_t s);
-int launch_sweep(const SweepArgs& a, hipStream_t s);
-int launch_local(const LocalArgs& a, int grid, hipStream_t s);
+int launch_sweep(const DevModel& hm, const SweepArgs& a, hipStream_t s);
+int launch_local(const DevModel& hm, const LocalArgs& a, int grid, hipStream_t s);

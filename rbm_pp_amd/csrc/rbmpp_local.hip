@@ -11,6 +11,7 @@
 // lane r owns P_{perm[r]} = -sum_c A(c, r) a_{perm[c]} (skew symmetry turns
 // the product into a contiguous row read of LDS).
 #include "rbmpp_kernels.h"
+#include "rbmpp_measure.h"
 
 template <int NMAX, bool LDSMAT>
 struct LocalSmem {
@@ -40,7 +41,7 @@ template <int NMAX, bool LDSMAT>
 __global__ __launch_bounds__(512) void k_local(LocalArgs a) {
   constexpr int KMAX = NMAX / RBMPP_WAVE;
   __shared__ LocalSmem<NMAX, LDSMAT> s;
-  const DevModel& m = a.m;
+  const DevModel& m = *a.m;
   const int tid = threadIdx.x, nth = blockDim.x;
   const int lane = tid & (RBMPP_WAVE - 1), wid = tid >> 6, nwv = nth >> 6;
   const int N = m.N, C = m.C, P = m.P, np = m.npair;
@@ -70,16 +71,7 @@ __global__ __launch_bounds__(512) void k_local(LocalArgs a) {
       if (custom) {
         s.bi1[0] = (short)a.ex_i1[cfg]; s.bi2[0] = (short)a.ex_i2[cfg]; nb = 1;
       } else if (want_eloc) {
-        for (int t = 0; t < 4; t++)
-          for (int i1 = 0; i1 < N; i1++) {
-            int ix = i1 / m.L, iy = i1 - ix * m.L;
-            if (t == 0) { ix = (ix + 1) % m.L; }
-            else if (t == 1) { iy = (iy + 1) % m.L; }
-            else if (t == 2) { ix = (ix + 1) % m.L; iy = (iy + 1) % m.L; }
-            else { ix = (ix + 1) % m.L; iy = (iy + m.L - 1) % m.L; }
-            const int i2 = ix * m.L + iy;
-            if (s.x[i1] != s.x[i2]) { s.bi1[nb] = (short)i1; s.bi2[nb] = (short)i2; nb++; }
-          }
+        nb = bond_list(m, s);
       }
       s.nb = nb;
     }
@@ -242,29 +234,10 @@ __global__ __launch_bounds__(512) void k_local(LocalArgs a) {
       double2 p = make_double2(0.0, 0.0);
       for (int q = 0; q < C; q++) p = cadd(p, s.psiq[q]);
       s.psi = p;
-      double2 e = make_double2(0.0, 0.0);
-      if (want_eloc && !custom) {
-        int b = 0;
-        for (int t = 0; t < 4; t++) {
-          const double J = t <= 1 ? m.J1 : m.J2;
-          const double sg = t <= 1 ? -1.0 : 1.0;
-          for (int i1 = 0; i1 < N; i1++) {
-            int ix = i1 / m.L, iy = i1 - ix * m.L;
-            if (t == 0) { ix = (ix + 1) % m.L; }
-            else if (t == 1) { iy = (iy + 1) % m.L; }
-            else if (t == 2) { ix = (ix + 1) % m.L; iy = (iy + 1) % m.L; }
-            else { ix = (ix + 1) % m.L; iy = (iy + m.L - 1) % m.L; }
-            const int i2 = ix * m.L + iy;
-            const double d = J * (double)s.x[i1] * (double)s.x[i2];
-            e = cadd(e, cscale(p, d));
-            if (s.x[i1] != s.x[i2]) { e = cadd(e, cscale(s.psib[b], 2.0 * J * sg)); b++; }
-          }
-        }
-      }
       const double rw = a.shift >= 0.0 ? 1.0 / (cabs2(p) + a.shift) : 1.0;
       a.psi[cfg] = p;
       if (a.rw) a.rw[cfg] = rw;
-      if (want_eloc && !custom) a.eloc[cfg] = cscale(e, rw);
+      if (want_eloc && !custom) a.eloc[cfg] = cscale(eloc_from_bonds(m, s, p), rw);
       if (custom && a.ex_psi) a.ex_psi[cfg] = s.psib[0];
       s.red[0] = rw;
     }
@@ -274,60 +247,9 @@ __global__ __launch_bounds__(512) void k_local(LocalArgs a) {
     }
     __syncthreads();
     if (want_o) {
-      // O_loc(k) = d psi / d gamma_k (calc_phys_quantity.f90:114-222) times rw
-      const double rw = s.red[0];
-      const int nv = m.nv;
       const bool cplx_rows = !m.real_only;
-      double* ore = a.orows + (size_t)(cplx_rows ? 2 * cfg : cfg) * nv;
-      double* oim = ore + nv;
-      const int ac = m.ac, ar = m.ar;
-      const int offr = 2 * ac * (N + 1);
-      for (int k = tid; k < nv; k += nth) {
-        double2 o = make_double2(0.0, 0.0);
-        if (k < 2 * ac) {
-          // complex bias f: k = 2f (re), 2f+1 (im) = i * re
-          const int f = k >> 1;
-          for (int q = 0; q < C; q++) {
-            const double2 t = s.thc[q * ac + f];
-            const double th = tanh(t.x), tn = tan(t.y);
-            // (th + i tn) / (1 + i th tn)
-            const double den = 1.0 + (th * tn) * (th * tn);
-            const double2 c2 = make_double2((th + tn * (th * tn)) / den, (tn - th * (th * tn)) / den);
-            o = cadd(o, cmul(c2, s.psiq[q]));
-          }
-          if (k & 1) o = make_double2(-o.y, o.x);
-        } else if (k < offr) {
-          const int kk = k - 2 * ac, f = kk / (2 * N), iw = (kk - f * 2 * N) >> 1;
-          for (int q = 0; q < C; q++) {
-            const int g = q >> 1;
-            const double sp = (q & 1) ? -1.0 : 1.0;
-            const double2 t = s.thc[q * ac + f];
-            const double th = tanh(t.x), tn = tan(t.y);
-            const double den = 1.0 + (th * tn) * (th * tn);
-            const double2 c2 = make_double2((th + tn * (th * tn)) / den, (tn - th * (th * tn)) / den);
-            const double2 c1 = cmul(c2, s.psiq[q]);
-            const int i = m.invmap[g * N + iw];
-            o = cadd(o, cscale(c1, (double)s.x[i] * sp));
-          }
-          if (kk & 1) o = make_double2(-o.y, o.x);
-        } else if (k < offr + ar) {
-          const int f = k - offr;
-          for (int q = 0; q < C; q++) o = cadd(o, cscale(s.psiq[q], tanh(s.thr[q * ar + f])));
-        } else if (k < m.nv_rbm) {
-          const int kk = k - offr - ar, f = kk / N, iw = kk - f * N;
-          for (int q = 0; q < C; q++) {
-            const int g = q >> 1;
-            const double sp = (q & 1) ? -1.0 : 1.0;
-            const int i = m.invmap[g * N + iw];
-            o = cadd(o, cscale(cscale(s.psiq[q], tanh(s.thr[q * ar + f])), (double)s.x[i] * sp));
-          }
-        } else {
-          const int orb = k - m.nv_rbm;
-          o = make_double2(s.opp[0][orb], s.opp[1][orb]);
-        }
-        ore[k] = o.x * rw;
-        if (cplx_rows) oim[k] = o.y * rw;
-      }
+      double* ore = a.orows + (size_t)(cplx_rows ? 2 * cfg : cfg) * m.nv;
+      wg_orow(m, s, s.red[0], ore, ore + m.nv, cplx_rows);
     }
     if (tid == 0 && a.status) a.status[cfg] = WS_OK;
   }
@@ -338,10 +260,10 @@ static void launch_local_t(const LocalArgs& a, int grid, hipStream_t st) {
   hipLaunchKernelGGL((k_local<NMAX, LDSMAT>), dim3(grid), dim3(512), 0, st, a);
 }
 
-int launch_local(const LocalArgs& a, int grid, hipStream_t st) {
-  const int N = a.m.N;
-  if (a.m.C > RBMPP_CMAX || a.m.mr > RBMPP_MRMAX || a.m.mc > RBMPP_MCMAX) return -1;
-  if (a.m.nsl > 4 * N) return -1;
+int launch_local(const DevModel& hm, const LocalArgs& a, int grid, hipStream_t st) {
+  const int N = hm.N;
+  if (hm.C > RBMPP_CMAX || hm.mr > RBMPP_MRMAX || hm.mc > RBMPP_MCMAX) return -1;
+  if (hm.nsl > 4 * N) return -1;
   if (N <= 64) launch_local_t<64, true>(a, grid, st);
   else if (N <= RBMPP_NLDS) launch_local_t<128, true>(a, grid, st);
   else if (N <= 256) {

@@ -10,6 +10,7 @@
 // pass, reusing P and Q.  All threads draw the same Philox numbers, so the
 // control flow is uniform across the workgroup.
 #include "rbmpp_kernels.h"
+#include "rbmpp_measure.h"
 
 // ------------------------------------------------------------------
 // update_slater_elements (slater.f90:7-64).  se[q][c][r] = Slater_Elm(r,c,q).
@@ -43,18 +44,28 @@ void launch_slater(const DevModel& m, const double* f, double* se, int* bad, hip
 }
 
 // ------------------------------------------------------------------
-// shared-memory layout of the sampler workgroup
+// shared-memory layout of the sampler workgroup.  Each wave (one copy)
+// keeps a per-row quad {s_j|a_j, t_j|b_j, p_j, q_j} for broadcasts; the quads
+// share storage with the LDS recompute matrix, which is only live between
+// proposals.
 template <int NMAX, bool LDSMAT>
 struct SweepSmem {
-  static constexpr int NW = 16;
-  static constexpr int PQ = 2 * RBMPP_CMAX * NMAX;  // per-copy P, Q
-  static constexpr int AB = 2 * NW * NMAX;          // per-wave gathered a, b (then s, t)
+  static constexpr int NW = RBMPP_CMAX;
+  static constexpr int VEC = NW * 4 * NMAX;
   static constexpr int MAT = LDSMAT ? (NMAX < RBMPP_NLDS ? NMAX * NMAX : RBMPP_NLDS * RBMPP_NLDS) : 0;
-  static constexpr int UNION = (PQ + AB) > MAT ? (PQ + AB) : MAT;
+  // fused measurement: one copy's X^-1 (at offset 0) + per-wave a, b vectors
+  static constexpr int MWAVES = 8;
+  static constexpr int MEAS = LDSMAT ? MAT + MWAVES * 2 * NMAX : 0;
+  static constexpr int UNION = VEC > MEAS ? VEC : MEAS;
+  static constexpr int MB = LDSMAT ? 4 * NMAX : 1;  // bonds / orbital bins of the fused measurement
   double u[UNION];
+  double2 psib[MB];
+  double opp[2][MB];
+  short bi1[MB], bi2[MB];
+  int nb;
   double thr[RBMPP_MRMAX];
   double2 thc[RBMPP_MCMAX];
-  double pf[RBMPP_CMAX], pfn[RBMPP_CMAX], bma[RBMPP_CMAX], vba[RBMPP_CMAX], invab[RBMPP_CMAX];
+  double pf[RBMPP_CMAX], pfn[RBMPP_CMAX];
   double2 psiq[RBMPP_CMAX];
   double gjbuf[4 * NMAX];
   double red[32];
@@ -64,17 +75,18 @@ struct SweepSmem {
   unsigned sgnbits[RBMPP_CMAX / 2];
   int redi[32];
   int piv, flag, status;
+  int guard;  // set by a copy whose updated X^-1 exceeds the conditioning bound
   unsigned long long ctr;
 };
 
-// ------------------------------------------------------------------
-// Full recompute of a walker from s.x and the current electron order
-// s.eidx (make_sample.f90:78-97): theta (calc_theta), sign_PP, per-copy
-// Pf and X^-1 (calculate_MAll), amplitude.  With check=1 the recomputed
-// theta and sign are compared with the incremental values
-// (make_sample.f90:84-92) and the Pf / X^-1 drift is returned.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int NMAX, bool LDSMAT>
-__device__ void wg_recompute(const DevModel& m, SweepSmem<NMAX, LDSMAT>& s, double* Mw,
+__device__ __attribute__((noinline)) void wg_recompute(const DevModel& m, SweepSmem<NMAX, LDSMAT>& s, double* Mw,
                              double* gscratch, int check, double* drift_pf, double* drift_inv) {
   const int tid = threadIdx.x, nth = blockDim.x;
   const int N = m.N, C = m.C, P = m.P;
@@ -178,7 +190,7 @@ __device__ void wg_x_to_ele(const DevModel& m, SweepSmem<NMAX, LDSMAT>& s) {
 // |psi| > 1e-300.  The serial draws run on thread 0; the counter is then
 // broadcast so every thread's stream copy stays in step.
 template <int NMAX, bool LDSMAT>
-__device__ int wg_initial_x(const DevModel& m, SweepSmem<NMAX, LDSMAT>& s, Rng& rng, double* Mw,
+__device__ __attribute__((noinline)) int wg_initial_x(const DevModel& m, SweepSmem<NMAX, LDSMAT>& s, Rng& rng, double* Mw,
                             double* gscratch) {
   for (int attempt = 0; attempt < 1000; attempt++) {
     if (threadIdx.x == 0) {
@@ -203,15 +215,170 @@ __device__ int wg_initial_x(const DevModel& m, SweepSmem<NMAX, LDSMAT>& s, Rng& 
 }
 
 // ------------------------------------------------------------------
+// ------------------------------------------------------------------
+// Measurement of the stored sample with the walker's own X^-1 (fused
+// alternative to calc_phys_quantity_opt, calc_phys_quantity.f90:69-234, which
+// recomputes Pf and X^-1 from scratch; here they come from the sampler's
+// fast-update chain, refreshed by the full recompute every K accepts).  Per
+// copy the N x N X^-1 is staged in LDS; half of the waves evaluate the Pf
+// ratio of every antiparallel bond (calculate_new_PfM_two_child), the other
+// half scatter the Pfaffian derivative into the orbital bins
+// (calculate_derivative_Pfaffian_child).  Writes the reweighted E_loc,
+// amplitude, reweight factor and O row(s) of sample `row`.
+template <int NMAX, bool LDSMAT>
+__device__ __attribute__((noinline)) void wg_measure_sample(const DevModel& m, SweepSmem<NMAX, LDSMAT>& s,
+                                                            const double* Mw, size_t row, const SweepArgs& a) {
+  constexpr int KMAX = NMAX / RBMPP_WAVE;
+  const int tid = threadIdx.x, nth = blockDim.x;
+  const int lane = tid & (RBMPP_WAVE - 1);
+  const int wid = __builtin_amdgcn_readfirstlane(tid >> 6), nwv = nth >> 6;
+  const int N = m.N, C = m.C, np = m.npair;
+  const size_t NN = (size_t)N * N;
+  const int nbw = nwv >= 2 ? min(nwv / 2, SweepSmem<NMAX, LDSMAT>::MWAVES) : 1;  // bond waves
+  if (tid == 0) s.nb = bond_list(m, s);
+  for (int k = tid; k < 4 * NMAX; k += nth) {
+    s.psib[k] = make_double2(0.0, 0.0);
+    s.opp[0][k] = 0.0;
+    s.opp[1][k] = 0.0;
+  }
+  __syncthreads();
+  const int nb = s.nb;
+  double* Ms = s.u;
+  for (int q = 0; q < C; q++) {
+    const int g = q >> 1, sidx = q & 1;
+    const double* Mq = Mw + (size_t)q * NN;
+    const double* Sq = m.se + (size_t)q * 4 * NN;
+    const double pfq = s.pf[q];
+    __syncthreads();
+    for (size_t e = 2 * (size_t)tid; e < NN; e += 2 * (size_t)nth) *(double2*)(Ms + e) = *(const double2*)(Mq + e);
+    __syncthreads();
+    const bool pp_wave = (nwv >= 2) ? (wid >= nbw) : true;
+    const bool bond_wave = (nwv >= 2) ? (wid < nbw) : true;
+    if (pp_wave) {
+      // InvM(a,b) = Ms[a*N + b]; weights cs, cc, ss times Pf (slater.f90:136-167)
+      const double cs = pfq * m.sp_cs[sidx], cc = pfq * m.sp_cc[sidx], ss = pfq * m.sp_ss[sidx];
+      const double2 wq = psi_wo_pf(m, q, s.thc, s.thr, s.sgn[g]);
+      const int t0 = (nwv >= 2) ? (tid - nbw * RBMPP_WAVE) : tid;
+      const int tn = (nwv >= 2) ? (nth - nbw * RBMPP_WAVE) : nth;
+      for (int e = t0; e < np * np; e += tn) {
+        const int msj = e / np, msi = e - msj * np;
+        const int ti = m.wmap[g * N + s.eidx[msi]], tip = m.wmap[g * N + s.eidx[msi + np]];
+        const int tj = m.wmap[g * N + s.eidx[msj]], tjp = m.wmap[g * N + s.eidx[msj + np]];
+        const double d1 = -Ms[(size_t)msi * N + msj] * cs;
+        const double d2 = Ms[(size_t)msi * N + msj + np] * cc;
+        const double d3 = -Ms[(size_t)(msi + np) * N + msj] * ss;
+        const double d4 = Ms[(size_t)(msi + np) * N + msj + np] * cs;
+        const int o1 = m.orb[ti * N + tj], o2 = m.orb[ti * N + tjp];
+        const int o3 = m.orb[tip * N + tj], o4 = m.orb[tip * N + tjp];
+        atomicAdd(&s.opp[0][o1], d1 * wq.x);
+        atomicAdd(&s.opp[0][o2], d2 * wq.x);
+        atomicAdd(&s.opp[0][o3], d3 * wq.x);
+        atomicAdd(&s.opp[0][o4], d4 * wq.x);
+        if (!m.real_only) {
+          atomicAdd(&s.opp[1][o1], d1 * wq.y);
+          atomicAdd(&s.opp[1][o2], d2 * wq.y);
+          atomicAdd(&s.opp[1][o3], d3 * wq.y);
+          atomicAdd(&s.opp[1][o4], d4 * wq.y);
+        }
+      }
+    }
+    if (bond_wave && wid < nbw) {
+      double* va = s.u + SweepSmem<NMAX, LDSMAT>::MAT + (size_t)wid * 2 * NMAX;
+      double* vb = va + NMAX;
+      for (int b = wid; b < nb; b += nbw) {
+        const int i1 = s.bi1[b], i2 = s.bi2[b];
+        const int msa = s.ecfg[i1], msb = s.ecfg[i2];
+        const int rsa = i2 + (msa >= np ? N : 0), rsb = i1 + (msb >= np ? N : 0);
+        const double* colA = Sq + (size_t)rsa * 2 * N;
+        const double* colB = Sq + (size_t)rsb * 2 * N;
+        for (int i = lane; i < N; i += RBMPP_WAVE) {
+          int t = s.eidx[i];
+          if (i == msa) t = i2;
+          else if (i == msb) t = i1;
+          const int r = t + (i >= np ? N : 0);
+          va[i] = colA[r];
+          vb[i] = colB[r];
+        }
+        wave_sync();
+        // P(i) = sum_j InvM(j,i) a_j (lane-owned i); Q(msa), Q(msb) by reduction
+        double Pk[KMAX];
+#pragma unroll
+        for (int k = 0; k < KMAX; k++) Pk[k] = 0.0;
+        for (int j = 0; j < N; j++) {
+          const double aj = va[j];
+          const double* rowj = Ms + (size_t)j * N;
+#pragma unroll
+          for (int k = 0; k < KMAX; k++) {
+            const int i = lane + k * RBMPP_WAVE;
+            if (i < N) Pk[k] = fma(rowj[i], aj, Pk[k]);
+          }
+        }
+        double qa = 0.0, qb = 0.0, bdotp = 0.0;
+        for (int j = lane; j < N; j += RBMPP_WAVE) {
+          qa = fma(Ms[(size_t)j * N + msa], vb[j], qa);
+          qb = fma(Ms[(size_t)j * N + msb], vb[j], qb);
+        }
+#pragma unroll
+        for (int k = 0; k < KMAX; k++) {
+          const int i = lane + k * RBMPP_WAVE;
+          if (i < N) bdotp = fma(vb[i], Pk[k], bdotp);
+        }
+        qa = wave_sum(qa);
+        qb = wave_sum(qb);
+        const double bMa = wave_sum(bdotp);
+        double pa = 0.0, pb = 0.0;
+#pragma unroll
+        for (int k = 0; k < KMAX; k++) {
+          const double ta = __shfl(Pk[k], msa & 63, RBMPP_WAVE);
+          const double tb = __shfl(Pk[k], msb & 63, RBMPP_WAVE);
+          if ((msa >> 6) == k) pa = ta;
+          if ((msb >> 6) == k) pb = tb;
+        }
+        const double invm_ab = Ms[(size_t)msb * N + msa];
+        const double vec_ba = vb[msa];
+        const double ratio = invm_ab * vec_ba + invm_ab * bMa + pa * qb - pb * qa;
+        int sfac = -1;
+        if (m.subB[i1]) sfac = -sfac;
+        if (m.subB[i2]) sfac = -sfac;
+        const double2 c = psi_wo_pf_exchanged(m, q, s.thc, s.thr, sfac * s.sgn[g], i1, i2, s.x[i1], s.x[i2]);
+        if (lane == 0) s.psib[b] = cadd(s.psib[b], cscale(c, ratio * pfq));
+        wave_sync();
+      }
+    }
+  }
+  __syncthreads();
+  for (int q = tid; q < C; q += nth) s.psiq[q] = cscale(psi_wo_pf(m, q, s.thc, s.thr, s.sgn[q >> 1]), s.pf[q]);
+  __syncthreads();
+  if (tid == 0) {
+    double2 p = make_double2(0.0, 0.0);
+    for (int q = 0; q < C; q++) p = cadd(p, s.psiq[q]);
+    const double rw = 1.0 / (cabs2(p) + a.shift);
+    a.m_psi[row] = p;
+    a.m_rw[row] = rw;
+    a.m_eloc[row] = cscale(eloc_from_bonds(m, s, p), rw);
+    s.red[0] = rw;
+  }
+  __syncthreads();
+  const bool cplx_rows = !m.real_only;
+  double* ore = a.m_orows + (cplx_rows ? 2 * row : row) * (size_t)m.nv;
+  wg_orow(m, s, s.red[0], ore, ore + m.nv, cplx_rows);
+  __syncthreads();
+}
+
+// ------------------------------------------------------------------
+// Lane layout of the streaming loops: lane l owns the column pair
+// (2l, 2l+1) of every 128-column block, so one 16-byte load per lane and
+// row covers a whole 100-wide row of X^-1 in one instruction.
 template <int NMAX, bool LDSMAT>
 __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
-  constexpr int KMAX = NMAX / RBMPP_WAVE;
+  constexpr int KD = NMAX >= 128 ? NMAX / 128 : 1;
+  constexpr int UN = 8;  // rows in flight per wave
   __shared__ SweepSmem<NMAX, LDSMAT> s;
-  const DevModel& m = a.m;
+  const DevModel& m = *a.m;
   const int w = blockIdx.x;
   if (w >= a.nwalk) return;
   const int tid = threadIdx.x, nth = blockDim.x;
-  const int lane = tid & (RBMPP_WAVE - 1), wid = tid >> 6, nwv = nth >> 6;
+  const int lane = tid & (RBMPP_WAVE - 1), wid = tid >> 6;
   const int N = m.N, C = m.C, P = m.P;
   const size_t NN = (size_t)N * N;
   double* Mw = a.ws.minv + (size_t)w * C * NN;
@@ -220,7 +387,10 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
   Rng rng;
   rng.k0 = a.key0; rng.k1 = a.key1; rng.stream = (uint32_t)w;
   rng.ctr = a.ws.ctr[w];
-  if (tid == 0) s.status = WS_OK;
+  if (tid == 0) {
+    s.status = WS_OK;
+    s.guard = 0;
+  }
   for (int i = tid; i < C * m.ar; i += nth) s.thr[i] = a.ws.thr[(size_t)w * m.mr + i];
   for (int i = tid; i < C * m.ac; i += nth) s.thc[i] = a.ws.thc[(size_t)w * m.mc + i];
   for (int q = tid; q < C; q += nth) s.pf[q] = a.ws.pf[(size_t)w * C + q];
@@ -243,10 +413,11 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
   int nfast = a.ws.nfastup[w];
   long long naccept = 0;
   int nrec = 0;
-  double* PQ = s.u;                                   // [C][2][NMAX]
-  double* AB = s.u + 2 * RBMPP_CMAX * NMAX;           // [NW][2][NMAX]
-  double* va = AB + (size_t)wid * 2 * NMAX;
-  double* vb = va + NMAX;
+  // this wave's projection copy and its broadcast quads
+  const int q = __builtin_amdgcn_readfirstlane(wid);  // wave-uniform
+  double* wv = s.u + (size_t)wid * 4 * NMAX;
+  double* Mq = Mw + (size_t)q * NN;
+  const double* Sq = m.se + (size_t)q * 4 * NN;
 
   for (int i0 = -a.nwarm; i0 <= a.nsamp && status == WS_OK; i0++) {
     // ---- periodic recompute (make_sample.f90:74-128)
@@ -304,158 +475,252 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
       if (m.subB[i2]) sfac = -sfac;
       const int rsa = i2 + (msa >= m.npair ? N : 0);  // new row of electron msa
       const int rsb = i1 + (msb >= m.npair ? N : 0);
-      // ---- ratio per copy (wave per copy)
-      for (int q0 = 0; q0 < C; q0 += nwv) {
-        const int q = q0 + wid;
-        if (q < C) {
-          const double* S = m.se + (size_t)q * 4 * N * N;
-          const double* colA = S + (size_t)rsa * 2 * N;
-          const double* colB = S + (size_t)rsb * 2 * N;
-          for (int i = lane; i < N; i += RBMPP_WAVE) {
-            int t = s.eidx[i];
-            if (i == msa) t = i2;
-            else if (i == msb) t = i1;
-            const int r = t + (i >= m.npair ? N : 0);
-            va[i] = colA[r];
-            vb[i] = colB[r];
-          }
-        }
-        __syncthreads();
-        if (q < C) {
-          const double* Mq = Mw + (size_t)q * NN;
-          double Pk[KMAX], Qk[KMAX];
+
+      // ---- Pfaffian ratio of copy q (calculate_new_PfM_two_child)
+      double2 Pk[KD], Qk[KD], bk[KD];
+      double bMa = 0.0, vec_ba = 0.0, invm_ab = 0.0, pa = 0.0, pb = 0.0, qa = 0.0, qb = 0.0;
+      double rq = 1.0;  // this copy's Pf ratio
+      if (q < C) {
+        const double* colA = Sq + (size_t)rsa * 2 * N;
+        const double* colB = Sq + (size_t)rsb * 2 * N;
 #pragma unroll
-          for (int k = 0; k < KMAX; k++) { Pk[k] = 0.0; Qk[k] = 0.0; }
-#pragma unroll 4
-          for (int j = 0; j < N; j++) {
-            const double aj = va[j], bj = vb[j];
-            const double* row = Mq + (size_t)j * N;
+        for (int k = 0; k < KD; k++) {
+          const int c = 2 * lane + 128 * k;
+          double2 av = make_double2(0.0, 0.0), bv = make_double2(0.0, 0.0);
+          if (c < N) {
 #pragma unroll
-            for (int k = 0; k < KMAX; k++) {
-              const int i = lane + k * RBMPP_WAVE;
-              const double mv = (i < N) ? row[i] : 0.0;
-              Pk[k] = fma(mv, aj, Pk[k]);
-              Qk[k] = fma(mv, bj, Qk[k]);
+            for (int e = 0; e < 2; e++) {
+              const int i = c + e;
+              int t = s.eidx[i];
+              if (i == msa) t = i2;
+              else if (i == msb) t = i1;
+              const int r = t + (i >= m.npair ? N : 0);
+              const double aa = colA[r], bb = colB[r];
+              if (e == 0) { av.x = aa; bv.x = bb; } else { av.y = aa; bv.y = bb; }
             }
+            wv[4 * c + 0] = av.x; wv[4 * c + 1] = bv.x;
+            wv[4 * c + 4] = av.y; wv[4 * c + 5] = bv.y;
           }
-          double* Pq = PQ + (size_t)q * 2 * NMAX;
-          double* Qq = Pq + NMAX;
-          double part = 0.0;
-#pragma unroll
-          for (int k = 0; k < KMAX; k++) {
-            const int i = lane + k * RBMPP_WAVE;
-            if (i < N) { Pq[i] = Pk[k]; Qq[i] = Qk[k]; part = fma(vb[i], Pk[k], part); }
-          }
-          const double bMa = wave_sum(part);
-          if (lane == 0) s.bma[q] = bMa;
+          bk[k] = bv;
+          Pk[k] = make_double2(0.0, 0.0);
+          Qk[k] = make_double2(0.0, 0.0);
         }
-        __syncthreads();
-        if (q < C && lane == 0) {
-          const double* Mq = Mw + (size_t)q * NN;
-          const double* Pq = PQ + (size_t)q * 2 * NMAX;
-          const double* Qq = Pq + NMAX;
-          const double invm_ab = Mq[(size_t)msb * N + msa];
-          const double vec_ba = vb[msa];
-          const double bMa = s.bma[q];
-          const double ratio = invm_ab * vec_ba + invm_ab * bMa + Pq[msa] * Qq[msb] - Pq[msb] * Qq[msa];
-          s.vba[q] = vec_ba;
-          s.invab[q] = invm_ab;
-          s.pfn[q] = ratio * s.pf[q];
-          const double2 c = psi_wo_pf_exchanged(m, q, s.thc, s.thr, sfac * s.sgn[q >> 1], i1, i2, x1, x2);
-          s.psiq[q] = cscale(c, s.pfn[q]);
-        }
-        __syncthreads();
-      }
-      // ---- Metropolis test (make_sample.f90:203-204)
-      double2 psin = make_double2(0.0, 0.0);
-      for (int q = 0; q < C; q++) psin = cadd(psin, s.psiq[q]);
-      const double r1 = (cabs2(psin) + a.shift) / (cabs2(s.psi) + a.shift);
-      if (rng.u01() < r1 * r1) {
-        // ---- accept: rank-4 update of every copy's X^-1
-        for (int q0 = 0; q0 < C; q0 += nwv) {
-          const int q = q0 + wid;
-          double Ai[KMAX], Bi[KMAX], Ci[KMAX], Di[KMAX], R1i[KMAX], R2i[KMAX];
-          double* Mq = Mw + (size_t)q * NN;
-          const double* Pq = PQ + (size_t)q * 2 * NMAX;
-          const double* Qq = Pq + NMAX;
-          double a_ = 0, b_ = 0, c_ = 0, d_ = 0, fID = 0;
-          if (q < C) {
-            const double Pa = Pq[msa], Pb = Pq[msb], Qa = Qq[msa], Qb = Qq[msb];
-            a_ = -Pa; b_ = Pb; c_ = Qa; d_ = -Qb;
-            const double e_ = -s.bma[q] - s.vba[q];
-            const double f_ = s.invab[q];
-            const double Det = a_ * d_ - b_ * c_ - e_ * f_;
-            const double InvDet = 1.0 / Det;
-            fID = f_ * InvDet;
-            const double eDet = e_ * Det;
-            // s_i = InvDet * InvM(i,msa) = -InvDet * InvM(msa,i); t_i likewise
-            for (int i = lane; i < N; i += RBMPP_WAVE) {
-              va[i] = -InvDet * Mq[(size_t)msa * N + i];
-              vb[i] = -InvDet * Mq[(size_t)msb * N + i];
-            }
+        invm_ab = Mq[(size_t)msb * N + msa];
+        wave_sync();
+        // P = X^-T a, Q = X^-T b over the rows of this copy's X^-1: UN rows
+        // of 16-byte loads in flight per lane; lanes past the last column
+        // read a clamped column and discard the result
+        {
+          const double* colp[KD];
 #pragma unroll
-            for (int k = 0; k < KMAX; k++) {
-              const int i = lane + k * RBMPP_WAVE;
-              Ai[k] = Bi[k] = Ci[k] = Di[k] = R1i[k] = R2i[k] = 0.0;
-              if (i < N) {
-                const double p_i = Pq[i], q_i = Qq[i];
-                const double s_i = -InvDet * Mq[(size_t)msa * N + i];
-                const double t_i = -InvDet * Mq[(size_t)msb * N + i];
-                Ai[k] = a_ * q_i + c_ * p_i + eDet * s_i;
-                Bi[k] = b_ * q_i + d_ * p_i - eDet * t_i;
-                Ci[k] = -a_ * t_i - b_ * s_i + fID * p_i;
-                Di[k] = -c_ * t_i - d_ * s_i - fID * q_i;
-                if (i == msa) { Ai[k] += c_; Bi[k] += d_; Ci[k] += fID; }
-                if (i == msb) { Ai[k] += a_; Bi[k] += b_; Di[k] -= fID; }
-                R1i[k] = c_ * t_i + d_ * s_i + fID * q_i;
-                R2i[k] = a_ * t_i + b_ * s_i - fID * p_i;
+          for (int k = 0; k < KD; k++) colp[k] = Mq + min(2 * lane + 128 * k, N - 2);
+          int j = 0;
+          for (; j + UN <= N; j += UN) {
+            double2 mr[UN][KD];
+#pragma unroll
+            for (int u = 0; u < UN; u++)
+#pragma unroll
+              for (int k = 0; k < KD; k++) mr[u][k] = *(const double2*)(colp[k] + (size_t)(j + u) * N);
+#pragma unroll
+            for (int u = 0; u < UN; u++) {
+              const double2 ab = *(const double2*)(wv + 4 * (j + u));
+#pragma unroll
+              for (int k = 0; k < KD; k++) {
+                Pk[k].x = fma(mr[u][k].x, ab.x, Pk[k].x);
+                Pk[k].y = fma(mr[u][k].y, ab.x, Pk[k].y);
+                Qk[k].x = fma(mr[u][k].x, ab.y, Qk[k].x);
+                Qk[k].y = fma(mr[u][k].y, ab.y, Qk[k].y);
               }
             }
           }
-          __syncthreads();
-          if (q < C) {
-            for (int j = 0; j < N; j++) {
-              const double tj = vb[j], sj = va[j], qj = Qq[j], pj = Pq[j];
-              double* row = Mq + (size_t)j * N;
+          for (; j < N; j++) {
+            const double2 ab = *(const double2*)(wv + 4 * j);
 #pragma unroll
-              for (int k = 0; k < KMAX; k++) {
-                const int i = lane + k * RBMPP_WAVE;
-                if (i < N) {
-                  double v = row[i];
-                  v = fma(Ai[k], tj, v);
-                  v = fma(Bi[k], sj, v);
-                  v = fma(Ci[k], qj, v);
-                  v = fma(Di[k], pj, v);
-                  if (j == msa) v -= R1i[k];
-                  if (j == msb) v -= R2i[k];
-                  if (j == msa && i == msb) v -= fID;
-                  if (j == msb && i == msa) v += fID;
-                  row[i] = v;
+            for (int k = 0; k < KD; k++) {
+              const double2 mv = *(const double2*)(colp[k] + (size_t)j * N);
+              Pk[k].x = fma(mv.x, ab.x, Pk[k].x);
+              Pk[k].y = fma(mv.y, ab.x, Pk[k].y);
+              Qk[k].x = fma(mv.x, ab.y, Qk[k].x);
+              Qk[k].y = fma(mv.y, ab.y, Qk[k].y);
+            }
+          }
+        }
+        double part = 0.0;
+#pragma unroll
+        for (int k = 0; k < KD; k++) {
+          const int c = 2 * lane + 128 * k;
+          if (c < N) {
+            wv[4 * c + 2] = Pk[k].x; wv[4 * c + 3] = Qk[k].x;
+            wv[4 * c + 6] = Pk[k].y; wv[4 * c + 7] = Qk[k].y;
+            part = fma(bk[k].x, Pk[k].x, part);
+            part = fma(bk[k].y, Pk[k].y, part);
+          }
+        }
+        bMa = wave_sum(part);
+        wave_sync();
+        pa = wv[4 * msa + 2]; qa = wv[4 * msa + 3];
+        pb = wv[4 * msb + 2]; qb = wv[4 * msb + 3];
+        vec_ba = wv[4 * msa + 1];
+        const double ratio = invm_ab * vec_ba + invm_ab * bMa + pa * qb - pb * qa;
+        rq = ratio;
+        const double pfn = ratio * s.pf[q];
+        const double2 c = psi_wo_pf_exchanged(m, q, s.thc, s.thr, sfac * s.sgn[q >> 1], i1, i2, x1, x2);
+        if (lane == 0) {
+          s.pfn[q] = pfn;
+          s.psiq[q] = cscale(c, pfn);
+        }
+      }
+      __syncthreads();
+      // ---- Metropolis test (make_sample.f90:203-204)
+      double2 psin = make_double2(0.0, 0.0);
+      for (int qq = 0; qq < C; qq++) psin = cadd(psin, s.psiq[qq]);
+      const double r1 = (cabs2(psin) + a.shift) / (cabs2(s.psi) + a.shift);
+      if (rng.u01() < r1 * r1) {
+        // ---- accept: rank-4 update of this copy's X^-1 (calculate_new_InvM_two_child)
+        if (q < C) {
+          const double a_ = -pa, b_ = pb, c_ = qa, d_ = -qb;
+          const double e_ = -bMa - vec_ba;
+          const double f_ = invm_ab;
+          const double Det = a_ * d_ - b_ * c_ - e_ * f_;
+          const double InvDet = 1.0 / Det;
+          const double fID = f_ * InvDet;
+          const double eDet = e_ * Det;
+          double2 A[KD], B[KD], Cc[KD], D[KD], R1[KD], R2[KD];
+#pragma unroll
+          for (int k = 0; k < KD; k++) {
+            const double2 z = make_double2(0.0, 0.0);
+            A[k] = z; B[k] = z; Cc[k] = z; D[k] = z; R1[k] = z; R2[k] = z;
+            const int c = 2 * lane + 128 * k;
+            if (c < N) {
+              // s_i = InvDet InvM(i,msa) = -InvDet InvM(msa,i); t_i likewise
+              const double2 ra = *(const double2*)(Mq + (size_t)msa * N + c);
+              const double2 rb = *(const double2*)(Mq + (size_t)msb * N + c);
+              const double2 si = make_double2(-InvDet * ra.x, -InvDet * ra.y);
+              const double2 ti = make_double2(-InvDet * rb.x, -InvDet * rb.y);
+              wv[4 * c + 0] = si.x; wv[4 * c + 1] = ti.x;
+              wv[4 * c + 4] = si.y; wv[4 * c + 5] = ti.y;
+#define COEF(comp, col)                                                         \
+  {                                                                             \
+    const double p_i = Pk[k].comp, q_i = Qk[k].comp, s_i = si.comp, t_i = ti.comp; \
+    A[k].comp = a_ * q_i + c_ * p_i + eDet * s_i;                               \
+    B[k].comp = b_ * q_i + d_ * p_i - eDet * t_i;                               \
+    Cc[k].comp = -a_ * t_i - b_ * s_i + fID * p_i;                              \
+    D[k].comp = -c_ * t_i - d_ * s_i - fID * q_i;                               \
+    if ((col) == msa) { A[k].comp += c_; B[k].comp += d_; Cc[k].comp += fID; }  \
+    if ((col) == msb) { A[k].comp += a_; B[k].comp += b_; D[k].comp -= fID; }   \
+    R1[k].comp = c_ * t_i + d_ * s_i + fID * q_i;                               \
+    R2[k].comp = a_ * t_i + b_ * s_i - fID * p_i;                               \
+  }
+              COEF(x, c)
+              COEF(y, c + 1)
+#undef COEF
+            }
+          }
+          wave_sync();
+          // rank-4 update of every row (streamed, UN rows in flight) ...
+          {
+            double* colp[KD];
+            bool live[KD];
+#pragma unroll
+            for (int k = 0; k < KD; k++) {
+              const int c = 2 * lane + 128 * k;
+              live[k] = c < N;
+              colp[k] = Mq + min(c, N - 2);
+            }
+            int j = 0;
+            for (; j + UN <= N; j += UN) {
+              double2 mr[UN][KD];
+#pragma unroll
+              for (int u = 0; u < UN; u++)
+#pragma unroll
+                for (int k = 0; k < KD; k++) mr[u][k] = *(const double2*)(colp[k] + (size_t)(j + u) * N);
+#pragma unroll
+              for (int u = 0; u < UN; u++) {
+                const double2 st = *(const double2*)(wv + 4 * (j + u));      // s_j, t_j
+                const double2 pq = *(const double2*)(wv + 4 * (j + u) + 2);  // p_j, q_j
+#pragma unroll
+                for (int k = 0; k < KD; k++) {
+                  double2 v = mr[u][k];
+                  v.x = fma(A[k].x, st.y, v.x); v.y = fma(A[k].y, st.y, v.y);
+                  v.x = fma(B[k].x, st.x, v.x); v.y = fma(B[k].y, st.x, v.y);
+                  v.x = fma(Cc[k].x, pq.y, v.x); v.y = fma(Cc[k].y, pq.y, v.y);
+                  v.x = fma(D[k].x, pq.x, v.x); v.y = fma(D[k].y, pq.x, v.y);
+                  // the reference's update leaves the diagonal exactly zero
+                  // (its terms cancel exactly; here the regrouped FMAs leave
+                  // ~1e-17, which would feed the f_ii derivative)
+                  const int cc = 2 * lane + 128 * k;
+                  if (j + u == cc) v.x = 0.0;
+                  if (j + u == cc + 1) v.y = 0.0;
+                  if (live[k]) *(double2*)(colp[k] + (size_t)(j + u) * N) = v;
                 }
               }
             }
-            // angles of this copy (update_theta)
-            const int g = q >> 1;
-            const double sp = (q & 1) ? -1.0 : 1.0;
-            const int iw1 = m.wmap[g * N + i1], iw2 = m.wmap[g * N + i2];
-            for (int f = lane; f < m.ar; f += RBMPP_WAVE) {
-              double t = s.thr[q * m.ar + f];
-              t = t - 2.0 * m.Wr[f * N + iw1] * (double)x1 * sp;
-              t = t - 2.0 * m.Wr[f * N + iw2] * (double)x2 * sp;
-              s.thr[q * m.ar + f] = t;
+            for (; j < N; j++) {
+              const double2 st = *(const double2*)(wv + 4 * j);
+              const double2 pq = *(const double2*)(wv + 4 * j + 2);
+#pragma unroll
+              for (int k = 0; k < KD; k++) {
+                double2 v = *(const double2*)(colp[k] + (size_t)j * N);
+                v.x = fma(A[k].x, st.y, v.x); v.y = fma(A[k].y, st.y, v.y);
+                v.x = fma(B[k].x, st.x, v.x); v.y = fma(B[k].y, st.x, v.y);
+                v.x = fma(Cc[k].x, pq.y, v.x); v.y = fma(Cc[k].y, pq.y, v.y);
+                v.x = fma(D[k].x, pq.x, v.x); v.y = fma(D[k].y, pq.x, v.y);
+                const int cc = 2 * lane + 128 * k;
+                if (j == cc) v.x = 0.0;
+                if (j == cc + 1) v.y = 0.0;
+                if (live[k]) *(double2*)(colp[k] + (size_t)j * N) = v;
+              }
             }
-            for (int f = lane; f < m.ac; f += RBMPP_WAVE) {
-              double2 t = s.thc[q * m.ac + f];
-              const double2 w1 = m.Wc[f * N + iw1], w2 = m.Wc[f * N + iw2];
-              const double a1 = 2.0 * (double)x1 * sp, a2 = 2.0 * (double)x2 * sp;
-              t.x = t.x - w1.x * a1; t.y = t.y - w1.y * a1;
-              t.x = t.x - w2.x * a2; t.y = t.y - w2.y * a2;
-              s.thc[q * m.ac + f] = t;
+            // ... then the row fix-ups of rows msa, msb and the two corner
+            // entries (pfupdate_two.f90:382-383,397-398)
+#pragma unroll
+            for (int k = 0; k < KD; k++) {
+              const int c = 2 * lane + 128 * k;
+              if (live[k]) {
+                double2 va = *(const double2*)(Mq + (size_t)msa * N + c);
+                double2 vb = *(const double2*)(Mq + (size_t)msb * N + c);
+                va.x -= R1[k].x; va.y -= R1[k].y;
+                vb.x -= R2[k].x; vb.y -= R2[k].y;
+                if (c == msb) va.x -= fID;
+                if (c + 1 == msb) va.y -= fID;
+                if (c == msa) vb.x += fID;
+                if (c + 1 == msa) vb.y += fID;
+                if (c == msa) va.x = 0.0;
+                if (c + 1 == msa) va.y = 0.0;
+                if (c == msb) vb.x = 0.0;
+                if (c + 1 == msb) vb.y = 0.0;
+                *(double2*)(Mq + (size_t)msa * N + c) = va;
+                *(double2*)(Mq + (size_t)msb * N + c) = vb;
+              }
             }
-            if (lane == 0) s.pf[q] = s.pfn[q];
           }
-          __syncthreads();
+          // conditioning guard: a Pf ratio outside (1/g, g) means this copy
+          // passes close to a singular configuration
+          if (a.cond_guard > 0.0 && lane == 0) {
+            const double ar = fabs(rq);
+            if (!(ar * a.cond_guard > 1.0 && ar < a.cond_guard)) s.guard = 1;
+          }
+          // angles of this copy (update_theta)
+          const int g = q >> 1;
+          const double sp = (q & 1) ? -1.0 : 1.0;
+          const int iw1 = m.wmap[g * N + i1], iw2 = m.wmap[g * N + i2];
+          for (int f = lane; f < m.ar; f += RBMPP_WAVE) {
+            double t = s.thr[q * m.ar + f];
+            t = t - 2.0 * m.Wr[f * N + iw1] * (double)x1 * sp;
+            t = t - 2.0 * m.Wr[f * N + iw2] * (double)x2 * sp;
+            s.thr[q * m.ar + f] = t;
+          }
+          for (int f = lane; f < m.ac; f += RBMPP_WAVE) {
+            double2 t = s.thc[q * m.ac + f];
+            const double2 w1 = m.Wc[f * N + iw1], w2 = m.Wc[f * N + iw2];
+            const double a1 = 2.0 * (double)x1 * sp, a2 = 2.0 * (double)x2 * sp;
+            t.x = t.x - w1.x * a1; t.y = t.y - w1.y * a1;
+            t.x = t.x - w2.x * a2; t.y = t.y - w2.y * a2;
+            s.thc[q * m.ac + f] = t;
+          }
+          if (lane == 0) s.pf[q] = s.pfn[q];
         }
+        __syncthreads();
         if (tid == 0) {
           s.x[i1] = -x1;
           s.x[i2] = -x2;
@@ -469,6 +734,17 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
         __syncthreads();
         if (i0 > 0) naccept++;
         nacc_tmp++;
+        if (s.guard) {
+          // a copy passed close to a singular configuration: its
+          // fast-updated X^-1 would lose accuracy
+          wg_recompute(m, s, Mw, gscr, 0, nullptr, nullptr);
+          status = s.status;
+          nrec++;
+          nacc_tmp = 0;
+          if (tid == 0) s.guard = 0;
+          __syncthreads();
+          if (status) break;
+        }
       }
     }
     if (status) break;
@@ -477,6 +753,9 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
       const size_t row = (size_t)w * a.nsamp + (i0 - 1);
       for (int i = tid; i < N; i += nth) a.xs[row * N + i] = (signed char)s.x[i];
       if (tid == 0) a.psis[row] = s.psi;
+      if constexpr (LDSMAT) {
+        if (a.measure) wg_measure_sample(m, s, Mw, row, a);
+      }
     }
   }
   __syncthreads();
@@ -488,7 +767,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
   }
   for (int i = tid; i < C * m.ar; i += nth) a.ws.thr[(size_t)w * m.mr + i] = s.thr[i];
   for (int i = tid; i < C * m.ac; i += nth) a.ws.thc[(size_t)w * m.mc + i] = s.thc[i];
-  for (int q = tid; q < C; q += nth) a.ws.pf[(size_t)w * C + q] = s.pf[q];
+  for (int qq = tid; qq < C; qq += nth) a.ws.pf[(size_t)w * C + qq] = s.pf[qq];
   for (int g = tid; g < P; g += nth) a.ws.sgn[(size_t)w * P + g] = s.sgn[g];
   if (tid == 0) {
     a.ws.psi[w] = s.psi;
@@ -502,17 +781,18 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
 }
 
 template <int NMAX, bool LDSMAT>
-static void launch_sweep_t(const SweepArgs& a, hipStream_t st) {
-  const int nw = a.m.C < 16 ? a.m.C : 16;
-  hipLaunchKernelGGL((k_sweep<NMAX, LDSMAT>), dim3(a.nwalk), dim3(64 * nw), 0, st, a);
+static void launch_sweep_t(int C, const SweepArgs& a, hipStream_t st) {
+  // one wave per projection copy
+  hipLaunchKernelGGL((k_sweep<NMAX, LDSMAT>), dim3(a.nwalk), dim3(64 * C), 0, st, a);
 }
 
-int launch_sweep(const SweepArgs& a, hipStream_t st) {
-  const int N = a.m.N;
-  if (a.m.C > RBMPP_CMAX || a.m.mr > RBMPP_MRMAX || a.m.mc > RBMPP_MCMAX) return -1;
-  if (N <= 64) launch_sweep_t<64, true>(a, st);
-  else if (N <= RBMPP_NLDS) launch_sweep_t<128, true>(a, st);
-  else if (N <= 256) launch_sweep_t<256, false>(a, st);
+int launch_sweep(const DevModel& hm, const SweepArgs& a, hipStream_t st) {
+  const int N = hm.N;
+  if (hm.C > RBMPP_CMAX || hm.mr > RBMPP_MRMAX || hm.mc > RBMPP_MCMAX || (N & 1)) return -1;
+  if (N > RBMPP_NLDS && !a.ws.scratch) return -1;
+  if (N <= 64) launch_sweep_t<64, true>(hm.C, a, st);
+  else if (N <= RBMPP_NLDS) launch_sweep_t<128, true>(hm.C, a, st);
+  else if (N <= 256) launch_sweep_t<256, false>(hm.C, a, st);
   else return -1;
   return 0;
 }

@@ -35,6 +35,6 @@ def test_library_exports_every_declared_symbol():
 def test_config_struct_matches_header():
     from rbm_pp_amd import api
     # rbmpp_config: 14 int32 + 2 double + uint64 + 2 double, no padding
-    assert ctypes.sizeof(api.Config) == 14 * 4 + 2 * 8 + 8 + 2 * 8
+    assert ctypes.sizeof(api.Config) == 14 * 4 + 2 * 8 + 8 + 3 * 8
     assert ctypes.sizeof(api.SrConfig) == 4 * 8 + 2 * 4
     assert ctypes.sizeof(api.SrReport) == 7 * 8 + 2 * 4

@@ -10,6 +10,7 @@ from rbm_pp_amd import lattice as lt
 pytestmark = pytest.mark.gpu
 
 RTOL = 1e-10
+GUARD = 1e4  # the engine's default conditioning guard
 
 
 def _api():
@@ -110,7 +111,7 @@ def test_sampler_trajectory_bit_exact(oracle_lib, L, ar, ac, W, ns, K):
         rng = oracle_lib.Rng("philox", seed, stream=w)
         st = orc.new_state()
         out = orc.make_sample(rng, True, nsample=ns, nwarmup=nwarm, fixed_k=K, ref_draws=False,
-                              state=st)
+                              state=st, cond_guard=GUARD)
         states.append((rng, st))
         assert np.array_equal(x_gpu[w * ns:(w + 1) * ns], out["x"]), f"walker {w}"
         np.testing.assert_allclose(psi_gpu[w * ns:(w + 1) * ns], out["psi"], rtol=1e-9)
@@ -123,7 +124,7 @@ def test_sampler_trajectory_bit_exact(oracle_lib, L, ar, ac, W, ns, K):
     for w in range(W):
         rng, st = states[w]
         out = orc.make_sample(rng, False, nsample=ns, nwarmup=nwarm, fixed_k=K, ref_draws=False,
-                              shift=1e-3, state=st)
+                              shift=1e-3, state=st, cond_guard=GUARD)
         assert np.array_equal(x2[w * ns:(w + 1) * ns], out["x"]), f"walker {w} (second call)"
 
 
@@ -159,6 +160,46 @@ def test_measure_s_and_sr_step(oracle_lib, L, ar, ac):
     assert rep["lambda_"] == pytest.approx(rep_ref[0], rel=1e-8)
     assert _relerr(gam - gam0, gam_ref - gam0) < 1e-8
     np.testing.assert_allclose(eng.gamma(), gam, rtol=0, atol=0)
+
+
+@pytest.mark.parametrize("L,ar,ac", [(4, 2, 1), (6, 2, 0), (10, 4, 0)])
+def test_fused_measurement_matches_recompute(oracle_lib, L, ar, ac):
+    """Measurement fused into the sampler (fast-updated X^-1) against the
+    oracle's calc_phys_quantity_opt (X^-1 recomputed per sample) on the same
+    samples: equal to the fast-update drift (bound 1e-8)."""
+    api = _api()
+    spec, p = _model(L, ar, ac, sigma_w=0.1)
+    W, ns = 8, 6
+    eng = api.Engine(spec, n_walkers=W, n_samples=ns, seed=77, n_warmup=5, recompute_k=16, params=p,
+                     fused_measure=True)
+    shift = 1e-3
+    eng.make_sample(True, shift=shift)
+    got = eng.calc_phys_quantity_opt(shift=shift)
+    xs, _ = eng.get_samples()
+    orc = oracle_lib.Oracle(spec, p)
+    ref = orc.measure(xs, shift=shift, want_s=True)
+    norm = ref["norm"]
+    assert got["norm"] == pytest.approx(norm, rel=1e-8)
+    assert got["Etot"] == pytest.approx(ref["Etot"] / norm, rel=1e-8)
+    assert _relerr(got["gvec"], ref["gvec"] / norm) < 1e-8
+    assert _relerr(got["ovec"], ref["ovec"] / norm) < 1e-8
+    ob = ref["ovec"] / norm
+    S_ref = ref["smat"] / norm - np.outer(ob, ob)
+    S = eng.smat()
+    assert _relerr(S, S_ref) < 1e-8
+    # the f_ii rows of S are exactly zero, as the SR step's checks require
+    # (SR_pdposv.f90:18-40): the fast-updated X^-1 keeps a zero diagonal
+    d = np.diag(S)
+    assert (d[:spec.nv_rbm] > 0).all()
+    assert (d[spec.nv_rbm:] > 0).sum() == spec.n_slater - spec.n_slater // spec.N
+    # a different shift falls back to the recompute path (chain-drawn samples:
+    # the energy sum cancels partly, bound 1e-9)
+    got2 = eng.calc_phys_quantity_opt(shift=2e-3)
+    ref2 = orc.measure(xs, shift=2e-3, want_s=False)
+    assert got2["Etot"] == pytest.approx(ref2["Etot"] / ref2["norm"], rel=1e-9)
+    eng.make_sample(False, shift=shift)
+    eng.calc_phys_quantity_opt(shift=shift)
+    eng.sr_step(eng.gamma(), s_factor0=1e-3)
 
 
 def test_s_build_fp32_error_bound(oracle_lib):
@@ -197,3 +238,30 @@ def test_vmc_energy_4x4_against_enumeration(oracle_lib):
     el = (loc["eloc"] / loc["psi"]).real
     err = el.reshape(W, ns).mean(1).std() / np.sqrt(W)
     assert abs(r["Etot"] - e_exact) < 5 * err + 1e-12
+
+
+def test_guarded_chain_through_near_singular_configuration(oracle_lib):
+    """Walker 354 of this 10x10 workload passes close to a singular
+    configuration of one projection copy in its second call (see the oracle
+    test of the same chain).  With the conditioning guard the GPU chain
+    visits the same configurations as the guarded oracle chain and its
+    amplitudes stay exact to 1e-6."""
+    api = _api()
+    spec = lt.j1j2_model(10, alphar=4, J2=0.5)
+    p = lt.random_params(spec, seed_w=1, seed_f=2, sigma_w=0.01)
+    W, ns, w = 360, 49, 354
+    eng = api.Engine(spec, n_walkers=W, n_samples=ns, seed=1, n_warmup=30, recompute_k=32,
+                     params=p, cond_guard=GUARD)
+    eng.make_sample(True)
+    eng.make_sample(False)
+    xs, psi = eng.get_samples()
+    orc = oracle_lib.Oracle(spec, p)
+    rng = oracle_lib.Rng("philox", 1, stream=w)
+    st = orc.new_state()
+    orc.make_sample(rng, True, nsample=ns, nwarmup=30, fixed_k=32, ref_draws=False, state=st,
+                    cond_guard=GUARD)
+    out = orc.make_sample(rng, False, nsample=ns, nwarmup=30, fixed_k=32, ref_draws=False,
+                          state=st, cond_guard=GUARD)
+    assert np.array_equal(xs[w * ns:(w + 1) * ns], out["x"])
+    exact = eng.local(xs[w * ns:(w + 1) * ns], want_o=False)["psi"]
+    assert _relerr(psi[w * ns:(w + 1) * ns] / exact, np.ones(ns)) < 1e-6

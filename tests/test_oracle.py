@@ -284,3 +284,31 @@ def test_file_formats_round_trip(tmp_path):
     np.testing.assert_allclose(p2.Wc, p.Wc, atol=1e-15)
     g = p.gamma()
     np.testing.assert_array_equal(lt.Params.from_gamma(spec, g).gamma(), g)
+
+
+def test_fast_update_near_singular_chain_and_guard(oracle_lib):
+    """The reference's fast update (pfupdate_two.f90) run through a chain that
+    passes close to a singular configuration of one projection copy: 10x10,
+    alpha=4, walker 354 of Philox key 1, second make_sample call.  Without a
+    guard the chain's Pf of that copy loses all accuracy (relative error of
+    the stored amplitude > 1e10 against a fresh evaluation); with a recompute
+    whenever a copy's Pf ratio leaves (1e-4, 1e4) every stored amplitude
+    stays within 1e-6 of the fresh evaluation."""
+    spec = lt.j1j2_model(10, alphar=4, J2=0.5)
+    p = lt.random_params(spec, seed_w=1, seed_f=2, sigma_w=0.01)
+    orc = oracle_lib.Oracle(spec, p)
+    worst = {}
+    for guard in (0.0, 1e4):
+        rng = oracle_lib.Rng("philox", 1, stream=354)
+        st = orc.new_state()
+        orc.make_sample(rng, True, nsample=49, nwarmup=30, fixed_k=32, ref_draws=False, state=st,
+                        cond_guard=guard)
+        out = orc.make_sample(rng, False, nsample=49, nwarmup=30, fixed_k=32, ref_draws=False,
+                              state=st, cond_guard=guard)
+        err = 0.0
+        for k in range(49):
+            ex = orc.eval(out["x"][k])["psi"][0]
+            err = max(err, abs(out["psi"][k] - ex) / abs(ex))
+        worst[guard] = err
+    assert worst[0.0] > 1e10
+    assert worst[1e4] < 1e-6
