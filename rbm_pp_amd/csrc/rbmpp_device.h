@@ -59,6 +59,24 @@ struct Rng {
   __device__ __forceinline__ int nran0(int n) { return ((int)((double)n * u01())) % n; }
 };
 
+// ------------------------------------------------ raw buffer streaming
+// One copy's N x N block of X^-1 through a wave-uniform buffer resource: the
+// lane's column pair is a VGPR byte offset, the row a scalar offset, so a
+// streamed row costs no 64-bit address arithmetic.  Lanes past the block's
+// width get an offset beyond num_records: the hardware returns zeros for
+// their loads and drops their stores (robust buffer access).
+typedef unsigned int rbmpp_u32x4 __attribute__((ext_vector_type(4)));
+constexpr int RBMPP_OOB = 0x40000000;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t block_rsrc(const double* base, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, bytes, 0x00020000);
+}
+__device__ __forceinline__ double2 bload2(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+}
+__device__ __forceinline__ void bstore2(double2 v, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(rbmpp_u32x4, v), r, voff, soff, 0);
+}
+
 // ----------------------------------------------------------- reductions
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

@@ -368,8 +368,10 @@ __device__ __attribute__((noinline)) void wg_measure_sample(const DevModel& m, S
 // ------------------------------------------------------------------
 // Lane layout of the streaming loops: lane l owns the column pair
 // (2l, 2l+1) of every 128-column block, so one 16-byte load per lane and
-// row covers a whole 100-wide row of X^-1 in one instruction.
-template <int NMAX, bool LDSMAT>
+// row covers a whole 100-wide row of X^-1 in one instruction.  MEAS
+// instantiates the measurement fused into the store of each sample (kept
+// out of the plain sampler so it does not weigh on its register allocation).
+template <int NMAX, bool LDSMAT, bool MEAS>
 __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
   constexpr int KD = NMAX >= 128 ? NMAX / 128 : 1;
   constexpr int UN = 8;  // rows in flight per wave
@@ -511,16 +513,21 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
         // of 16-byte loads in flight per lane; lanes past the last column
         // read a clamped column and discard the result
         {
-          const double* colp[KD];
+          const __amdgpu_buffer_rsrc_t rs = block_rsrc(Mq, N * N * 8);
+          int vo[KD];
 #pragma unroll
-          for (int k = 0; k < KD; k++) colp[k] = Mq + min(2 * lane + 128 * k, N - 2);
+          for (int k = 0; k < KD; k++) {
+            const int c = 2 * lane + 128 * k;
+            vo[k] = c < N ? 8 * c : RBMPP_OOB;
+          }
+          const int rowb = 8 * N;
           int j = 0;
           for (; j + UN <= N; j += UN) {
             double2 mr[UN][KD];
 #pragma unroll
             for (int u = 0; u < UN; u++)
 #pragma unroll
-              for (int k = 0; k < KD; k++) mr[u][k] = *(const double2*)(colp[k] + (size_t)(j + u) * N);
+              for (int k = 0; k < KD; k++) mr[u][k] = bload2(rs, vo[k], (j + u) * rowb);
 #pragma unroll
             for (int u = 0; u < UN; u++) {
               const double2 ab = *(const double2*)(wv + 4 * (j + u));
@@ -537,7 +544,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
             const double2 ab = *(const double2*)(wv + 4 * j);
 #pragma unroll
             for (int k = 0; k < KD; k++) {
-              const double2 mv = *(const double2*)(colp[k] + (size_t)j * N);
+              const double2 mv = bload2(rs, vo[k], j * rowb);
               Pk[k].x = fma(mv.x, ab.x, Pk[k].x);
               Pk[k].y = fma(mv.y, ab.x, Pk[k].y);
               Qk[k].x = fma(mv.x, ab.y, Qk[k].x);
@@ -619,21 +626,23 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
           wave_sync();
           // rank-4 update of every row (streamed, UN rows in flight) ...
           {
-            double* colp[KD];
+            const __amdgpu_buffer_rsrc_t rs = block_rsrc(Mq, N * N * 8);
+            int vo[KD];
             bool live[KD];
 #pragma unroll
             for (int k = 0; k < KD; k++) {
               const int c = 2 * lane + 128 * k;
               live[k] = c < N;
-              colp[k] = Mq + min(c, N - 2);
+              vo[k] = live[k] ? 8 * c : RBMPP_OOB;
             }
+            const int rowb = 8 * N;
             int j = 0;
             for (; j + UN <= N; j += UN) {
               double2 mr[UN][KD];
 #pragma unroll
               for (int u = 0; u < UN; u++)
 #pragma unroll
-                for (int k = 0; k < KD; k++) mr[u][k] = *(const double2*)(colp[k] + (size_t)(j + u) * N);
+                for (int k = 0; k < KD; k++) mr[u][k] = bload2(rs, vo[k], (j + u) * rowb);
 #pragma unroll
               for (int u = 0; u < UN; u++) {
                 const double2 st = *(const double2*)(wv + 4 * (j + u));      // s_j, t_j
@@ -651,7 +660,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
                   const int cc = 2 * lane + 128 * k;
                   if (j + u == cc) v.x = 0.0;
                   if (j + u == cc + 1) v.y = 0.0;
-                  if (live[k]) *(double2*)(colp[k] + (size_t)(j + u) * N) = v;
+                  bstore2(v, rs, vo[k], (j + u) * rowb);
                 }
               }
             }
@@ -660,7 +669,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
               const double2 pq = *(const double2*)(wv + 4 * j + 2);
 #pragma unroll
               for (int k = 0; k < KD; k++) {
-                double2 v = *(const double2*)(colp[k] + (size_t)j * N);
+                double2 v = bload2(rs, vo[k], j * rowb);
                 v.x = fma(A[k].x, st.y, v.x); v.y = fma(A[k].y, st.y, v.y);
                 v.x = fma(B[k].x, st.x, v.x); v.y = fma(B[k].y, st.x, v.y);
                 v.x = fma(Cc[k].x, pq.y, v.x); v.y = fma(Cc[k].y, pq.y, v.y);
@@ -668,7 +677,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
                 const int cc = 2 * lane + 128 * k;
                 if (j == cc) v.x = 0.0;
                 if (j == cc + 1) v.y = 0.0;
-                if (live[k]) *(double2*)(colp[k] + (size_t)j * N) = v;
+                bstore2(v, rs, vo[k], j * rowb);
               }
             }
             // ... then the row fix-ups of rows msa, msb and the two corner
@@ -753,9 +762,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
       const size_t row = (size_t)w * a.nsamp + (i0 - 1);
       for (int i = tid; i < N; i += nth) a.xs[row * N + i] = (signed char)s.x[i];
       if (tid == 0) a.psis[row] = s.psi;
-      if constexpr (LDSMAT) {
-        if (a.measure) wg_measure_sample(m, s, Mw, row, a);
-      }
+      if constexpr (LDSMAT && MEAS) wg_measure_sample(m, s, Mw, row, a);
     }
   }
   __syncthreads();
@@ -783,7 +790,13 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
 template <int NMAX, bool LDSMAT>
 static void launch_sweep_t(int C, const SweepArgs& a, hipStream_t st) {
   // one wave per projection copy
-  hipLaunchKernelGGL((k_sweep<NMAX, LDSMAT>), dim3(a.nwalk), dim3(64 * C), 0, st, a);
+  if constexpr (LDSMAT) {
+    if (a.measure) {
+      hipLaunchKernelGGL((k_sweep<NMAX, LDSMAT, true>), dim3(a.nwalk), dim3(64 * C), 0, st, a);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_sweep<NMAX, LDSMAT, false>), dim3(a.nwalk), dim3(64 * C), 0, st, a);
 }
 
 int launch_sweep(const DevModel& hm, const SweepArgs& a, hipStream_t st) {
