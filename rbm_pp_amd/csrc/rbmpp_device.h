@@ -78,6 +78,12 @@ __device__ __forceinline__ void bstore2(double2 v, __amdgpu_buffer_rsrc_t r, int
 }
 
 // ----------------------------------------------------------- reductions
+// orders LDS traffic between the lanes of one wave
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, RBMPP_WAVE);

@@ -83,3 +83,5 @@ struct LocalArgs {
 void launch_slater(const DevModel& m, const double* f, double* se, int* bad, hipStream_t s);
 int launch_sweep(const DevModel& hm, const SweepArgs& a, hipStream_t s);
 int launch_local(const DevModel& hm, const LocalArgs& a, int grid, hipStream_t s);
+// the N <= RBMPP_NLDS estimator (register Gauss-Jordan, 1024 threads)
+void launch_meas(const LocalArgs& a, int grid, hipStream_t s);

@@ -263,9 +263,8 @@ static void launch_local_t(const LocalArgs& a, int grid, hipStream_t st) {
 int launch_local(const DevModel& hm, const LocalArgs& a, int grid, hipStream_t st) {
   const int N = hm.N;
   if (hm.C > RBMPP_CMAX || hm.mr > RBMPP_MRMAX || hm.mc > RBMPP_MCMAX) return -1;
-  if (hm.nsl > 4 * N) return -1;
-  if (N <= 64) launch_local_t<64, true>(a, grid, st);
-  else if (N <= RBMPP_NLDS) launch_local_t<128, true>(a, grid, st);
+  if (hm.nsl > 4 * N || (N & 1)) return -1;
+  if (N <= RBMPP_NLDS) launch_meas(a, grid, st);
   else if (N <= 256) {
     if (!a.scratch) return -1;
     launch_local_t<256, false>(a, grid, st);

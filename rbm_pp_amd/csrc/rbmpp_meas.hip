@@ -1,0 +1,513 @@
+// Per-configuration estimators for N <= 104 on gfx950: Pf and X^-1 of every
+// projection copy (calculate_MAll, matrix.f90:6-110), the J1-J2 local energy
+// (calc_local_energy_correlation, calc_phys_quantity.f90:306-378) and the
+// log-derivatives O_k (calc_phys_quantity.f90:114-222, slater.f90:94-170).
+// One workgroup of 1024 threads per configuration (grid-stride).
+//
+// Per copy:
+//  * X is gathered into registers, 8 x 2 elements per thread (the matrix
+//    zero-padded to 128 x 128), and inverted there by
+//    Gauss-Jordan exchange steps on 2 x 2 pivot blocks.  Pivots are chosen
+//    logically (the first remaining index c and the remaining r maximising
+//    |X(c, r)|, the Parlett-Reid rule of dskpfa, matrix.f90:133-151) instead
+//    of by swapping rows, so every step costs two workgroup barriers and the
+//    result is X^-1 in the original order.  Each exchange keeps the matrix
+//    exactly skew-symmetric (its products are rounded symmetrically), so
+//    InvM = 0.5 (X^-1(j,i) - X^-1(i,j)) of matrix.f90:97-106 is X^-1 itself.
+//  * The pair-product log-derivatives are scattered from the registers.
+//  * X^-1 goes to LDS once; each wave then takes two antiparallel bonds per
+//    pass, forms P = X^-1 a for both with one read of X^-1, and evaluates the
+//    exchange ratio of pfupdate_two.f90:27-88 with wave reductions.
+#include "rbmpp_kernels.h"
+#include "rbmpp_measure.h"
+
+namespace {
+
+constexpr int MNT = 1024;               // threads per workgroup
+constexpr int MNW = MNT / RBMPP_WAVE;   // waves
+constexpr int MPAD = 128;               // padded order of the register tile
+constexpr int MNMAX = RBMPP_NLDS;       // largest N
+constexpr int MLD = MNMAX + 1;          // LDS leading dimension bound (n + 1)
+
+struct MeasSmem {
+  double inv[MNMAX * MLD];       // inv[j * (n+1) + i] = InvM(j, i) = X^-1(i, j)
+  double2 avec[MNW][MNMAX];      // per wave: a vectors of its two bonds
+  double rowbuf[2][2][MPAD];     // pivot rows c and r, double buffered by step
+  double opp[2][4 * MNMAX];      // pair-product O, re / im
+  double2 psib[4 * MNMAX];       // per antiparallel bond: sum over copies of psi(x')
+  double rat[4 * MNMAX];         // per antiparallel bond: Pf ratio of the current copy
+  double thr[RBMPP_MRMAX];
+  double2 thc[RBMPP_MCMAX];
+  double pf[RBMPP_CMAX];
+  double2 psiq[RBMPP_CMAX], psiwo[RBMPP_CMAX];
+  double selv[2];
+  int sel[2];
+  int x[MNMAX], eidx[MNMAX], ecfg[MNMAX], site[MNMAX];
+  short bi1[4 * MNMAX], bi2[4 * MNMAX];
+  int sgn[RBMPP_CMAX / 2];
+  unsigned sgnbits[RBMPP_CMAX / 2];
+  int nb, status;
+};
+
+// Register tile of the Gauss-Jordan sweep: element (i, j) of the zero-padded
+// 112 x 128 matrix is x[i >> 4][j >> 6] of lane (j & 63) in wave (i & 15);
+// each thread holds RA x RB = 7 x 2 elements.
+constexpr int RA = 7, RB = 2;
+static_assert(16 * RA >= MNMAX && 64 * RB >= MNMAX, "register tile too small");
+
+// The tile is one vector value: a runtime (wave-uniform) row or column index
+// becomes an element extract instead of a dynamically indexed private array,
+// which would live in scratch memory.
+typedef double tile_t __attribute__((ext_vector_type(RA * RB)));
+#define TILE(x, a, b) (x)[(a) * RB + (b)]
+
+__device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ double uniform_d(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+
+// Eliminated indices of the padded order 128, as four wave-uniform words.
+struct Done128 {
+  uint32_t w0, w1, w2, w3;
+  __device__ __forceinline__ uint32_t word(int k) const {
+    return k == 0 ? w0 : (k == 1 ? w1 : (k == 2 ? w2 : w3));
+  }
+  __device__ __forceinline__ int first_free() const {
+    int c = MPAD;
+    if (~w3) c = 96 + __builtin_ctz(~w3);
+    if (~w2) c = 64 + __builtin_ctz(~w2);
+    if (~w1) c = 32 + __builtin_ctz(~w1);
+    if (~w0) c = __builtin_ctz(~w0);
+    return c;
+  }
+  // remaining indices strictly between lo and hi (lo < hi)
+  __device__ __forceinline__ int free_between(int lo, int hi) const {
+    int cnt = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int b0 = max(lo + 1 - 32 * k, 0), b1 = min(hi - 32 * k, 32);
+      if (b1 > b0) {
+        const uint32_t w = b1 - b0 == 32 ? 0xffffffffu : (((1u << (b1 - b0)) - 1u) << b0);
+        cnt += __builtin_popcount(~word(k) & w);
+      }
+    }
+    return cnt;
+  }
+  __device__ __forceinline__ void mark(int i) {
+    const uint32_t bit = 1u << (i & 31);
+    const int k = i >> 5;
+    w0 |= k == 0 ? bit : 0u;
+    w1 |= k == 1 ? bit : 0u;
+    w2 |= k == 2 ? bit : 0u;
+    w3 |= k == 3 ? bit : 0u;
+  }
+};
+
+// Pfaffian and inverse of the skew-symmetric n x n matrix held in the
+// register tile; on return the tile holds X^-1.  Each step eliminates the
+// pivot block {c, r}: with a = X(c, r) the exchange gives
+//   X'(c,r) = -1/a, X'(r,c) = 1/a, X'(c,j) = -X(r,j)/a, X'(r,j) = X(c,j)/a,
+//   X'(i,c) = -X(i,r)/a, X'(i,r) = X(i,c)/a,
+//   X'(i,j) = X(i,j) + (X(i,c) X(r,j) - X(i,r) X(c,j))/a,
+// and Pf(X) = (-1)^{#remaining between c and r} a Pf(Schur complement).
+// Returns 0 for an exactly singular pivot.  Called by all 1024 threads.
+__device__ double reg_skew_pf_inverse(tile_t& x, int n, MeasSmem& s) {
+  const int tid = threadIdx.x, tr = tid >> 6, tc = tid & 63;
+  Done128 done;
+  {
+    uint32_t w[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int lo = 32 * k;
+      w[k] = n >= lo + 32 ? 0u : (n <= lo ? 0xffffffffu : ~((1u << (n - lo)) - 1u));
+    }
+    done = Done128{w[0], w[1], w[2], w[3]};
+  }
+  double pf = 1.0;
+  for (int step = 0; 2 * step < n; step++) {
+    const int sb = step & 1;
+    const int c = done.first_free();
+    // pivot search along row c (held by wave c & 15), which also publishes row c
+    if (tr == (c & 15)) {
+      double bv = -1.0, sv = 0.0;
+      int bi = MPAD;
+#pragma unroll
+      for (int b = 0; b < RB; b++) {
+        const int j = tc + 64 * b;
+        double v = 0.0;
+#pragma unroll
+        for (int aa = 0; aa < RA; aa++)
+          if (tr + 16 * aa == c) v = TILE(x, aa, b);
+        // published with 1 in place of X(c,c) = 0: see the update below
+        s.rowbuf[sb][0][j] = j == c ? 1.0 : v;
+        const uint32_t w = tc < 32 ? done.word(2 * b) : done.word(2 * b + 1);
+        const bool ok = !((w >> (tc & 31)) & 1u) && j != c;
+        if (ok && fabs(v) > bv) { bv = fabs(v); sv = v; bi = j; }
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(bv, o, RBMPP_WAVE);
+        const double osv = __shfl_xor(sv, o, RBMPP_WAVE);
+        const int oi = __shfl_xor(bi, o, RBMPP_WAVE);
+        if (ov > bv || (ov == bv && oi < bi)) { bv = ov; sv = osv; bi = oi; }
+      }
+      if (tc == 0) {
+        s.sel[sb] = bi;
+        s.selv[sb] = sv;
+      }
+    }
+    __syncthreads();
+    const int r = uniform_i(s.sel[sb]);
+    const double a = uniform_d(s.selv[sb]);
+    if (r >= n || a == 0.0) return 0.0;  // uniform
+    // row r = -(column r), published by the lanes holding column r
+#pragma unroll
+    for (int b = 0; b < RB; b++)
+      if (tc + 64 * b == r) {
+#pragma unroll
+        for (int aa = 0; aa < RA; aa++) {
+          const int i = tr + 16 * aa;
+          s.rowbuf[sb][1][i] = i == r ? 1.0 : -TILE(x, aa, b);
+        }
+      }
+    __syncthreads();
+    pf *= (done.free_between(c, r) & 1) ? -a : a;
+    const double inva = 1.0 / a;
+    double rcj[RB], rrj[RB], rci[RA], rri[RA];
+#pragma unroll
+    for (int b = 0; b < RB; b++) {
+      rcj[b] = s.rowbuf[sb][0][tc + 64 * b];
+      rrj[b] = s.rowbuf[sb][1][tc + 64 * b];
+    }
+#pragma unroll
+    for (int aa = 0; aa < RA; aa++) {
+      rci[aa] = s.rowbuf[sb][0][tr + 16 * aa];
+      rri[aa] = s.rowbuf[sb][1][tr + 16 * aa];
+    }
+    // Schur-complement update of every element.  With the published rows
+    // holding 1 at (c,c) and (r,r), the same expression also gives the new
+    // pivot columns: X(i,c) + (X(r,i) * 1 - X(c,i) * X(r,c)) / a = X(r,i)/a =
+    // -X(i,r)/a, and X(i,r) + (X(r,i) * X(c,r) - X(c,i) * 1) / a = X(i,c)/a.
+    // The diagonal stays exactly zero (both products are the same numbers).
+#pragma unroll
+    for (int aa = 0; aa < RA; aa++)
+#pragma unroll
+      for (int b = 0; b < RB; b++) {
+        // rounded products kept apart, so X'(j,i) = -X'(i,j) exactly
+        double t;
+        {
+#pragma clang fp contract(off)
+          t = rri[aa] * rcj[b] - rci[aa] * rrj[b];
+        }
+        TILE(x, aa, b) = fma(t, inva, TILE(x, aa, b));
+      }
+    // The pivot rows are set apart (a wave-uniform test, compile-time tile
+    // indices).
+#pragma unroll
+    for (int aa = 0; aa < RA; aa++) {
+      const int i = tr + 16 * aa;
+      if (i == c) {
+#pragma unroll
+        for (int b = 0; b < RB; b++) {
+          const int j = tc + 64 * b;
+          TILE(x, aa, b) = j == c ? 0.0 : (j == r ? -inva : -rrj[b] * inva);
+        }
+      } else if (i == r) {
+#pragma unroll
+        for (int b = 0; b < RB; b++) {
+          const int j = tc + 64 * b;
+          TILE(x, aa, b) = j == r ? 0.0 : (j == c ? inva : rcj[b] * inva);
+        }
+      }
+    }
+
+    done.mark(c);
+    done.mark(r);
+  }
+  return pf;
+}
+
+// cold helpers kept out of line, so their transcendental temporaries do not
+// weigh on the register allocation of the sweep and bond loops
+__device__ __attribute__((noinline)) double2 exchanged_amp(const DevModel& m, int q, const MeasSmem& s,
+                                                           int sgn, int i1, int i2) {
+  return psi_wo_pf_exchanged(m, q, s.thc, s.thr, sgn, i1, i2, s.x[i1], s.x[i2]);
+}
+__device__ __attribute__((noinline)) void theta_sign(const DevModel& m, MeasSmem& s) { wg_theta_sign(m, s, 0); }
+__device__ __attribute__((noinline)) void o_row(const DevModel& m, const MeasSmem& s, double rw, double* ore,
+                                                double* oim, bool cplx_rows) {
+  wg_orow(m, s, rw, ore, oim, cplx_rows);
+}
+__device__ __attribute__((noinline)) int bonds_of(const DevModel& m, MeasSmem& s) { return bond_list(m, s); }
+__device__ __attribute__((noinline)) double2 eloc_of(const DevModel& m, const MeasSmem& s, double2 p) {
+  return eloc_from_bonds(m, s, p);
+}
+
+__global__ __launch_bounds__(MNT) void k_meas(LocalArgs a) {
+  __shared__ MeasSmem s;
+  const DevModel& m = *a.m;
+  const int tid = threadIdx.x, nth = MNT;
+  const int lane = tid & (RBMPP_WAVE - 1), wid = tid >> 6;
+  const int tr = tid >> 6, tc = tid & 63;  // register tile coordinates
+  const int N = m.N, C = m.C, np = m.npair;
+  const int ld = N + 1;
+  const bool want_eloc = a.mode & 1, want_o = a.mode & 2, want_inv = a.mode & 4, custom = a.mode & 8;
+  // pivot-row entries past the register tile are read but never written
+  for (int k = tid; k < 2 * 2 * MPAD; k += nth) (&s.rowbuf[0][0][0])[k] = 0.0;
+  const bool h0 = lane < N;               // lane owns index lane
+  const bool h1 = lane + RBMPP_WAVE < N;  // lane also owns index lane + 64
+
+  for (int cfg = blockIdx.x; cfg < a.nconf; cfg += gridDim.x) {
+    __syncthreads();
+    for (int i = tid; i < N; i += nth) s.x[i] = a.xs[(size_t)cfg * N + i];
+    if (tid == 0) s.status = WS_OK;
+    for (int k = tid; k < 4 * MNMAX; k += nth) {
+      s.opp[0][k] = 0.0;
+      s.opp[1][k] = 0.0;
+      s.psib[k] = make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      // x_to_Ele_Idx_Ele_Cfg (interface_for_PP.f90:5-33)
+      int u = 0, d = np;
+      for (int i = 0; i < N; i++) {
+        if (s.x[i] == 1) { s.eidx[u] = i; s.ecfg[i] = u; u++; }
+        else { s.eidx[d] = i; s.ecfg[i] = d; d++; }
+      }
+      if (u != np) s.status = WS_INIT;
+      int nb = 0;
+      if (custom) {
+        s.bi1[0] = (short)a.ex_i1[cfg];
+        s.bi2[0] = (short)a.ex_i2[cfg];
+        nb = 1;
+      } else if (want_eloc) {
+        nb = bonds_of(m, s);
+      }
+      s.nb = nb;
+    }
+    __syncthreads();
+    if (s.status != WS_OK) {
+      if (tid == 0 && a.status) a.status[cfg] = s.status;
+      continue;
+    }
+    theta_sign(m, s);
+    for (int q = tid; q < C; q += nth) s.psiwo[q] = psi_wo_pf(m, q, s.thc, s.thr, s.sgn[q >> 1]);
+    const int nb = s.nb;
+
+    for (int q = 0; q < C; q++) {
+      const int g = q >> 1, sidx = q & 1;
+      const double* S = m.se + (size_t)q * 4 * N * N;
+      __syncthreads();  // the previous copy's readers of inv and site are done
+      for (int e = tid; e < N; e += nth) s.site[e] = m.wmap[g * N + s.eidx[e]];
+      // X(i,j) = -Slater_Elm(r_i, r_j) = se[q][r_i][r_j] (matrix.f90:72-78)
+      tile_t x;
+      {
+        int rj[RB];
+#pragma unroll
+        for (int b = 0; b < RB; b++) {
+          const int j = tc + 64 * b;
+          rj[b] = j < N ? s.eidx[j] + (j >= np ? N : 0) : -1;
+        }
+#pragma unroll
+        for (int aa = 0; aa < RA; aa++) {
+          const int i = tr + 16 * aa;
+          const int ri = i < N ? s.eidx[i] + (i >= np ? N : 0) : -1;
+#pragma unroll
+          for (int b = 0; b < RB; b++)
+            TILE(x, aa, b) = (ri >= 0 && rj[b] >= 0) ? S[(size_t)ri * 2 * N + rj[b]] : 0.0;
+        }
+      }
+      const double pfq = reg_skew_pf_inverse(x, N, s);
+      if (pfq == 0.0) {
+        if (tid == 0) s.status = WS_SINGULAR;
+        __syncthreads();
+        break;
+      }
+      // inv[i * ld + j] = InvM(i, j) = X^-1(j, i) = -X^-1(i, j): lanes write
+      // consecutive addresses
+#pragma unroll
+      for (int aa = 0; aa < RA; aa++) {
+        const int i = tr + 16 * aa;
+#pragma unroll
+        for (int b = 0; b < RB; b++) {
+          const int j = tc + 64 * b;
+          if (i < N && j < N) s.inv[i * ld + j] = -TILE(x, aa, b);
+        }
+      }
+      if (want_inv) {
+        // reference layout: out[j*N + i] = InvM(i,j) = X^-1(j,i)
+        double* out = a.invm + ((size_t)cfg * C + q) * N * N;
+#pragma unroll
+        for (int aa = 0; aa < RA; aa++)
+#pragma unroll
+          for (int b = 0; b < RB; b++) {
+            const int i = tr + 16 * aa, j = tc + 64 * b;
+            if (i < N && j < N) out[(size_t)i * N + j] = TILE(x, aa, b);
+          }
+      }
+      if (tid == 0) {
+        s.pf[q] = pfq;
+        s.psiq[q] = cscale(s.psiwo[q], pfq);
+      }
+      if (want_o) {
+        // calculate_derivative_Pfaffian_child (slater.f90:136-167) weighted by
+        // psi_wo_pf(q) (calc_phys_quantity.f90:209-216): entry (u, v) of X^-1
+        // lands in the orbital of (site(v), site(u)) with weight -cs, cc, -ss
+        // or cs by the spins of electrons u and v
+        const double cs = pfq * m.sp_cs[sidx], cc = pfq * m.sp_cc[sidx], ss = pfq * m.sp_ss[sidx];
+        const double2 wq = s.psiwo[q];
+#pragma unroll
+        for (int aa = 0; aa < RA; aa++) {
+          const int u = tr + 16 * aa;
+#pragma unroll
+          for (int b = 0; b < RB; b++) {
+            const int v = tc + 64 * b;
+            if (u < N && v < N) {
+              const double w = u < np ? (v < np ? -cs : -ss) : (v < np ? cc : cs);
+              const double d = w * TILE(x, aa, b);
+              const int o = m.orb[s.site[v] * N + s.site[u]];
+              atomicAdd(&s.opp[0][o], d * wq.x);
+              if (!m.real_only) atomicAdd(&s.opp[1][o], d * wq.y);
+            }
+          }
+        }
+      }
+      __syncthreads();  // inv complete
+      // exchange ratios (calculate_new_PfM_two_child, pfupdate_two.f90:27-88):
+      // wave w takes the bond pairs w, w + 16, ...
+      double2* av = s.avec[wid];
+      for (int p = wid; 2 * p < nb; p += MNW) {
+        const bool two = 2 * p + 1 < nb;
+        int bi[2], i1[2], i2[2], msa[2], msb[2], rsa[2], rsb[2];
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+          bi[t] = (t == 1 && !two) ? 2 * p : 2 * p + t;
+          i1[t] = s.bi1[bi[t]];
+          i2[t] = s.bi2[bi[t]];
+          msa[t] = s.ecfg[i1[t]];
+          msb[t] = s.ecfg[i2[t]];
+          rsa[t] = i2[t] + (msa[t] >= np ? N : 0);  // new row of electron msa
+          rsb[t] = i1[t] + (msb[t] >= np ? N : 0);
+        }
+        // a (to LDS) and b (registers) at electrons j = lane, lane + 64
+        double bv[2][2];
+#pragma unroll
+        for (int hh = 0; hh < 2; hh++) {
+          const int j = lane + RBMPP_WAVE * hh;
+          double av_[2] = {0.0, 0.0};
+          bv[0][hh] = 0.0;
+          bv[1][hh] = 0.0;
+          if (j < N) {
+            const int off = j >= np ? N : 0;
+#pragma unroll
+            for (int t = 0; t < 2; t++) {
+              int tt = s.eidx[j];
+              if (j == msa[t]) tt = i2[t];
+              else if (j == msb[t]) tt = i1[t];
+              const int rr = tt + off;
+              av_[t] = S[(size_t)rsa[t] * 2 * N + rr];
+              bv[t][hh] = S[(size_t)rsb[t] * 2 * N + rr];
+            }
+            av[j] = make_double2(av_[0], av_[1]);
+          }
+        }
+        wave_sync();
+        // P_t(i) = sum_j InvM(j, i) a_t(j) for i = lane, lane + 64
+        double P00 = 0.0, P01 = 0.0, P10 = 0.0, P11 = 0.0;  // P[t][half]
+#pragma unroll 4
+        for (int j = 0; j < N; j++) {
+          const double2 aj = av[j];
+          const double* row = s.inv + j * ld;
+          const double m0 = h0 ? row[lane] : 0.0;
+          const double m1 = h1 ? row[lane + RBMPP_WAVE] : 0.0;
+          P00 = fma(m0, aj.x, P00);
+          P10 = fma(m0, aj.y, P10);
+          P01 = fma(m1, aj.x, P01);
+          P11 = fma(m1, aj.y, P11);
+        }
+        const double Pt[2][2] = {{P00, P01}, {P10, P11}};
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+          if (t == 1 && !two) break;  // wave-uniform
+          // b^T P, and Q(msa), Q(msb) = -sum_j InvM(msa/msb, j) b(j) (skew symmetry)
+          const double* ra = s.inv + msa[t] * ld;
+          const double* rb = s.inv + msb[t] * ld;
+          double bp = bv[t][0] * Pt[t][0];
+          double qa_ = h0 ? ra[lane] * bv[t][0] : 0.0;
+          double qb_ = h0 ? rb[lane] * bv[t][0] : 0.0;
+          if (h1) {
+            bp = fma(bv[t][1], Pt[t][1], bp);
+            qa_ = fma(ra[lane + RBMPP_WAVE], bv[t][1], qa_);
+            qb_ = fma(rb[lane + RBMPP_WAVE], bv[t][1], qb_);
+          }
+          const double bMa = wave_sum(bp);
+          const double qa = -wave_sum(qa_);
+          const double qb = -wave_sum(qb_);
+          // P(msa), P(msb), b(msa) from their owning lanes
+          const double pa0 = __shfl(Pt[t][0], msa[t] & 63, RBMPP_WAVE);
+          const double pa1 = __shfl(Pt[t][1], msa[t] & 63, RBMPP_WAVE);
+          const double pb0 = __shfl(Pt[t][0], msb[t] & 63, RBMPP_WAVE);
+          const double pb1 = __shfl(Pt[t][1], msb[t] & 63, RBMPP_WAVE);
+          const double ba0 = __shfl(bv[t][0], msa[t] & 63, RBMPP_WAVE);
+          const double ba1 = __shfl(bv[t][1], msa[t] & 63, RBMPP_WAVE);
+          const double pa = msa[t] >= RBMPP_WAVE ? pa1 : pa0;
+          const double pb = msb[t] >= RBMPP_WAVE ? pb1 : pb0;
+          const double vec_ba = msa[t] >= RBMPP_WAVE ? ba1 : ba0;
+          if (lane == 0) {
+            const double invm_ab = s.inv[msb[t] * ld + msa[t]];  // InvM(msb, msa)
+            s.rat[bi[t]] = invm_ab * vec_ba + invm_ab * bMa + pa * qb - pb * qa;
+          }
+        }
+        wave_sync();  // av is rewritten by the next pass
+      }
+      __syncthreads();
+      // psi(x') of this copy for every bond, one thread per bond
+      for (int b = tid; b < nb; b += nth) {
+        const int j1 = s.bi1[b], j2 = s.bi2[b];
+        int sfac = -1;
+        if (m.subB[j1]) sfac = -sfac;
+        if (m.subB[j2]) sfac = -sfac;
+        const double pfn = s.rat[b] * pfq;
+        const double2 c = exchanged_amp(m, q, s, sfac * s.sgn[g], j1, j2);
+        s.psib[b] = cadd(s.psib[b], cscale(c, pfn));
+        if (custom && a.ex_pf) a.ex_pf[(size_t)cfg * C + q] = pfn;
+      }
+    }
+    __syncthreads();
+    if (s.status != WS_OK) {
+      if (tid == 0 && a.status) a.status[cfg] = s.status;
+      continue;
+    }
+    // amplitude, local energy, reweighting
+    if (tid == 0) {
+      double2 p = make_double2(0.0, 0.0);
+      for (int q = 0; q < C; q++) p = cadd(p, s.psiq[q]);
+      const double rw = a.shift >= 0.0 ? 1.0 / (cabs2(p) + a.shift) : 1.0;
+      a.psi[cfg] = p;
+      if (a.rw) a.rw[cfg] = rw;
+      if (want_eloc && !custom) a.eloc[cfg] = cscale(eloc_of(m, s, p), rw);
+      if (custom && a.ex_psi) a.ex_psi[cfg] = s.psib[0];
+      s.selv[0] = rw;
+    }
+    for (int q = tid; q < C; q += nth) {
+      if (a.pf) a.pf[(size_t)cfg * C + q] = s.pf[q];
+      if (a.psi_q) a.psi_q[(size_t)cfg * C + q] = s.psiq[q];
+    }
+    __syncthreads();
+    if (want_o) {
+      const bool cplx_rows = !m.real_only;
+      double* ore = a.orows + (size_t)(cplx_rows ? 2 * cfg : cfg) * m.nv;
+      o_row(m, s, s.selv[0], ore, ore + m.nv, cplx_rows);
+    }
+    if (tid == 0 && a.status) a.status[cfg] = WS_OK;
+  }
+}
+
+}  // namespace
+
+void launch_meas(const LocalArgs& a, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_meas, dim3(grid), dim3(MNT), 0, st, a);
+}

@@ -79,11 +79,6 @@ struct SweepSmem {
   unsigned long long ctr;
 };
 
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 template <int NMAX, bool LDSMAT>
 __device__ __attribute__((noinline)) void wg_recompute(const DevModel& m, SweepSmem<NMAX, LDSMAT>& s, double* Mw,
