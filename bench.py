@@ -68,7 +68,8 @@ def main():
     ap.add_argument("--J2", type=float, default=0.5)
     ap.add_argument("--walkers", type=int, default=4096)
     ap.add_argument("--samples", type=int, default=200000, help="stored samples per SR step")
-    ap.add_argument("--therm", type=int, default=200, help="thermalisation sweeps of the first call")
+    ap.add_argument("--therm", type=int, default=200,
+                    help="thermalisation sweeps, run as make_sample launches of the step's shape")
     ap.add_argument("--recompute-k", type=int, default=32)
     ap.add_argument("--cpu-sweeps", type=int, default=600)
     ap.add_argument("--cpu-procs", type=int, default=0)
@@ -79,15 +80,31 @@ def main():
                          "recomputing Pf and X^-1 per sample as the reference does")
     ap.add_argument("--cond-guard", type=float, default=1e4,
                     help="recompute after an accept whose Pf ratio of a copy leaves (1/g, g); 0: off")
+    ap.add_argument("--sampler-only", action="store_true",
+                    help="profiling aid: thermalise, run one make_sample launch and exit")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="profiling aid: three device-to-device copies of MIB MiB and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)",
               file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_calibration:
+        # a plain copy of known size, to scale the HBM byte counters
+        import torch
+        n = args.copy_calibration << 20
+        src = torch.empty(n, dtype=torch.uint8, device="cuda").fill_(1)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n, "copies": 3}))
+        return
+
     # CPU baseline first: worker processes are forked before the GPU is touched
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and not args.sampler_only:
         procs = args.cpu_procs or min(16, os.cpu_count() or 1)
         log(f"cpu baseline: {procs} chains x {args.cpu_sweeps} sweeps (oracle make_sample)")
         rate, wall = cpu_baseline(args.L, args.alphar, args.J2, args.cpu_sweeps, procs)
@@ -104,14 +121,28 @@ def main():
     p = lt.random_params(spec, seed_w=1, seed_f=2, sigma_w=0.01)
     nspw = math.ceil(args.samples / args.walkers)
     eng = api.Engine(spec, n_walkers=args.walkers, n_samples=nspw, seed=1, n_warmup=-1,
-                     n_warmup_first=args.therm, recompute_k=args.recompute_k,
+                     n_warmup_first=-1, recompute_k=args.recompute_k,
                      s_precision=api.S_FP32, params=p, fused_measure=args.fused_measure,
                      cond_guard=args.cond_guard)
     N, Cc = spec.N, spec.C
     log(f"model L={args.L} N={N} copies={Cc} nv={spec.nv} walkers={args.walkers} samples/walker={nspw}")
+    # every k_sweep launch has the shape of a step's: nwarm + 1 + nspw sweeps
+    nwarm = max(30, nspw // 100)
+    sweeps_per_launch_walker = nwarm + 1 + nspw
     t0 = time.perf_counter()
     eng.make_sample(True, shift=0.0)
-    log(f"thermalisation ({args.therm} sweeps/walker): {time.perf_counter() - t0:.1f} s")
+    done = sweeps_per_launch_walker
+    while done < args.therm:
+        eng.make_sample(False, shift=0.0)
+        done += sweeps_per_launch_walker
+    log(f"thermalisation ({done} sweeps/walker): {time.perf_counter() - t0:.1f} s")
+    if args.sampler_only:
+        nacc = eng.make_sample(False, shift=0.0)
+        t = eng.timings()
+        print(json.dumps({"kernel": "k_sweep", "walkers": args.walkers,
+                          "sweeps_per_walker": nwarm + 1 + nspw, "launch_s": t["sample"],
+                          "acceptance": nacc / (args.walkers * nspw * N)}))
+        return
 
     wf_list = [0.0] * 10
     shift = 0.0
@@ -167,8 +198,7 @@ def main():
 
     samples = ns_step * args.steps
     acc = sum(r["nacc"] for r in recs) / samples / N
-    nwarm = max(30, nspw // 100)
-    sweeps_per_launch = args.walkers * (nwarm + 1 + nspw)
+    sweeps_per_launch = args.walkers * sweeps_per_launch_walker
     t_sweep = sum(r["sweep_kernel"] for r in recs) / len(recs)
     # algorithmic HBM bytes per sweep of one walker: each of the N proposals
     # reads every copy's X^-1 (8 C N^2 bytes); an accepted one (fraction A)
@@ -180,6 +210,15 @@ def main():
     # this kernel's own minimum: the update vectors come from the ratio pass,
     # so an accepted move adds only the read and write of the update
     b_min = 8.0 * Cc * N ** 3 * (1.0 + 2.0 * acc)
+    # measured HBM bytes per walker sweep of this kernel and workload
+    # (rocprofv3 FETCH_SIZE and WRITE_SIZE, each scaled by its reading for a
+    # copy of known size), when the committed counter summary covers it
+    traffic = None
+    pmc = os.path.join(ROOT, "profiles", f"k_sweep_j1j2_{args.L}x{args.L}_pmc.md")
+    if os.path.exists(pmc):
+        for line in open(pmc):
+            if line.startswith("traffic_bytes_per_walker_sweep:"):
+                traffic = float(line.split(":")[1]) * sweeps_per_launch
     out = {
         "metric": "MC samples/s + SR-step s, 10x10 J1-J2; mixed-prec vs FP64 at equal energy",
         "value": samples / t_total,
@@ -195,7 +234,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"j1j2_{args.L}x{args.L}_J2_{args.J2}_alphar{args.alphar}_c4v_parity",
                    "walkers": args.walkers, "samples_per_sr_step": ns_step,
-                   "sweeps_per_walker_per_step": nwarm + 1 + nspw, "recompute_k": args.recompute_k,
+                   "sweeps_per_walker_per_step": sweeps_per_launch_walker,
+                   "recompute_k": args.recompute_k,
                    "s_precision": "fp32 products, fp64 centring/shift/solve",
                    "measurement": "fused into sampler" if args.fused_measure else "recompute per sample",
                    "cond_guard": args.cond_guard,
@@ -207,7 +247,7 @@ def main():
         "acceptance": acc,
         "energy_per_site": recs[-1]["E"] / 4 / N,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_sweep", "launch_s": t_sweep,
                      "achieved_min_traffic_GBs": b_min * sweeps_per_launch / t_sweep / 1e9},
         "cpu_baseline": cpu,
