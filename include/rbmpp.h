@@ -153,7 +153,8 @@ int rbmpp_get_samples(const rbmpp_handle* h, int32_t* x, double* psi);
 int64_t rbmpp_n_samples(const rbmpp_handle* h);
 /* Per-sample values of the last measurement (calc_phys_quantity_opt,
  * main.f90:351-367, before the sums): psi[2s+{0,1}], the reweighted local
- * energy eloc[2s+{0,1}] = E_loc * psi-weight, rw[s] = 1/(|psi|^2 + shift).
+ * energy eloc[2s+{0,1}] = rw[s] * sum_x' H psi(x'), rw[s] = 1/(|psi| + shift)
+ * (calc_phys_quantity.f90:85-92).
  * Any pointer may be NULL.  Valid after rbmpp_measure / rbmpp_make_sample
  * with fused measurement. */
 int rbmpp_get_measured(const rbmpp_handle* h, double* psi, double* eloc, double* rw);
@@ -175,8 +176,9 @@ int rbmpp_ratio_exchange(rbmpp_handle* h, int32_t n, const int32_t* x, const int
  * complex, oloc[s][Nv] complex (d psi / d gamma_k); oloc may be NULL. */
 int rbmpp_local(rbmpp_handle* h, int32_t n, const int32_t* x, double* eloc, double* psi,
                 double* oloc);
-/* The centred, scaled-free S = <O O^T>/norm - <O><O>^T of the last
- * rbmpp_measure, Nv x Nv column-major, in the configured precision. */
+/* The centred S = <O O^T>/norm - <O><O>^T of the last rbmpp_measure (before
+ * the SR step's scaling and shift), Nv x Nv column-major, in the configured
+ * precision; no counterpart in the reference (its Smat is the raw sum). */
 int rbmpp_get_smat(rbmpp_handle* h, double* s);
 /* phase timings of the last calls in seconds:
  * {slater, sample, measure, s_build, solve, sr_total, recompute_count, reserved} */

@@ -3,6 +3,7 @@ in-tree with hipcc."""
 from __future__ import annotations
 
 import os
+import shutil
 import subprocess
 import sys
 
@@ -38,7 +39,30 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     subprocess.check_call(["hipcc", "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
+    build_fortran()
     return LIB
+
+
+FDIR = os.path.join(HERE, "fortran")
+DRIVER = os.path.join(FDIR, "rbmpp_vmc")
+
+
+def build_fortran() -> str | None:
+    """The Fortran binding module and the main.f90-shaped driver, linked
+    against librbmpp.so (amdflang; skipped when the image has no Fortran
+    compiler)."""
+    fc = shutil.which("amdflang") or shutil.which("flang")
+    if fc is None:
+        return None
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    mod_o = os.path.join(objdir, "rbmpp_mod.o")
+    drv_o = os.path.join(objdir, "rbmpp_vmc.o")
+    subprocess.check_call([fc, "-O2", "-J", objdir, "-c", os.path.join(FDIR, "rbmpp_mod.f90"), "-o", mod_o])
+    subprocess.check_call([fc, "-O2", "-J", objdir, "-c", os.path.join(FDIR, "rbmpp_vmc.f90"), "-o", drv_o])
+    subprocess.check_call([fc, "-o", DRIVER, drv_o, mod_o, "-L" + HERE, "-lrbmpp",
+                           "-Wl,-rpath,$ORIGIN/.."])
+    return DRIVER
 
 
 if __name__ == "__main__":

@@ -31,7 +31,7 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
 }
 __device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double cabs2(double2 a) { return hypot(a.x, a.y); }
+__device__ __forceinline__ double cmod(double2 a) { return hypot(a.x, a.y); }
 
 // ------------------------------------------------------------- Philox
 // Philox4x32-10 (Salmon et al., SC'11): counter (ctr_lo, ctr_hi, stream, 0),
@@ -273,7 +273,7 @@ __device__ void wg_theta_sign(const DevModel& m, S& s, int check) {
       t.x = t.x + w.x * a;
       t.y = t.y + w.y * a;
     }
-    if (check && cabs2(make_double2(t.x - s.thc[idx].x, t.y - s.thc[idx].y)) > 1.0e-5) s.status = -40;
+    if (check && cmod(make_double2(t.x - s.thc[idx].x, t.y - s.thc[idx].y)) > 1.0e-5) s.status = -40;
     s.thc[idx] = t;
   }
   for (int g = tid; g < P; g += nth) s.sgnbits[g] = 0u;

@@ -485,7 +485,7 @@ __global__ __launch_bounds__(MNT) void k_meas(LocalArgs a) {
     if (tid == 0) {
       double2 p = make_double2(0.0, 0.0);
       for (int q = 0; q < C; q++) p = cadd(p, s.psiq[q]);
-      const double rw = a.shift >= 0.0 ? 1.0 / (cabs2(p) + a.shift) : 1.0;
+      const double rw = a.shift >= 0.0 ? 1.0 / (cmod(p) + a.shift) : 1.0;
       a.psi[cfg] = p;
       if (a.rw) a.rw[cfg] = rw;
       if (want_eloc && !custom) a.eloc[cfg] = cscale(eloc_of(m, s, p), rw);

@@ -204,7 +204,7 @@ __device__ __attribute__((noinline)) int wg_initial_x(const DevModel& m, SweepSm
     wg_x_to_ele(m, s);
     wg_recompute(m, s, Mw, gscratch, 0, nullptr, nullptr);
     if (s.status) return s.status;
-    if (cabs2(s.psi) > 1e-300) return WS_OK;
+    if (cmod(s.psi) > 1e-300) return WS_OK;
   }
   return WS_INIT;
 }
@@ -347,7 +347,7 @@ __device__ __attribute__((noinline)) void wg_measure_sample(const DevModel& m, S
   if (tid == 0) {
     double2 p = make_double2(0.0, 0.0);
     for (int q = 0; q < C; q++) p = cadd(p, s.psiq[q]);
-    const double rw = 1.0 / (cabs2(p) + a.shift);
+    const double rw = 1.0 / (cmod(p) + a.shift);
     a.m_psi[row] = p;
     a.m_rw[row] = rw;
     a.m_eloc[row] = cscale(eloc_from_bonds(m, s, p), rw);
@@ -440,7 +440,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
         wg_x_to_ele(m, s);
         wg_recompute(m, s, Mw, gscr, 0, nullptr, nullptr);
         if (s.status) { status = s.status; break; }
-        const double r1 = (cabs2(s.psi) + a.shift) / (cabs2(psi_old) + a.shift);
+        const double r1 = (cmod(s.psi) + a.shift) / (cmod(psi_old) + a.shift);
         if (!(rng.u01() < r1 * r1)) {
           for (int i = tid; i < N; i += nth) s.x[i] = -s.x[i];
           __syncthreads();
@@ -576,7 +576,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
       // ---- Metropolis test (make_sample.f90:203-204)
       double2 psin = make_double2(0.0, 0.0);
       for (int qq = 0; qq < C; qq++) psin = cadd(psin, s.psiq[qq]);
-      const double r1 = (cabs2(psin) + a.shift) / (cabs2(s.psi) + a.shift);
+      const double r1 = (cmod(psin) + a.shift) / (cmod(s.psi) + a.shift);
       if (rng.u01() < r1 * r1) {
         // ---- accept: rank-4 update of this copy's X^-1 (calculate_new_InvM_two_child)
         if (q < C) {

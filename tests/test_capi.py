@@ -38,3 +38,48 @@ def test_config_struct_matches_header():
     assert ctypes.sizeof(api.Config) == 14 * 4 + 2 * 8 + 8 + 3 * 8
     assert ctypes.sizeof(api.SrConfig) == 4 * 8 + 2 * 4
     assert ctypes.sizeof(api.SrReport) == 7 * 8 + 2 * 4
+
+
+FMOD = os.path.join(ROOT, "rbm_pp_amd", "fortran", "rbmpp_mod.f90")
+
+
+def test_fortran_binding_declares_every_entry_point():
+    src = open(FMOD).read()
+    bound = sorted(set(re.findall(r'bind\(C,\s*name="(rbmpp_[a-z_0-9]+)"\)', src)))
+    assert bound == _declared()
+
+
+_PROBE = """
+program probe
+  use iso_c_binding
+  use rbmpp_mod
+  implicit none
+  type(rbmpp_config), target :: c
+  type(rbmpp_sr_config), target :: s
+  type(rbmpp_sr_report), target :: r
+  integer(c_intptr_t) :: b
+  b = transfer(c_loc(c), b)
+  print '(5I8)', c_sizeof(c), c_sizeof(s), c_sizeof(r), &
+    transfer(c_loc(c%seed), b) - b, transfer(c_loc(c%cond_guard), b) - b
+end program
+"""
+
+
+def test_fortran_struct_layout_matches_c(tmp_path):
+    import shutil
+    import subprocess
+    from rbm_pp_amd import api
+    fc = shutil.which("amdflang") or shutil.which("flang")
+    if fc is None:
+        pytest.skip("no Fortran compiler")
+    if not os.path.exists(LIB):
+        pytest.skip("library not built")
+    (tmp_path / "probe.f90").write_text(_PROBE)
+    subprocess.check_call([fc, "-J", str(tmp_path), "-c", FMOD, "-o", str(tmp_path / "m.o")])
+    libdir = os.path.dirname(LIB)
+    subprocess.check_call([fc, "-J", str(tmp_path), str(tmp_path / "probe.f90"), str(tmp_path / "m.o"),
+                           "-L" + libdir, "-lrbmpp", "-Wl,-rpath," + libdir, "-o", str(tmp_path / "probe")])
+    out = subprocess.check_output([str(tmp_path / "probe")]).split()
+    sizes = [int(t) for t in out]
+    assert sizes == [ctypes.sizeof(api.Config), ctypes.sizeof(api.SrConfig), ctypes.sizeof(api.SrReport),
+                     api.Config.seed.offset, api.Config.cond_guard.offset]
