@@ -54,6 +54,6 @@ def test_fortran_driver_matches_python_mirror(tmp_path):
     np.testing.assert_allclose(Wr, po.Wr, atol=1e-14)
     np.testing.assert_allclose(br, po.br, atol=1e-14)
     f = lt.read_zqp_orbital(str(tmp_path / "zqp_orbital_opt.dat"), spec.n_slater)
-    np.testing.assert_allclose(f, po.f, rtol=1e-13)
+    np.testing.assert_allclose(f, po.f, rtol=0, atol=1e-14)  # e25.15 in the file
     avg = open(tmp_path / "Average_Energy.txt").read().split()
     assert len(avg) == 10 and np.isfinite([float(t) for t in avg]).all()
