@@ -208,7 +208,7 @@ int rbmpp_create(const rbmpp_config* cfg, const int32_t* w_map, const double* pa
   if ((rc = dalloc(h, &ws.x, W * N)) || (rc = dalloc(h, &ws.eidx, W * N)) || (rc = dalloc(h, &ws.ecfg, W * N)) ||
       (rc = dalloc(h, &ws.thr, W * std::max(h->mr, 1))) || (rc = dalloc(h, &ws.thc, W * std::max(h->mc, 1))) ||
       (rc = dalloc(h, &ws.pf, W * C)) || (rc = dalloc(h, &ws.sgn, W * P)) || (rc = dalloc(h, &ws.psi, W)) ||
-      (rc = dalloc(h, &ws.minv, W * C * NN)) || (rc = dalloc(h, &ws.nacc_tmp, W)) ||
+      (rc = dalloc(h, &ws.minv, W * C * (NN / 2))) || (rc = dalloc(h, &ws.nacc_tmp, W)) ||
       (rc = dalloc(h, &ws.nfastup, W)) || (rc = dalloc(h, &ws.ctr, W)) || (rc = dalloc(h, &ws.status, W)) ||
       (rc = dalloc(h, &ws.naccept, W)) || (rc = dalloc(h, &ws.nrecompute, W)))
     return rc;
@@ -744,12 +744,22 @@ int rbmpp_get_walkers(const rbmpp_handle* h0, int32_t* x, double* pf, double* ps
   if (pf) HIPCHK(h, hipMemcpy(pf, h->ws.pf, sizeof(double) * W * C, hipMemcpyDeviceToHost));
   if (psi) HIPCHK(h, hipMemcpy(psi, h->ws.psi, sizeof(double2) * W, hipMemcpyDeviceToHost));
   if (invm) {
-    // device row-major M[r][c] = InvM(r,c) -> reference layout InvM(i,j) at j*N+i
-    std::vector<double> m(W * C * N * N);
+    // folded device slots ((d-1)*N + r holds InvM(r, (r+d) mod N), d = 1..N/2)
+    // -> reference layout InvM(i,j) at j*N+i
+    const size_t H = N / 2, FS = N * H;
+    std::vector<double> m(W * C * FS);
     HIPCHK(h, hipMemcpy(m.data(), h->ws.minv, sizeof(double) * m.size(), hipMemcpyDeviceToHost));
-    for (size_t b = 0; b < W * C; b++)
-      for (size_t r = 0; r < N; r++)
-        for (size_t c2 = 0; c2 < N; c2++) invm[b * N * N + c2 * N + r] = m[b * N * N + r * N + c2];
+    for (size_t b = 0; b < W * C; b++) {
+      double* out = invm + b * N * N;
+      for (size_t r = 0; r < N; r++) out[r * N + r] = 0.0;
+      for (size_t d = 1; d <= H; d++)
+        for (size_t r = 0; r < (d == H ? H : N); r++) {
+          const size_t c2 = (r + d) % N;
+          const double v = m[b * FS + (d - 1) * N + r];
+          out[c2 * N + r] = v;   // InvM(r, c2)
+          out[r * N + c2] = -v;  // InvM(c2, r)
+        }
+    }
   }
   if (naccept_per_walker) {
     std::vector<long long> v(W);

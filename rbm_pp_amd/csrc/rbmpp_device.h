@@ -76,6 +76,40 @@ __device__ __forceinline__ double2 bload2(__amdgpu_buffer_rsrc_t r, int voff, in
 __device__ __forceinline__ void bstore2(double2 v, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
   __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(rbmpp_u32x4, v), r, voff, soff, 0);
 }
+__device__ __forceinline__ double bload1(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ __forceinline__ void bstore1(double v, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
+}
+
+// ------------------------------------------ folded skew-symmetric storage
+// A skew-symmetric N x N matrix M (N divisible by 4) stored as its strictly
+// upper part in cyclic-diagonal order: N/2 rows of N doubles, slot
+// (d-1)*N + i holding M(i, (i+d) mod N) for d = 1..N/2; in the last row
+// (d = N/2) only i < N/2 is used, the pair (i, i+N/2) being stored once.
+// M(i,i) = 0 and M(j,i) = -M(i,j) are implied, so the two triangles cannot
+// drift apart and a streamed pass moves half the bytes of the full matrix.
+__device__ __forceinline__ size_t fold_size(int N) { return (size_t)N * (N / 2); }
+// slot of the pair (r, c), r != c; *neg set when the slot holds M(c, r)
+__device__ __forceinline__ int fold_slot(int N, int r, int c, bool* neg) {
+  const int H = N >> 1;
+  int d = c - r;
+  if (d < 0) d += N;
+  if (d < H || (d == H && r < H)) {
+    *neg = false;
+    return (d - 1) * N + r;
+  }
+  *neg = true;
+  return (N - d - 1) * N + c;
+}
+__device__ __forceinline__ double fold_get(const double* G, int N, int r, int c) {
+  if (r == c) return 0.0;
+  bool neg;
+  const double v = G[fold_slot(N, r, c, &neg)];
+  return neg ? -v : v;
+}
 
 // ----------------------------------------------------------- reductions
 // orders LDS traffic between the lanes of one wave
@@ -94,6 +128,16 @@ __device__ __forceinline__ double wave_max(double v) {
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, RBMPP_WAVE));
   return v;
 }
+
+// The RBM tables an amplitude update reads: the model's own (global memory)
+// or copies staged in LDS by the kernel
+struct RbmTables {
+  const double* Wr;
+  const double2* Wc;
+  const int* wmap;
+  const unsigned char* subB;
+};
+__device__ __forceinline__ RbmTables model_tables(const DevModel& m) { return {m.Wr, m.Wc, m.wmap, m.subB}; }
 
 // amplitude of projection copy q without the Pfaffian (sub.f90:30-48), from
 // angle arrays thc[q*ac+f], thr[q*ar+f] and the sign of operation g.
@@ -119,17 +163,17 @@ __device__ __forceinline__ double2 psi_wo_pf(const DevModel& m, int q, const dou
 
 // Same, for the configuration after exchanging the spins at sites i1, i2
 // (update_theta, RBM.f90:95-148): angles are updated on the fly.
-__device__ __forceinline__ double2 psi_wo_pf_exchanged(const DevModel& m, int q, const double2* thc,
-                                                       const double* thr, int sgn_new, int i1, int i2,
-                                                       int x1, int x2) {
+__device__ __forceinline__ double2 psi_wo_pf_exchanged(const DevModel& m, const RbmTables& tb, int q,
+                                                       const double2* thc, const double* thr, int sgn_new,
+                                                       int i1, int i2, int x1, int x2) {
   int g = q >> 1, sidx = q & 1;
   double s = sidx ? -1.0 : 1.0;
-  int iw1 = m.wmap[g * m.N + i1], iw2 = m.wmap[g * m.N + i2];
+  int iw1 = tb.wmap[g * m.N + i1], iw2 = tb.wmap[g * m.N + i2];
   double2 c = make_double2(1.0, 0.0);
   for (int k = 0; k < m.ac; k++) c = cscale(c, 0.5);
   for (int f = 0; f < m.ac; f++) {
     double2 t = thc[q * m.ac + f];
-    double2 w1 = m.Wc[f * m.N + iw1], w2 = m.Wc[f * m.N + iw2];
+    double2 w1 = tb.Wc[f * m.N + iw1], w2 = tb.Wc[f * m.N + iw2];
     double a1 = 2.0 * (double)x1 * s, a2 = 2.0 * (double)x2 * s;
     t.x = t.x - w1.x * a1; t.y = t.y - w1.y * a1;
     t.x = t.x - w2.x * a2; t.y = t.y - w2.y * a2;
@@ -140,8 +184,8 @@ __device__ __forceinline__ double2 psi_wo_pf_exchanged(const DevModel& m, int q,
   double r1 = 1.0;
   for (int f = 0; f < m.ar; f++) {
     double t = thr[q * m.ar + f];
-    t = t - 2.0 * m.Wr[f * m.N + iw1] * (double)x1 * s;
-    t = t - 2.0 * m.Wr[f * m.N + iw2] * (double)x2 * s;
+    t = t - 2.0 * tb.Wr[f * m.N + iw1] * (double)x1 * s;
+    t = t - 2.0 * tb.Wr[f * m.N + iw2] * (double)x2 * s;
     r1 = r1 * cosh(t);
   }
   c = cscale(c, r1);
@@ -149,6 +193,12 @@ __device__ __forceinline__ double2 psi_wo_pf_exchanged(const DevModel& m, int q,
   c = cmul(c, m.chi[g]);
   c = cscale(c, m.sp_w[sidx]);
   return c;
+}
+
+__device__ __forceinline__ double2 psi_wo_pf_exchanged(const DevModel& m, int q, const double2* thc,
+                                                       const double* thr, int sgn_new, int i1, int i2,
+                                                       int x1, int x2) {
+  return psi_wo_pf_exchanged(m, model_tables(m), q, thc, thr, sgn_new, i1, i2, x1, x2);
 }
 
 // --------------------------------------------------------------------

@@ -28,7 +28,7 @@ struct WalkerState {
   double* pf;              // [W][C]
   int* sgn;                // [W][P]
   double2* psi;            // [W]
-  double* minv;            // [W][C][N][N]  minv[w][q][r*N+c] = InvM(r,c) of copy q
+  double* minv;            // [W][C][N*N/2] InvM of copy q in folded skew storage (fold_slot)
   double* scratch;         // [W][N][N]     recompute matrix when N > RBMPP_NLDS
   int* nacc_tmp;           // [W] accepts since the last recompute
   int* nfastup;            // [W] adaptive recompute period

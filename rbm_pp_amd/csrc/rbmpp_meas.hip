@@ -20,12 +20,14 @@
 //    exchange ratio of pfupdate_two.f90:27-88 with wave reductions.
 #include "rbmpp_kernels.h"
 #include "rbmpp_measure.h"
+#include "rbmpp_skew.h"
 
 namespace {
 
+using namespace rbmpp_skew;
+
 constexpr int MNT = 1024;               // threads per workgroup
 constexpr int MNW = MNT / RBMPP_WAVE;   // waves
-constexpr int MPAD = 128;               // padded order of the register tile
 constexpr int MNMAX = RBMPP_NLDS;       // largest N
 constexpr int MLD = MNMAX + 1;          // LDS leading dimension bound (n + 1)
 
@@ -48,187 +50,6 @@ struct MeasSmem {
   unsigned sgnbits[RBMPP_CMAX / 2];
   int nb, status;
 };
-
-// Register tile of the Gauss-Jordan sweep: element (i, j) of the zero-padded
-// 112 x 128 matrix is x[i >> 4][j >> 6] of lane (j & 63) in wave (i & 15);
-// each thread holds RA x RB = 7 x 2 elements.
-constexpr int RA = 7, RB = 2;
-static_assert(16 * RA >= MNMAX && 64 * RB >= MNMAX, "register tile too small");
-
-// The tile is one vector value: a runtime (wave-uniform) row or column index
-// becomes an element extract instead of a dynamically indexed private array,
-// which would live in scratch memory.
-typedef double tile_t __attribute__((ext_vector_type(RA * RB)));
-#define TILE(x, a, b) (x)[(a) * RB + (b)]
-
-__device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
-__device__ __forceinline__ double uniform_d(double v) {
-  const long long b = __builtin_bit_cast(long long, v);
-  const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
-  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
-  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
-}
-
-// Eliminated indices of the padded order 128, as four wave-uniform words.
-struct Done128 {
-  uint32_t w0, w1, w2, w3;
-  __device__ __forceinline__ uint32_t word(int k) const {
-    return k == 0 ? w0 : (k == 1 ? w1 : (k == 2 ? w2 : w3));
-  }
-  __device__ __forceinline__ int first_free() const {
-    int c = MPAD;
-    if (~w3) c = 96 + __builtin_ctz(~w3);
-    if (~w2) c = 64 + __builtin_ctz(~w2);
-    if (~w1) c = 32 + __builtin_ctz(~w1);
-    if (~w0) c = __builtin_ctz(~w0);
-    return c;
-  }
-  // remaining indices strictly between lo and hi (lo < hi)
-  __device__ __forceinline__ int free_between(int lo, int hi) const {
-    int cnt = 0;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const int b0 = max(lo + 1 - 32 * k, 0), b1 = min(hi - 32 * k, 32);
-      if (b1 > b0) {
-        const uint32_t w = b1 - b0 == 32 ? 0xffffffffu : (((1u << (b1 - b0)) - 1u) << b0);
-        cnt += __builtin_popcount(~word(k) & w);
-      }
-    }
-    return cnt;
-  }
-  __device__ __forceinline__ void mark(int i) {
-    const uint32_t bit = 1u << (i & 31);
-    const int k = i >> 5;
-    w0 |= k == 0 ? bit : 0u;
-    w1 |= k == 1 ? bit : 0u;
-    w2 |= k == 2 ? bit : 0u;
-    w3 |= k == 3 ? bit : 0u;
-  }
-};
-
-// Pfaffian and inverse of the skew-symmetric n x n matrix held in the
-// register tile; on return the tile holds X^-1.  Each step eliminates the
-// pivot block {c, r}: with a = X(c, r) the exchange gives
-//   X'(c,r) = -1/a, X'(r,c) = 1/a, X'(c,j) = -X(r,j)/a, X'(r,j) = X(c,j)/a,
-//   X'(i,c) = -X(i,r)/a, X'(i,r) = X(i,c)/a,
-//   X'(i,j) = X(i,j) + (X(i,c) X(r,j) - X(i,r) X(c,j))/a,
-// and Pf(X) = (-1)^{#remaining between c and r} a Pf(Schur complement).
-// Returns 0 for an exactly singular pivot.  Called by all 1024 threads.
-__device__ double reg_skew_pf_inverse(tile_t& x, int n, MeasSmem& s) {
-  const int tid = threadIdx.x, tr = tid >> 6, tc = tid & 63;
-  Done128 done;
-  {
-    uint32_t w[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const int lo = 32 * k;
-      w[k] = n >= lo + 32 ? 0u : (n <= lo ? 0xffffffffu : ~((1u << (n - lo)) - 1u));
-    }
-    done = Done128{w[0], w[1], w[2], w[3]};
-  }
-  double pf = 1.0;
-  for (int step = 0; 2 * step < n; step++) {
-    const int sb = step & 1;
-    const int c = done.first_free();
-    // pivot search along row c (held by wave c & 15), which also publishes row c
-    if (tr == (c & 15)) {
-      double bv = -1.0, sv = 0.0;
-      int bi = MPAD;
-#pragma unroll
-      for (int b = 0; b < RB; b++) {
-        const int j = tc + 64 * b;
-        double v = 0.0;
-#pragma unroll
-        for (int aa = 0; aa < RA; aa++)
-          if (tr + 16 * aa == c) v = TILE(x, aa, b);
-        // published with 1 in place of X(c,c) = 0: see the update below
-        s.rowbuf[sb][0][j] = j == c ? 1.0 : v;
-        const uint32_t w = tc < 32 ? done.word(2 * b) : done.word(2 * b + 1);
-        const bool ok = !((w >> (tc & 31)) & 1u) && j != c;
-        if (ok && fabs(v) > bv) { bv = fabs(v); sv = v; bi = j; }
-      }
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) {
-        const double ov = __shfl_xor(bv, o, RBMPP_WAVE);
-        const double osv = __shfl_xor(sv, o, RBMPP_WAVE);
-        const int oi = __shfl_xor(bi, o, RBMPP_WAVE);
-        if (ov > bv || (ov == bv && oi < bi)) { bv = ov; sv = osv; bi = oi; }
-      }
-      if (tc == 0) {
-        s.sel[sb] = bi;
-        s.selv[sb] = sv;
-      }
-    }
-    __syncthreads();
-    const int r = uniform_i(s.sel[sb]);
-    const double a = uniform_d(s.selv[sb]);
-    if (r >= n || a == 0.0) return 0.0;  // uniform
-    // row r = -(column r), published by the lanes holding column r
-#pragma unroll
-    for (int b = 0; b < RB; b++)
-      if (tc + 64 * b == r) {
-#pragma unroll
-        for (int aa = 0; aa < RA; aa++) {
-          const int i = tr + 16 * aa;
-          s.rowbuf[sb][1][i] = i == r ? 1.0 : -TILE(x, aa, b);
-        }
-      }
-    __syncthreads();
-    pf *= (done.free_between(c, r) & 1) ? -a : a;
-    const double inva = 1.0 / a;
-    double rcj[RB], rrj[RB], rci[RA], rri[RA];
-#pragma unroll
-    for (int b = 0; b < RB; b++) {
-      rcj[b] = s.rowbuf[sb][0][tc + 64 * b];
-      rrj[b] = s.rowbuf[sb][1][tc + 64 * b];
-    }
-#pragma unroll
-    for (int aa = 0; aa < RA; aa++) {
-      rci[aa] = s.rowbuf[sb][0][tr + 16 * aa];
-      rri[aa] = s.rowbuf[sb][1][tr + 16 * aa];
-    }
-    // Schur-complement update of every element.  With the published rows
-    // holding 1 at (c,c) and (r,r), the same expression also gives the new
-    // pivot columns: X(i,c) + (X(r,i) * 1 - X(c,i) * X(r,c)) / a = X(r,i)/a =
-    // -X(i,r)/a, and X(i,r) + (X(r,i) * X(c,r) - X(c,i) * 1) / a = X(i,c)/a.
-    // The diagonal stays exactly zero (both products are the same numbers).
-#pragma unroll
-    for (int aa = 0; aa < RA; aa++)
-#pragma unroll
-      for (int b = 0; b < RB; b++) {
-        // rounded products kept apart, so X'(j,i) = -X'(i,j) exactly
-        double t;
-        {
-#pragma clang fp contract(off)
-          t = rri[aa] * rcj[b] - rci[aa] * rrj[b];
-        }
-        TILE(x, aa, b) = fma(t, inva, TILE(x, aa, b));
-      }
-    // The pivot rows are set apart (a wave-uniform test, compile-time tile
-    // indices).
-#pragma unroll
-    for (int aa = 0; aa < RA; aa++) {
-      const int i = tr + 16 * aa;
-      if (i == c) {
-#pragma unroll
-        for (int b = 0; b < RB; b++) {
-          const int j = tc + 64 * b;
-          TILE(x, aa, b) = j == c ? 0.0 : (j == r ? -inva : -rrj[b] * inva);
-        }
-      } else if (i == r) {
-#pragma unroll
-        for (int b = 0; b < RB; b++) {
-          const int j = tc + 64 * b;
-          TILE(x, aa, b) = j == r ? 0.0 : (j == c ? inva : rcj[b] * inva);
-        }
-      }
-    }
-
-    done.mark(c);
-    done.mark(r);
-  }
-  return pf;
-}
 
 // cold helpers kept out of line, so their transcendental temporaries do not
 // weigh on the register allocation of the sweep and bond loops
@@ -320,7 +141,7 @@ __global__ __launch_bounds__(MNT) void k_meas(LocalArgs a) {
             TILE(x, aa, b) = (ri >= 0 && rj[b] >= 0) ? S[(size_t)ri * 2 * N + rj[b]] : 0.0;
         }
       }
-      const double pfq = reg_skew_pf_inverse(x, N, s);
+      const double pfq = reg_skew_pf_inverse(x, N, &s.rowbuf[0][0][0], s.sel, s.selv);
       if (pfq == 0.0) {
         if (tid == 0) s.status = WS_SINGULAR;
         __syncthreads();

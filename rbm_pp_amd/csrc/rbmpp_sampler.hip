@@ -3,14 +3,17 @@
 // One workgroup owns one walker (Markov chain) for a whole make_sample call
 // (make_sample.f90:4-240): thermalisation and stored sweeps run inside one
 // launch.  Wave w of the workgroup owns projection copy q = w: per proposal
-// it streams that copy's X^-1 (N x N FP64) once to form P = X^-T a and
-// Q = X^-T b, which give the Pfaffian ratio of the spin exchange
-// (pfupdate_two.f90:27-88); on acceptance the same wave applies the rank-4
-// antisymmetric update (pfupdate_two.f90:113-224) in one read-modify-write
-// pass, reusing P and Q.  All threads draw the same Philox numbers, so the
-// control flow is uniform across the workgroup.
+// it streams that copy's X^-1 once to form P = X^-T a and Q = X^-T b, which
+// give the Pfaffian ratio of the spin exchange (pfupdate_two.f90:27-88); on
+// acceptance the same wave applies the rank-4 antisymmetric update
+// (pfupdate_two.f90:113-224) in one read-modify-write pass, reusing P and Q.
+// X^-1 is held in folded skew storage (N^2/2 FP64, rbmpp_device.h), so each
+// pass moves half the bytes of the full matrix and the walkers resident on
+// the chip keep their X^-1 within the last-level cache.  All threads draw the
+// same Philox numbers, so the control flow is uniform across the workgroup.
 #include "rbmpp_kernels.h"
 #include "rbmpp_measure.h"
+#include "rbmpp_skew.h"
 
 // ------------------------------------------------------------------
 // update_slater_elements (slater.f90:7-64).  se[q][c][r] = Slater_Elm(r,c,q).
@@ -76,6 +79,8 @@ struct SweepSmem {
   int redi[32];
   int piv, flag, status;
   int guard;  // set by a copy whose updated X^-1 exceeds the conditioning bound
+  int gsel[2];     // pivot index / value of the register Gauss-Jordan
+  double gselv[2];
   unsigned long long ctr;
 };
 
@@ -86,42 +91,100 @@ __device__ __attribute__((noinline)) void wg_recompute(const DevModel& m, SweepS
   const int tid = threadIdx.x, nth = blockDim.x;
   const int N = m.N, C = m.C, P = m.P;
   wg_theta_sign(m, s, check);
-  // Pf and X^-1 of each copy
+  // Pf and X^-1 of each copy: with 16 waves and N <= 112 the register-tile
+  // Gauss-Jordan (rbmpp_skew.h), otherwise row-swapping Gauss-Jordan on a
+  // matrix in LDS (or global scratch)
+  const bool regpath = LDSMAT && nth == 1024 && N <= 112;
   double* A = LDSMAT ? s.u : gscratch;
   const int ld = N;
   double dpf = 0.0, dinv = 0.0;
+  if (regpath) {
+    for (int k = tid; k < 4 * rbmpp_skew::MPAD; k += nth) s.u[k] = 0.0;  // pivot-row buffers
+  }
+  const int H = N >> 1;
   for (int q = 0; q < C; q++) {
     const double* S = m.se + (size_t)q * 4 * N * N;
+    double* Mq = Mw + (size_t)q * fold_size(N);
+    double lmax = 0.0, ldiff = 0.0, pfq = 0.0;
     __syncthreads();
-    // X(i,j) = -Slater_Elm(r_i, r_j) = Slater_Elm(r_j, r_i) (matrix.f90:72-78)
-    for (int e = tid; e < N * N; e += nth) {
-      const int i = e / N, j = e - i * N;
-      const int ri = s.eidx[i] + (i >= m.npair ? N : 0);
-      const int rj = s.eidx[j] + (j >= m.npair ? N : 0);
-      A[(size_t)i * ld + j] = S[(size_t)ri * 2 * N + rj];
-    }
-    __syncthreads();
-    const double pfq = wg_skew_inverse(A, N, ld, s.perm, s.gjbuf, s.red, s.redi, &s.piv);
-    if (pfq == 0.0) {
-      if (tid == 0) s.status = WS_SINGULAR;
+    if (regpath) {
+      using namespace rbmpp_skew;
+      const int tr = tid >> 6, tc = tid & 63;
+      tile_t x;
+      int rj[RB];
+#pragma unroll
+      for (int b = 0; b < RB; b++) {
+        const int j = tc + 64 * b;
+        rj[b] = j < N ? s.eidx[j] + (j >= m.npair ? N : 0) : -1;
+      }
+#pragma unroll
+      for (int aa = 0; aa < RA; aa++) {
+        const int i = tr + 16 * aa;
+        const int ri = i < N ? s.eidx[i] + (i >= m.npair ? N : 0) : -1;
+#pragma unroll
+        for (int b = 0; b < RB; b++)
+          TILE(x, aa, b) = (ri >= 0 && rj[b] >= 0) ? S[(size_t)ri * 2 * N + rj[b]] : 0.0;
+      }
+      pfq = reg_skew_pf_inverse(x, N, s.u, s.gsel, s.gselv);
+      if (pfq == 0.0) {
+        if (tid == 0) s.status = WS_SINGULAR;
+        __syncthreads();
+        return;
+      }
+      // M(r,c) = InvM(r,c) = X^-1(c,r) = -X^-1(r,c) (exactly skew); the
+      // element of each pair's folded slot is written by its holder
+#pragma unroll
+      for (int aa = 0; aa < RA; aa++) {
+        const int r = tr + 16 * aa;
+#pragma unroll
+        for (int b = 0; b < RB; b++) {
+          const int c = tc + 64 * b;
+          if (r < N && c < N) {
+            const double v = -TILE(x, aa, b);
+            lmax = fmax(lmax, fabs(v));
+            int d = c - r;
+            if (d < 0) d += N;
+            if (d > 0 && (d < H || (d == H && r < H))) {
+              const size_t slot = (size_t)(d - 1) * N + r;
+              if (check) ldiff = fmax(ldiff, fabs(Mq[slot] - v));
+              Mq[slot] = v;
+            }
+          }
+        }
+      }
+    } else {
+      // X(i,j) = -Slater_Elm(r_i, r_j) = Slater_Elm(r_j, r_i) (matrix.f90:72-78)
+      for (int e = tid; e < N * N; e += nth) {
+        const int i = e / N, j = e - i * N;
+        const int ri = s.eidx[i] + (i >= m.npair ? N : 0);
+        const int rj = s.eidx[j] + (j >= m.npair ? N : 0);
+        A[(size_t)i * ld + j] = S[(size_t)ri * 2 * N + rj];
+      }
       __syncthreads();
-      return;
-    }
-    // inverse permutation into gjbuf (as ints)
-    int* ip = (int*)s.gjbuf;
-    for (int r = tid; r < N; r += nth) ip[s.perm[r]] = r;
-    __syncthreads();
-    // M[j][i] = InvM(j,i) = 0.5 (Xinv(i,j) - Xinv(j,i))   (matrix.f90:97-106)
-    double* Mq = Mw + (size_t)q * N * N;
-    double lmax = 0.0, ldiff = 0.0;
-    for (int e = tid; e < N * N; e += nth) {
-      const int j = e / N, i = e - j * N;
-      const double xij = A[(size_t)ip[i] * ld + ip[j]];
-      const double xji = A[(size_t)ip[j] * ld + ip[i]];
-      const double v = 0.5 * (xij - xji);
-      if (check) ldiff = fmax(ldiff, fabs(Mq[e] - v));
-      lmax = fmax(lmax, fabs(v));
-      Mq[e] = v;
+      pfq = wg_skew_inverse(A, N, ld, s.perm, s.gjbuf, s.red, s.redi, &s.piv);
+      if (pfq == 0.0) {
+        if (tid == 0) s.status = WS_SINGULAR;
+        __syncthreads();
+        return;
+      }
+      // inverse permutation into gjbuf (as ints)
+      int* ip = (int*)s.gjbuf;
+      for (int r = tid; r < N; r += nth) ip[s.perm[r]] = r;
+      __syncthreads();
+      // M(r,c) = InvM(r,c) = 0.5 (Xinv(c,r) - Xinv(r,c)) (matrix.f90:97-106),
+      // written to the folded slots
+      for (int e = tid; e < N * H; e += nth) {
+        const int d = e / N + 1, r = e - (d - 1) * N;
+        if (d == H && r >= H) continue;
+        int c = r + d;
+        if (c >= N) c -= N;
+        const double xcr = A[(size_t)ip[c] * ld + ip[r]];
+        const double xrc = A[(size_t)ip[r] * ld + ip[c]];
+        const double v = 0.5 * (xcr - xrc);
+        if (check) ldiff = fmax(ldiff, fabs(Mq[e] - v));
+        lmax = fmax(lmax, fabs(v));
+        Mq[e] = v;
+      }
     }
     if (check) {
       // block max of (ldiff, lmax)
@@ -241,11 +304,21 @@ __device__ __attribute__((noinline)) void wg_measure_sample(const DevModel& m, S
   double* Ms = s.u;
   for (int q = 0; q < C; q++) {
     const int g = q >> 1, sidx = q & 1;
-    const double* Mq = Mw + (size_t)q * NN;
+    const double* Mq = Mw + (size_t)q * fold_size(N);
     const double* Sq = m.se + (size_t)q * 4 * NN;
     const double pfq = s.pf[q];
     __syncthreads();
-    for (size_t e = 2 * (size_t)tid; e < NN; e += 2 * (size_t)nth) *(double2*)(Ms + e) = *(const double2*)(Mq + e);
+    // unfold this copy's X^-1 into a full N x N block
+    for (int e = tid; e < N * (N / 2); e += nth) {
+      const int d = e / N + 1, r = e - (d - 1) * N;
+      if (d == N / 2 && r >= N / 2) continue;
+      int c = r + d;
+      if (c >= N) c -= N;
+      const double v = Mq[e];
+      Ms[(size_t)r * N + c] = v;
+      Ms[(size_t)c * N + r] = -v;
+    }
+    for (int r = tid; r < N; r += nth) Ms[(size_t)r * N + r] = 0.0;
     __syncthreads();
     const bool pp_wave = (nwv >= 2) ? (wid >= nbw) : true;
     const bool bond_wave = (nwv >= 2) ? (wid < nbw) : true;
@@ -378,7 +451,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
   const int lane = tid & (RBMPP_WAVE - 1), wid = tid >> 6;
   const int N = m.N, C = m.C, P = m.P;
   const size_t NN = (size_t)N * N;
-  double* Mw = a.ws.minv + (size_t)w * C * NN;
+  double* Mw = a.ws.minv + (size_t)w * C * fold_size(N);
   double* gscr = a.ws.scratch ? a.ws.scratch + (size_t)w * NN : nullptr;
 
   Rng rng;
@@ -413,8 +486,9 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
   // this wave's projection copy and its broadcast quads
   const int q = __builtin_amdgcn_readfirstlane(wid);  // wave-uniform
   double* wv = s.u + (size_t)wid * 4 * NMAX;
-  double* Mq = Mw + (size_t)q * NN;
+  double* Mq = Mw + (size_t)q * fold_size(N);
   const double* Sq = m.se + (size_t)q * 4 * NN;
+  const int H = N >> 1;
 
   for (int i0 = -a.nwarm; i0 <= a.nsamp && status == WS_OK; i0++) {
     // ---- periodic recompute (make_sample.f90:74-128)
@@ -480,6 +554,8 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
       if (q < C) {
         const double* colA = Sq + (size_t)rsa * 2 * N;
         const double* colB = Sq + (size_t)rsb * 2 * N;
+        double2* ab = (double2*)wv;               // (a_c, b_c): new X entries of electrons msa, msb
+        double2* pq = (double2*)(wv + 2 * NMAX);  // (P_c, Q_c)
 #pragma unroll
         for (int k = 0; k < KD; k++) {
           const int c = 2 * lane + 128 * k;
@@ -495,56 +571,93 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
               const double aa = colA[r], bb = colB[r];
               if (e == 0) { av.x = aa; bv.x = bb; } else { av.y = aa; bv.y = bb; }
             }
-            wv[4 * c + 0] = av.x; wv[4 * c + 1] = bv.x;
-            wv[4 * c + 4] = av.y; wv[4 * c + 5] = bv.y;
+            ab[c] = make_double2(av.x, bv.x);
+            ab[c + 1] = make_double2(av.y, bv.y);
           }
           bk[k] = bv;
           Pk[k] = make_double2(0.0, 0.0);
           Qk[k] = make_double2(0.0, 0.0);
         }
-        invm_ab = Mq[(size_t)msb * N + msa];
+        invm_ab = fold_get(Mq, N, msb, msa);
         wave_sync();
-        // P = X^-T a, Q = X^-T b over the rows of this copy's X^-1: UN rows
-        // of 16-byte loads in flight per lane; lanes past the last column
-        // read a clamped column and discard the result
+        // P_c = sum_j M(j,c) a_j and Q_c (with b) over the folded rows: row d
+        // holds G(d,c) = M(c,c+d), which contributes -G(d,c) a_{c+d}, and at
+        // the rotated column c-d the element M(c-d,c), which contributes
+        // +G(d,c-d) a_{c-d} (the same row again: an L1 hit).  The last row
+        // holds each pair (c, c+N/2) once, at c < N/2.
         {
-          const __amdgpu_buffer_rsrc_t rs = block_rsrc(Mq, N * N * 8);
-          int vo[KD];
+          const __amdgpu_buffer_rsrc_t rs = block_rsrc(Mq, (int)(fold_size(N) * 8));
+          int vo[KD], cl[KD];
 #pragma unroll
           for (int k = 0; k < KD; k++) {
             const int c = 2 * lane + 128 * k;
             vo[k] = c < N ? 8 * c : RBMPP_OOB;
+            cl[k] = c < N ? c : 0;
           }
           const int rowb = 8 * N;
-          int j = 0;
-          for (; j + UN <= N; j += UN) {
-            double2 mr[UN][KD];
+          int d = 1;
+          for (; d + UN <= H; d += UN) {
+            double2 gl[UN][KD];
+            double g0[UN][KD], g1[UN][KD];
 #pragma unroll
             for (int u = 0; u < UN; u++)
 #pragma unroll
-              for (int k = 0; k < KD; k++) mr[u][k] = bload2(rs, vo[k], (j + u) * rowb);
+              for (int k = 0; k < KD; k++) {
+                int r0 = cl[k] - (d + u);
+                if (r0 < 0) r0 += N;
+                const int r1 = r0 + 1 == N ? 0 : r0 + 1;
+                const bool live = vo[k] != RBMPP_OOB;
+                gl[u][k] = bload2(rs, vo[k], (d + u - 1) * rowb);
+                g0[u][k] = bload1(rs, live ? 8 * r0 : RBMPP_OOB, (d + u - 1) * rowb);
+                g1[u][k] = bload1(rs, live ? 8 * r1 : RBMPP_OOB, (d + u - 1) * rowb);
+              }
 #pragma unroll
-            for (int u = 0; u < UN; u++) {
-              const double2 ab = *(const double2*)(wv + 4 * (j + u));
+            for (int u = 0; u < UN; u++)
 #pragma unroll
               for (int k = 0; k < KD; k++) {
-                Pk[k].x = fma(mr[u][k].x, ab.x, Pk[k].x);
-                Pk[k].y = fma(mr[u][k].y, ab.x, Pk[k].y);
-                Qk[k].x = fma(mr[u][k].x, ab.y, Qk[k].x);
-                Qk[k].y = fma(mr[u][k].y, ab.y, Qk[k].y);
+                int p0 = cl[k] + (d + u), r0 = cl[k] - (d + u);
+                if (p0 >= N) p0 -= N;
+                if (r0 < 0) r0 += N;
+                const int p1 = p0 + 1 == N ? 0 : p0 + 1, r1 = r0 + 1 == N ? 0 : r0 + 1;
+                const double2 ap0 = ab[p0], ap1 = ab[p1], am0 = ab[r0], am1 = ab[r1];
+                Pk[k].x = fma(-gl[u][k].x, ap0.x, Pk[k].x); Pk[k].x = fma(g0[u][k], am0.x, Pk[k].x);
+                Qk[k].x = fma(-gl[u][k].x, ap0.y, Qk[k].x); Qk[k].x = fma(g0[u][k], am0.y, Qk[k].x);
+                Pk[k].y = fma(-gl[u][k].y, ap1.x, Pk[k].y); Pk[k].y = fma(g1[u][k], am1.x, Pk[k].y);
+                Qk[k].y = fma(-gl[u][k].y, ap1.y, Qk[k].y); Qk[k].y = fma(g1[u][k], am1.y, Qk[k].y);
               }
-            }
           }
-          for (; j < N; j++) {
-            const double2 ab = *(const double2*)(wv + 4 * j);
+          for (; d < H; d++) {
 #pragma unroll
             for (int k = 0; k < KD; k++) {
-              const double2 mv = bload2(rs, vo[k], j * rowb);
-              Pk[k].x = fma(mv.x, ab.x, Pk[k].x);
-              Pk[k].y = fma(mv.y, ab.x, Pk[k].y);
-              Qk[k].x = fma(mv.x, ab.y, Qk[k].x);
-              Qk[k].y = fma(mv.y, ab.y, Qk[k].y);
+              int p0 = cl[k] + d, r0 = cl[k] - d;
+              if (p0 >= N) p0 -= N;
+              if (r0 < 0) r0 += N;
+              const int p1 = p0 + 1 == N ? 0 : p0 + 1, r1 = r0 + 1 == N ? 0 : r0 + 1;
+              const bool live = vo[k] != RBMPP_OOB;
+              const double2 gl = bload2(rs, vo[k], (d - 1) * rowb);
+              const double g0 = bload1(rs, live ? 8 * r0 : RBMPP_OOB, (d - 1) * rowb);
+              const double g1 = bload1(rs, live ? 8 * r1 : RBMPP_OOB, (d - 1) * rowb);
+              const double2 ap0 = ab[p0], ap1 = ab[p1], am0 = ab[r0], am1 = ab[r1];
+              Pk[k].x = fma(-gl.x, ap0.x, Pk[k].x); Pk[k].x = fma(g0, am0.x, Pk[k].x);
+              Qk[k].x = fma(-gl.x, ap0.y, Qk[k].x); Qk[k].x = fma(g0, am0.y, Qk[k].x);
+              Pk[k].y = fma(-gl.y, ap1.x, Pk[k].y); Pk[k].y = fma(g1, am1.x, Pk[k].y);
+              Qk[k].y = fma(-gl.y, ap1.y, Qk[k].y); Qk[k].y = fma(g1, am1.y, Qk[k].y);
             }
+          }
+          // last row: columns c < N/2 read their own slot (pair (c, c+N/2)),
+          // columns c >= N/2 the slot c-N/2 (pair (c-N/2, c)); N/2 is even, so
+          // a lane's column pair falls on one side
+#pragma unroll
+          for (int k = 0; k < KD; k++) {
+            const bool live = vo[k] != RBMPP_OOB, lo = cl[k] < H;
+            const int soff = (H - 1) * rowb;
+            const double2 gl = bload2(rs, live && lo ? vo[k] : RBMPP_OOB, soff);
+            const double2 gr = bload2(rs, live && !lo ? 8 * (cl[k] - H) : RBMPP_OOB, soff);
+            const int o = lo ? cl[k] + H : cl[k] - H;
+            const double2 a0 = ab[o], a1 = ab[o + 1];
+            const double vx = gr.x - gl.x, vy = gr.y - gl.y;  // one of the two is zero
+            Pk[k].x = fma(vx, a0.x, Pk[k].x); Qk[k].x = fma(vx, a0.y, Qk[k].x);
+            Pk[k].y = fma(vy, a1.x, Pk[k].y); Qk[k].y = fma(vy, a1.y, Qk[k].y);
           }
         }
         double part = 0.0;
@@ -552,17 +665,17 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
         for (int k = 0; k < KD; k++) {
           const int c = 2 * lane + 128 * k;
           if (c < N) {
-            wv[4 * c + 2] = Pk[k].x; wv[4 * c + 3] = Qk[k].x;
-            wv[4 * c + 6] = Pk[k].y; wv[4 * c + 7] = Qk[k].y;
+            pq[c] = make_double2(Pk[k].x, Qk[k].x);
+            pq[c + 1] = make_double2(Pk[k].y, Qk[k].y);
             part = fma(bk[k].x, Pk[k].x, part);
             part = fma(bk[k].y, Pk[k].y, part);
           }
         }
         bMa = wave_sum(part);
         wave_sync();
-        pa = wv[4 * msa + 2]; qa = wv[4 * msa + 3];
-        pb = wv[4 * msb + 2]; qb = wv[4 * msb + 3];
-        vec_ba = wv[4 * msa + 1];
+        pa = pq[msa].x; qa = pq[msa].y;
+        pb = pq[msb].x; qb = pq[msb].y;
+        vec_ba = ab[msa].y;
         const double ratio = invm_ab * vec_ba + invm_ab * bMa + pa * qb - pb * qa;
         rq = ratio;
         const double pfn = ratio * s.pf[q];
@@ -587,114 +700,98 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
           const double InvDet = 1.0 / Det;
           const double fID = f_ * InvDet;
           const double eDet = e_ * Det;
-          double2 A[KD], B[KD], Cc[KD], D[KD], R1[KD], R2[KD];
+          // Delta M(j,c) = A_c t_j + B_c s_j + C_c q_j + D_c p_j with the
+          // column coefficients (A, B, C, D) published per column and the row
+          // scalars of the lane's own rows.  The reference's separate fix-ups
+          // of rows msa and msb (pfupdate_two.f90:382-383,397-398) are +D_c
+          // and +C_c: they enter as p_j + [j == msa] and q_j + [j == msb].
+          // The pass runs one column per lane (c = lane + 64 k) so that every
+          // store moves 64 bits: a wider store needs a wait state before its
+          // data registers are rewritten, which the compiler was seen to drop
+          // in this loop.
+          constexpr int KC = 2 * KD;
+          double2* cf = (double2*)wv;  // (A_c, B_c) at cf[2c], (C_c, D_c) at cf[2c+1]
+          const double2* pq = (const double2*)(wv + 2 * NMAX);  // (P_c, Q_c) of the ratio pass
+          double ps[KC], pt[KC], pp[KC], pqv[KC];
 #pragma unroll
-          for (int k = 0; k < KD; k++) {
-            const double2 z = make_double2(0.0, 0.0);
-            A[k] = z; B[k] = z; Cc[k] = z; D[k] = z; R1[k] = z; R2[k] = z;
-            const int c = 2 * lane + 128 * k;
+          for (int k = 0; k < KC; k++) {
+            const int c = lane + RBMPP_WAVE * k;
+            ps[k] = pt[k] = pp[k] = pqv[k] = 0.0;
             if (c < N) {
               // s_i = InvDet InvM(i,msa) = -InvDet InvM(msa,i); t_i likewise
-              const double2 ra = *(const double2*)(Mq + (size_t)msa * N + c);
-              const double2 rb = *(const double2*)(Mq + (size_t)msb * N + c);
-              const double2 si = make_double2(-InvDet * ra.x, -InvDet * ra.y);
-              const double2 ti = make_double2(-InvDet * rb.x, -InvDet * rb.y);
-              wv[4 * c + 0] = si.x; wv[4 * c + 1] = ti.x;
-              wv[4 * c + 4] = si.y; wv[4 * c + 5] = ti.y;
-#define COEF(comp, col)                                                         \
-  {                                                                             \
-    const double p_i = Pk[k].comp, q_i = Qk[k].comp, s_i = si.comp, t_i = ti.comp; \
-    A[k].comp = a_ * q_i + c_ * p_i + eDet * s_i;                               \
-    B[k].comp = b_ * q_i + d_ * p_i - eDet * t_i;                               \
-    Cc[k].comp = -a_ * t_i - b_ * s_i + fID * p_i;                              \
-    D[k].comp = -c_ * t_i - d_ * s_i - fID * q_i;                               \
-    if ((col) == msa) { A[k].comp += c_; B[k].comp += d_; Cc[k].comp += fID; }  \
-    if ((col) == msb) { A[k].comp += a_; B[k].comp += b_; D[k].comp -= fID; }   \
-    R1[k].comp = c_ * t_i + d_ * s_i + fID * q_i;                               \
-    R2[k].comp = a_ * t_i + b_ * s_i - fID * p_i;                               \
-  }
-              COEF(x, c)
-              COEF(y, c + 1)
-#undef COEF
+              ps[k] = -InvDet * fold_get(Mq, N, msa, c);
+              pt[k] = -InvDet * fold_get(Mq, N, msb, c);
+              pp[k] = pq[c].x;
+              pqv[k] = pq[c].y;
+            }
+          }
+          wave_sync();  // the (P, Q) table is read before the coefficients replace it
+#pragma unroll
+          for (int k = 0; k < KC; k++) {
+            const int c = lane + RBMPP_WAVE * k;
+            if (c < N) {
+              const double p_i = pp[k], q_i = pqv[k], s_i = ps[k], t_i = pt[k];
+              double A = a_ * q_i + c_ * p_i + eDet * s_i;
+              double B = b_ * q_i + d_ * p_i - eDet * t_i;
+              double Cc = -a_ * t_i - b_ * s_i + fID * p_i;
+              double D = -c_ * t_i - d_ * s_i - fID * q_i;
+              if (c == msa) { A += c_; B += d_; Cc += fID; }
+              if (c == msb) { A += a_; B += b_; D -= fID; }
+              cf[2 * c] = make_double2(A, B);
+              cf[2 * c + 1] = make_double2(Cc, D);
+              pp[k] = p_i + (c == msa ? 1.0 : 0.0);
+              pqv[k] = q_i + (c == msb ? 1.0 : 0.0);
             }
           }
           wave_sync();
-          // rank-4 update of every row (streamed, UN rows in flight) ...
+          // one read-modify-write pass over the folded rows: slot (d, j) is
+          // M(j, j+d); the last row only for j < N/2
           {
-            const __amdgpu_buffer_rsrc_t rs = block_rsrc(Mq, N * N * 8);
-            int vo[KD];
-            bool live[KD];
+            const __amdgpu_buffer_rsrc_t rs = block_rsrc(Mq, (int)(fold_size(N) * 8));
+            int vo[KC], vl[KC], cl[KC];
 #pragma unroll
-            for (int k = 0; k < KD; k++) {
-              const int c = 2 * lane + 128 * k;
-              live[k] = c < N;
-              vo[k] = live[k] ? 8 * c : RBMPP_OOB;
+            for (int k = 0; k < KC; k++) {
+              const int c = lane + RBMPP_WAVE * k;
+              vo[k] = c < N ? 8 * c : RBMPP_OOB;
+              vl[k] = c < H ? 8 * c : RBMPP_OOB;
+              cl[k] = c < N ? c : 0;
             }
             const int rowb = 8 * N;
-            int j = 0;
-            for (; j + UN <= N; j += UN) {
-              double2 mr[UN][KD];
+            int d = 1;
+            for (; d + UN - 1 <= H; d += UN) {
+              double g[UN][KC];
 #pragma unroll
               for (int u = 0; u < UN; u++)
 #pragma unroll
-                for (int k = 0; k < KD; k++) mr[u][k] = bload2(rs, vo[k], (j + u) * rowb);
+                for (int k = 0; k < KC; k++) g[u][k] = bload1(rs, d + u == H ? vl[k] : vo[k], (d + u - 1) * rowb);
 #pragma unroll
-              for (int u = 0; u < UN; u++) {
-                const double2 st = *(const double2*)(wv + 4 * (j + u));      // s_j, t_j
-                const double2 pq = *(const double2*)(wv + 4 * (j + u) + 2);  // p_j, q_j
+              for (int u = 0; u < UN; u++)
 #pragma unroll
-                for (int k = 0; k < KD; k++) {
-                  double2 v = mr[u][k];
-                  v.x = fma(A[k].x, st.y, v.x); v.y = fma(A[k].y, st.y, v.y);
-                  v.x = fma(B[k].x, st.x, v.x); v.y = fma(B[k].y, st.x, v.y);
-                  v.x = fma(Cc[k].x, pq.y, v.x); v.y = fma(Cc[k].y, pq.y, v.y);
-                  v.x = fma(D[k].x, pq.x, v.x); v.y = fma(D[k].y, pq.x, v.y);
-                  // the reference's update leaves the diagonal exactly zero
-                  // (its terms cancel exactly; here the regrouped FMAs leave
-                  // ~1e-17, which would feed the f_ii derivative)
-                  const int cc = 2 * lane + 128 * k;
-                  if (j + u == cc) v.x = 0.0;
-                  if (j + u == cc + 1) v.y = 0.0;
-                  bstore2(v, rs, vo[k], (j + u) * rowb);
+                for (int k = 0; k < KC; k++) {
+                  int p0 = cl[k] + (d + u);
+                  if (p0 >= N) p0 -= N;
+                  const double2 ab0 = cf[2 * p0], cd0 = cf[2 * p0 + 1];
+                  double v = g[u][k];
+                  v = fma(ab0.x, pt[k], v);
+                  v = fma(ab0.y, ps[k], v);
+                  v = fma(cd0.x, pqv[k], v);
+                  v = fma(cd0.y, pp[k], v);
+                  bstore1(v, rs, d + u == H ? vl[k] : vo[k], (d + u - 1) * rowb);
                 }
-              }
             }
-            for (; j < N; j++) {
-              const double2 st = *(const double2*)(wv + 4 * j);
-              const double2 pq = *(const double2*)(wv + 4 * j + 2);
+            for (; d <= H; d++) {
 #pragma unroll
-              for (int k = 0; k < KD; k++) {
-                double2 v = bload2(rs, vo[k], j * rowb);
-                v.x = fma(A[k].x, st.y, v.x); v.y = fma(A[k].y, st.y, v.y);
-                v.x = fma(B[k].x, st.x, v.x); v.y = fma(B[k].y, st.x, v.y);
-                v.x = fma(Cc[k].x, pq.y, v.x); v.y = fma(Cc[k].y, pq.y, v.y);
-                v.x = fma(D[k].x, pq.x, v.x); v.y = fma(D[k].y, pq.x, v.y);
-                const int cc = 2 * lane + 128 * k;
-                if (j == cc) v.x = 0.0;
-                if (j == cc + 1) v.y = 0.0;
-                bstore2(v, rs, vo[k], j * rowb);
-              }
-            }
-            // ... then the row fix-ups of rows msa, msb and the two corner
-            // entries (pfupdate_two.f90:382-383,397-398)
-#pragma unroll
-            for (int k = 0; k < KD; k++) {
-              const int c = 2 * lane + 128 * k;
-              if (live[k]) {
-                double2 va = *(const double2*)(Mq + (size_t)msa * N + c);
-                double2 vb = *(const double2*)(Mq + (size_t)msb * N + c);
-                va.x -= R1[k].x; va.y -= R1[k].y;
-                vb.x -= R2[k].x; vb.y -= R2[k].y;
-                if (c == msb) va.x -= fID;
-                if (c + 1 == msb) va.y -= fID;
-                if (c == msa) vb.x += fID;
-                if (c + 1 == msa) vb.y += fID;
-                if (c == msa) va.x = 0.0;
-                if (c + 1 == msa) va.y = 0.0;
-                if (c == msb) vb.x = 0.0;
-                if (c + 1 == msb) vb.y = 0.0;
-                *(double2*)(Mq + (size_t)msa * N + c) = va;
-                *(double2*)(Mq + (size_t)msb * N + c) = vb;
+              for (int k = 0; k < KC; k++) {
+                const int vv = d == H ? vl[k] : vo[k];
+                int p0 = cl[k] + d;
+                if (p0 >= N) p0 -= N;
+                const double2 ab0 = cf[2 * p0], cd0 = cf[2 * p0 + 1];
+                double v = bload1(rs, vv, (d - 1) * rowb);
+                v = fma(ab0.x, pt[k], v);
+                v = fma(ab0.y, ps[k], v);
+                v = fma(cd0.x, pqv[k], v);
+                v = fma(cd0.y, pp[k], v);
+                bstore1(v, rs, vv, (d - 1) * rowb);
               }
             }
           }

@@ -50,10 +50,13 @@ def test_fortran_driver_matches_python_mirror(tmp_path):
         shift = sum(wf_list) / 10.0 * 1e-3 / spec.N / spec.N
     gam_av /= spec.nav
     po = lt.Params.from_gamma(spec, gam_av)
+    # the orbital-derivative bins are summed with LDS atomics, whose order
+    # varies between runs: O, S and the solved step agree to rounding, which
+    # the SR solve turns into absolute parameter differences of ~1e-14
     _, _, Wr, br = lt.read_w(str(tmp_path / "optimized_W.txt"), 0, spec.alphar, spec.N)
-    np.testing.assert_allclose(Wr, po.Wr, atol=1e-14)
-    np.testing.assert_allclose(br, po.br, atol=1e-14)
+    np.testing.assert_allclose(Wr, po.Wr, rtol=0, atol=1e-13)
+    np.testing.assert_allclose(br, po.br, rtol=0, atol=1e-13)
     f = lt.read_zqp_orbital(str(tmp_path / "zqp_orbital_opt.dat"), spec.n_slater)
-    np.testing.assert_allclose(f, po.f, rtol=0, atol=1e-14)  # e25.15 in the file
+    np.testing.assert_allclose(f, po.f, rtol=0, atol=1e-13)  # e25.15 in the file
     avg = open(tmp_path / "Average_Energy.txt").read().split()
     assert len(avg) == 10 and np.isfinite([float(t) for t in avg]).all()

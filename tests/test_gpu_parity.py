@@ -265,3 +265,30 @@ def test_guarded_chain_through_near_singular_configuration(oracle_lib):
     assert np.array_equal(xs[w * ns:(w + 1) * ns], out["x"])
     exact = eng.local(xs[w * ns:(w + 1) * ns], want_o=False)["psi"]
     assert _relerr(psi[w * ns:(w + 1) * ns] / exact, np.ones(ns)) < 1e-6
+
+
+@pytest.mark.parametrize("L,ar,ac", [(4, 1, 0), (4, 1, 1)])
+def test_autograd_logpsi_against_finite_differences(L, ar, ac):
+    """d ln|psi| / d gamma from the hand-written adjoint (torch backward of
+    rbm_pp_amd.autograd) against central differences of the GPU forward,
+    h = 1e-6: relative 1e-6 of the largest component."""
+    import torch
+    api = _api()
+    from rbm_pp_amd.autograd import log_abs_psi
+    spec, p = _model(L, ar, ac, sigma_w=0.1)
+    eng = api.Engine(spec, n_walkers=1, n_samples=1, params=p)
+    xs = _configs(spec.N, 5, seed=42)
+    w = torch.tensor(np.linspace(0.5, 1.5, len(xs)))
+    gam = torch.tensor(p.gamma(), requires_grad=True)
+    y = (w * log_abs_psi(gam, eng, xs)).sum()
+    y.backward()
+    grad = gam.grad.numpy()
+    h = 1e-6
+    g0 = p.gamma()
+    fd = np.zeros_like(g0)
+    for k in range(len(g0)):
+        for sgn in (1, -1):
+            gk = g0.copy()
+            gk[k] += sgn * h
+            fd[k] += sgn * float((w * log_abs_psi(torch.tensor(gk), eng, xs)).sum()) / (2 * h)
+    np.testing.assert_allclose(grad, fd, rtol=0, atol=1e-6 * np.abs(fd).max())
