@@ -201,6 +201,49 @@ __device__ __forceinline__ double2 psi_wo_pf_exchanged(const DevModel& m, int q,
   return psi_wo_pf_exchanged(m, model_tables(m), q, thc, thr, sgn_new, i1, i2, x1, x2);
 }
 
+// Same, evaluated by one wave: lane f takes hidden units f, f+64, ... and the
+// factors are multiplied across the wave (butterfly), so the cosh and the
+// table reads of all hidden units proceed in parallel.  Every lane returns
+// the amplitude.  The products are those of the serial form in another
+// order (the 1/2 of each complex factor is exact).
+__device__ __forceinline__ double2 psi_wo_pf_exchanged_wave(const DevModel& m, const RbmTables& tb, int q,
+                                                            const double2* thc, const double* thr, int sgn_new,
+                                                            int i1, int i2, int x1, int x2, int lane) {
+  const int g = q >> 1, sidx = q & 1;
+  const double s = sidx ? -1.0 : 1.0;
+  const int iw1 = tb.wmap[g * m.N + i1], iw2 = tb.wmap[g * m.N + i2];
+  const double a1 = 2.0 * (double)x1 * s, a2 = 2.0 * (double)x2 * s;
+  double2 c = make_double2(1.0, 0.0);
+  for (int f = lane; f < m.ac; f += RBMPP_WAVE) {
+    double2 t = thc[q * m.ac + f];
+    const double2 w1 = tb.Wc[f * m.N + iw1], w2 = tb.Wc[f * m.N + iw2];
+    t.x = t.x - w1.x * a1; t.y = t.y - w1.y * a1;
+    t.x = t.x - w2.x * a2; t.y = t.y - w2.y * a2;
+    double sn, cs;
+    const double sh = sinh(t.x), ch = cosh(t.x);
+    sincos(t.y, &sn, &cs);
+    c = cmul(c, make_double2(0.5 * (ch * cs), 0.5 * (sh * sn)));
+  }
+  double r1 = 1.0;
+  for (int f = lane; f < m.ar; f += RBMPP_WAVE) {
+    double t = thr[q * m.ar + f];
+    t = t - 2.0 * tb.Wr[f * m.N + iw1] * (double)x1 * s;
+    t = t - 2.0 * tb.Wr[f * m.N + iw2] * (double)x2 * s;
+    r1 = r1 * cosh(t);
+  }
+#pragma unroll
+  for (int o = 1; o < RBMPP_WAVE; o <<= 1) {
+    r1 = r1 * __shfl_xor(r1, o, RBMPP_WAVE);
+    const double2 oc = make_double2(__shfl_xor(c.x, o, RBMPP_WAVE), __shfl_xor(c.y, o, RBMPP_WAVE));
+    c = cmul(c, oc);
+  }
+  c = cscale(c, r1);
+  c = cscale(c, (double)sgn_new);
+  c = cmul(c, m.chi[g]);
+  c = cscale(c, m.sp_w[sidx]);
+  return c;
+}
+
 // --------------------------------------------------------------------
 // Workgroup-cooperative Pfaffian and inverse of a skew-symmetric n x n
 // matrix A (row-major, leading dimension ld; LDS or global).  Gauss-Jordan

@@ -81,6 +81,13 @@ struct SweepSmem {
   int guard;  // set by a copy whose updated X^-1 exceeds the conditioning bound
   int gsel[2];     // pivot index / value of the register Gauss-Jordan
   double gselv[2];
+  // RBM tables staged for the proposals' amplitudes (Wr, Wc as doubles;
+  // W_map; sublattice flags), used when they fit
+  static constexpr int TW = LDSMAT ? 1536 : 2;
+  static constexpr int TM = LDSMAT ? (RBMPP_CMAX / 2) * NMAX : 1;
+  alignas(16) double tw[TW];
+  int tmap[TM];
+  unsigned char tsub[LDSMAT ? NMAX : 1];
   unsigned long long ctr;
 };
 
@@ -465,6 +472,18 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
   for (int i = tid; i < C * m.ac; i += nth) s.thc[i] = a.ws.thc[(size_t)w * m.mc + i];
   for (int q = tid; q < C; q += nth) s.pf[q] = a.ws.pf[(size_t)w * C + q];
   for (int g = tid; g < P; g += nth) s.sgn[g] = a.ws.sgn[(size_t)w * P + g];
+  RbmTables tb = model_tables(m);
+  if constexpr (LDSMAT) {
+    using SS = SweepSmem<NMAX, LDSMAT>;
+    const int nwr = m.ar * N, nwc = 2 * m.ac * N;
+    if (nwr + nwc <= SS::TW && P * N <= SS::TM) {
+      for (int i = tid; i < nwr; i += nth) s.tw[i] = m.Wr[i];
+      for (int i = tid; i < nwc; i += nth) s.tw[nwr + i] = ((const double*)m.Wc)[i];
+      for (int i = tid; i < P * N; i += nth) s.tmap[i] = m.wmap[i];
+      for (int i = tid; i < N; i += nth) s.tsub[i] = m.subB[i];
+      tb = {s.tw, (const double2*)(s.tw + nwr), s.tmap, s.tsub};
+    }
+  }
   __syncthreads();
 
   int status = WS_OK;
@@ -542,8 +561,8 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
       const int msa = s.ecfg[i1], msb = s.ecfg[i2];
       const int x1 = s.x[i1], x2 = s.x[i2];
       int sfac = -1;
-      if (m.subB[i1]) sfac = -sfac;
-      if (m.subB[i2]) sfac = -sfac;
+      if (tb.subB[i1]) sfac = -sfac;
+      if (tb.subB[i2]) sfac = -sfac;
       const int rsa = i2 + (msa >= m.npair ? N : 0);  // new row of electron msa
       const int rsb = i1 + (msb >= m.npair ? N : 0);
 
@@ -679,7 +698,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
         const double ratio = invm_ab * vec_ba + invm_ab * bMa + pa * qb - pb * qa;
         rq = ratio;
         const double pfn = ratio * s.pf[q];
-        const double2 c = psi_wo_pf_exchanged(m, q, s.thc, s.thr, sfac * s.sgn[q >> 1], i1, i2, x1, x2);
+        const double2 c = psi_wo_pf_exchanged_wave(m, tb, q, s.thc, s.thr, sfac * s.sgn[q >> 1], i1, i2, x1, x2, lane);
         if (lane == 0) {
           s.pfn[q] = pfn;
           s.psiq[q] = cscale(c, pfn);
@@ -804,16 +823,16 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
           // angles of this copy (update_theta)
           const int g = q >> 1;
           const double sp = (q & 1) ? -1.0 : 1.0;
-          const int iw1 = m.wmap[g * N + i1], iw2 = m.wmap[g * N + i2];
+          const int iw1 = tb.wmap[g * N + i1], iw2 = tb.wmap[g * N + i2];
           for (int f = lane; f < m.ar; f += RBMPP_WAVE) {
             double t = s.thr[q * m.ar + f];
-            t = t - 2.0 * m.Wr[f * N + iw1] * (double)x1 * sp;
-            t = t - 2.0 * m.Wr[f * N + iw2] * (double)x2 * sp;
+            t = t - 2.0 * tb.Wr[f * N + iw1] * (double)x1 * sp;
+            t = t - 2.0 * tb.Wr[f * N + iw2] * (double)x2 * sp;
             s.thr[q * m.ar + f] = t;
           }
           for (int f = lane; f < m.ac; f += RBMPP_WAVE) {
             double2 t = s.thc[q * m.ac + f];
-            const double2 w1 = m.Wc[f * N + iw1], w2 = m.Wc[f * N + iw2];
+            const double2 w1 = tb.Wc[f * N + iw1], w2 = tb.Wc[f * N + iw2];
             const double a1 = 2.0 * (double)x1 * sp, a2 = 2.0 * (double)x2 * sp;
             t.x = t.x - w1.x * a1; t.y = t.y - w1.y * a1;
             t.x = t.x - w2.x * a2; t.y = t.y - w2.y * a2;
