@@ -573,8 +573,9 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
       if (q < C) {
         const double* colA = Sq + (size_t)rsa * 2 * N;
         const double* colB = Sq + (size_t)rsb * 2 * N;
-        double2* ab = (double2*)wv;               // (a_c, b_c): new X entries of electrons msa, msb
-        double2* pq = (double2*)(wv + 2 * NMAX);  // (P_c, Q_c)
+        // (a_j, b_j) for j in [-N/2, 3N/2) at ab[j + N/2] (periodic in N), so
+        // that the shifted reads of the folded rows below need no wrap
+        double2* ab = (double2*)wv;
 #pragma unroll
         for (int k = 0; k < KD; k++) {
           const int c = 2 * lane + 128 * k;
@@ -590,8 +591,12 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
               const double aa = colA[r], bb = colB[r];
               if (e == 0) { av.x = aa; bv.x = bb; } else { av.y = aa; bv.y = bb; }
             }
-            ab[c] = make_double2(av.x, bv.x);
-            ab[c + 1] = make_double2(av.y, bv.y);
+            const double2 e0 = make_double2(av.x, bv.x), e1 = make_double2(av.y, bv.y);
+            ab[c + H] = e0;
+            ab[c + 1 + H] = e1;
+            const int cw = c >= H ? c - H : c + N + H;  // the periodic image (N/2 is even)
+            ab[cw] = e0;
+            ab[cw + 1] = e1;
           }
           bk[k] = bv;
           Pk[k] = make_double2(0.0, 0.0);
@@ -606,57 +611,59 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
         // holds each pair (c, c+N/2) once, at c < N/2.
         {
           const __amdgpu_buffer_rsrc_t rs = block_rsrc(Mq, (int)(fold_size(N) * 8));
-          int vo[KD], cl[KD];
+          int vo[KD], cb[KD];  // the lane's column pair (dead lanes: a valid one, results unused)
 #pragma unroll
           for (int k = 0; k < KD; k++) {
             const int c = 2 * lane + 128 * k;
             vo[k] = c < N ? 8 * c : RBMPP_OOB;
-            cl[k] = c < N ? c : 0;
+            cb[k] = c < N ? c : N - 2;
           }
           const int rowb = 8 * N;
+          // byte offset of column (t mod N) for t in (-N, N): the unsigned
+          // minimum of t and t + N picks the non-negative one
+          auto wrap8 = [N](int t8) -> int { return (int)min((unsigned)t8, (unsigned)(t8 + 8 * N)); };
+          static_assert(UN % 2 == 0, "batches start at odd rows");
           int d = 1;
           for (; d + UN <= H; d += UN) {
-            double2 gl[UN][KD];
-            double g0[UN][KD], g1[UN][KD];
+            // rows d..d+UN-1, d odd: row d+u is even for odd u, and then the
+            // rotated pair (c-d-u, c+1-d-u) is an aligned pair of one row
+            double2 gl[UN][KD], gr[UN][KD];
 #pragma unroll
             for (int u = 0; u < UN; u++)
 #pragma unroll
               for (int k = 0; k < KD; k++) {
-                int r0 = cl[k] - (d + u);
-                if (r0 < 0) r0 += N;
-                const int r1 = r0 + 1 == N ? 0 : r0 + 1;
-                const bool live = vo[k] != RBMPP_OOB;
-                gl[u][k] = bload2(rs, vo[k], (d + u - 1) * rowb);
-                g0[u][k] = bload1(rs, live ? 8 * r0 : RBMPP_OOB, (d + u - 1) * rowb);
-                g1[u][k] = bload1(rs, live ? 8 * r1 : RBMPP_OOB, (d + u - 1) * rowb);
+                const int soff = (d + u - 1) * rowb;
+                const int t0 = 8 * (cb[k] - (d + u));
+                gl[u][k] = bload2(rs, vo[k], soff);
+                if (u & 1) {
+                  gr[u][k] = bload2(rs, wrap8(t0), soff);
+                } else {
+                  gr[u][k] = make_double2(bload1(rs, wrap8(t0), soff), bload1(rs, wrap8(t0 + 8), soff));
+                }
               }
 #pragma unroll
-            for (int u = 0; u < UN; u++)
+            for (int k = 0; k < KD; k++) {
+              const double2* ap = ab + (cb[k] + H + d);             // a_{c+d+u} at ap[u]
+              const double2* am = ab + (cb[k] + H - d - (UN - 1));  // a_{c-d-u} at am[UN-1-u]
 #pragma unroll
-              for (int k = 0; k < KD; k++) {
-                int p0 = cl[k] + (d + u), r0 = cl[k] - (d + u);
-                if (p0 >= N) p0 -= N;
-                if (r0 < 0) r0 += N;
-                const int p1 = p0 + 1 == N ? 0 : p0 + 1, r1 = r0 + 1 == N ? 0 : r0 + 1;
-                const double2 ap0 = ab[p0], ap1 = ab[p1], am0 = ab[r0], am1 = ab[r1];
-                Pk[k].x = fma(-gl[u][k].x, ap0.x, Pk[k].x); Pk[k].x = fma(g0[u][k], am0.x, Pk[k].x);
-                Qk[k].x = fma(-gl[u][k].x, ap0.y, Qk[k].x); Qk[k].x = fma(g0[u][k], am0.y, Qk[k].x);
-                Pk[k].y = fma(-gl[u][k].y, ap1.x, Pk[k].y); Pk[k].y = fma(g1[u][k], am1.x, Pk[k].y);
-                Qk[k].y = fma(-gl[u][k].y, ap1.y, Qk[k].y); Qk[k].y = fma(g1[u][k], am1.y, Qk[k].y);
+              for (int u = 0; u < UN; u++) {
+                const double2 ap0 = ap[u], ap1 = ap[u + 1], am0 = am[UN - 1 - u], am1 = am[UN - u];
+                Pk[k].x = fma(-gl[u][k].x, ap0.x, Pk[k].x); Pk[k].x = fma(gr[u][k].x, am0.x, Pk[k].x);
+                Qk[k].x = fma(-gl[u][k].x, ap0.y, Qk[k].x); Qk[k].x = fma(gr[u][k].x, am0.y, Qk[k].x);
+                Pk[k].y = fma(-gl[u][k].y, ap1.x, Pk[k].y); Pk[k].y = fma(gr[u][k].y, am1.x, Pk[k].y);
+                Qk[k].y = fma(-gl[u][k].y, ap1.y, Qk[k].y); Qk[k].y = fma(gr[u][k].y, am1.y, Qk[k].y);
               }
+            }
           }
           for (; d < H; d++) {
 #pragma unroll
             for (int k = 0; k < KD; k++) {
-              int p0 = cl[k] + d, r0 = cl[k] - d;
-              if (p0 >= N) p0 -= N;
-              if (r0 < 0) r0 += N;
-              const int p1 = p0 + 1 == N ? 0 : p0 + 1, r1 = r0 + 1 == N ? 0 : r0 + 1;
-              const bool live = vo[k] != RBMPP_OOB;
-              const double2 gl = bload2(rs, vo[k], (d - 1) * rowb);
-              const double g0 = bload1(rs, live ? 8 * r0 : RBMPP_OOB, (d - 1) * rowb);
-              const double g1 = bload1(rs, live ? 8 * r1 : RBMPP_OOB, (d - 1) * rowb);
-              const double2 ap0 = ab[p0], ap1 = ab[p1], am0 = ab[r0], am1 = ab[r1];
+              const int soff = (d - 1) * rowb;
+              const int t0 = 8 * (cb[k] - d);
+              const double2 gl = bload2(rs, vo[k], soff);
+              const double g0 = bload1(rs, wrap8(t0), soff), g1 = bload1(rs, wrap8(t0 + 8), soff);
+              const double2 ap0 = ab[cb[k] + H + d], ap1 = ab[cb[k] + H + d + 1];
+              const double2 am0 = ab[cb[k] + H - d], am1 = ab[cb[k] + H - d + 1];
               Pk[k].x = fma(-gl.x, ap0.x, Pk[k].x); Pk[k].x = fma(g0, am0.x, Pk[k].x);
               Qk[k].x = fma(-gl.x, ap0.y, Qk[k].x); Qk[k].x = fma(g0, am0.y, Qk[k].x);
               Pk[k].y = fma(-gl.y, ap1.x, Pk[k].y); Pk[k].y = fma(g1, am1.x, Pk[k].y);
@@ -668,17 +675,20 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
           // a lane's column pair falls on one side
 #pragma unroll
           for (int k = 0; k < KD; k++) {
-            const bool live = vo[k] != RBMPP_OOB, lo = cl[k] < H;
+            const bool live = vo[k] != RBMPP_OOB, lo = cb[k] < H;
             const int soff = (H - 1) * rowb;
             const double2 gl = bload2(rs, live && lo ? vo[k] : RBMPP_OOB, soff);
-            const double2 gr = bload2(rs, live && !lo ? 8 * (cl[k] - H) : RBMPP_OOB, soff);
-            const int o = lo ? cl[k] + H : cl[k] - H;
+            const double2 gr = bload2(rs, live && !lo ? 8 * (cb[k] - H) : RBMPP_OOB, soff);
+            const int o = (lo ? cb[k] + H : cb[k] - H) + H;
             const double2 a0 = ab[o], a1 = ab[o + 1];
             const double vx = gr.x - gl.x, vy = gr.y - gl.y;  // one of the two is zero
             Pk[k].x = fma(vx, a0.x, Pk[k].x); Qk[k].x = fma(vx, a0.y, Qk[k].x);
             Pk[k].y = fma(vy, a1.x, Pk[k].y); Qk[k].y = fma(vy, a1.y, Qk[k].y);
           }
         }
+        vec_ba = ab[msa + H].y;
+        wave_sync();  // the (a, b) table is read before (P, Q) replaces it
+        double2* pq = (double2*)wv;  // (P_c, Q_c)
         double part = 0.0;
 #pragma unroll
         for (int k = 0; k < KD; k++) {
@@ -694,7 +704,6 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
         wave_sync();
         pa = pq[msa].x; qa = pq[msa].y;
         pb = pq[msb].x; qb = pq[msb].y;
-        vec_ba = ab[msa].y;
         const double ratio = invm_ab * vec_ba + invm_ab * bMa + pa * qb - pb * qa;
         rq = ratio;
         const double pfn = ratio * s.pf[q];
@@ -730,7 +739,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
           // in this loop.
           constexpr int KC = 2 * KD;
           double2* cf = (double2*)wv;  // (A_c, B_c) at cf[2c], (C_c, D_c) at cf[2c+1]
-          const double2* pq = (const double2*)(wv + 2 * NMAX);  // (P_c, Q_c) of the ratio pass
+          const double2* pq = (const double2*)wv;  // (P_c, Q_c) of the ratio pass
           double ps[KC], pt[KC], pp[KC], pqv[KC];
 #pragma unroll
           for (int k = 0; k < KC; k++) {
