@@ -25,6 +25,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# MI355X dense matrix-core peaks (TFLOP/s)
+MFMA_PEAK_TFS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
 
 
 def log(msg: str) -> None:
@@ -75,6 +77,8 @@ def main():
     ap.add_argument("--cpu-procs", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64-compare", action="store_true")
+    ap.add_argument("--s-precision", choices=["fp64", "fp32", "bf16x3"], default="fp32",
+                    help="S products of the timed SR steps (centring, shift and solve stay FP64)")
     ap.add_argument("--fused-measure", action="store_true",
                     help="measure inside the sampler with the fast-updated X^-1 instead of "
                          "recomputing Pf and X^-1 per sample as the reference does")
@@ -115,6 +119,7 @@ def main():
         log(f"cpu baseline: {rate:.1f} samples/s")
 
     from rbm_pp_amd import api
+    S_MODES = {"fp64": api.S_FP64, "fp32": api.S_FP32, "bf16x3": api.S_BF16X3}
     from rbm_pp_amd import lattice as lt
 
     spec = lt.j1j2_model(args.L, alphar=args.alphar, J2=args.J2)
@@ -122,7 +127,7 @@ def main():
     nspw = math.ceil(args.samples / args.walkers)
     eng = api.Engine(spec, n_walkers=args.walkers, n_samples=nspw, seed=1, n_warmup=-1,
                      n_warmup_first=-1, recompute_k=args.recompute_k,
-                     s_precision=api.S_FP32, params=p, fused_measure=args.fused_measure,
+                     s_precision=S_MODES[args.s_precision], params=p, fused_measure=args.fused_measure,
                      cond_guard=args.cond_guard)
     N, Cc = spec.N, spec.C
     log(f"model L={args.L} N={N} copies={Cc} nv={spec.nv} walkers={args.walkers} samples/walker={nspw}")
@@ -147,15 +152,17 @@ def main():
     wf_list = [0.0] * 10
     shift = 0.0
     gam = eng.gamma()
+    gam_prev = gam.copy()
 
     def one_step(rec):
-        nonlocal shift, gam
+        nonlocal shift, gam, gam_prev
         t0 = time.perf_counter()
         nacc = eng.make_sample(False, shift=shift)
         tt = eng.timings()
         t1 = time.perf_counter()
         r = eng.calc_phys_quantity_opt(shift=shift)
         t2 = time.perf_counter()
+        gam_prev = gam.copy()
         gam, rep = eng.sr_step(gam, s_factor0=1e-4, delta_tau=0.05, refine_iters=1)
         t3 = time.perf_counter()
         tt2 = eng.timings()
@@ -183,33 +190,52 @@ def main():
             f"recomputes/walker={x['recomputes'] / args.walkers:.1f}")
     t_total = time.perf_counter() - t_start
 
-    # FP64 S on the last samples, for the precision comparison
+    # The last step's samples through the three S precisions: S-build time and
+    # error, and the SR step from the same parameters against the FP64 one.
+    # Algorithmic flops of the S build: one triangle, nv (nv + 1) / 2 entries
+    # of rows-long dot products (2 flops per term).
     fp64 = None
     if not args.no_fp64_compare:
-        eng.set_s_precision(api.S_FP64)
-        S64 = eng.smat()
-        t64 = eng.timings()["s_build"]
-        eng.set_s_precision(api.S_FP32)
-        S32 = eng.smat()
-        t32 = eng.timings()["s_build"]
-        log(f"S build: fp64 {t64 * 1e3:.3f} ms, fp32 {t32 * 1e3:.3f} ms")
-        fp64 = dict(s_build_fp64_s=t64, s_build_fp32_s=t32,
-                    s_rel_err_fp32=float(np.linalg.norm(S32 - S64) / np.linalg.norm(S64)))
+        rows = ns_step if not spec.alphac else 2 * ns_step
+        f_s = spec.nv * (spec.nv + 1) * float(rows)
+        peaks = {"fp64": MFMA_PEAK_TFS["fp64"], "fp32": MFMA_PEAK_TFS["fp32"],
+                 "bf16x3": MFMA_PEAK_TFS["bf16"] / 3.0}
+        res = {}
+        for name, prec in (("fp64", api.S_FP64), ("fp32", api.S_FP32), ("bf16x3", api.S_BF16X3)):
+            eng.set_s_precision(prec)
+            S = eng.smat()
+            t_s = eng.timings()["s_build"]
+            g_new, _ = eng.sr_step(gam_prev, s_factor0=1e-4, delta_tau=0.05, refine_iters=1)
+            t_sr = eng.timings()["sr_total"]
+            res[name] = dict(S=S, t_s=t_s, t_sr=t_sr, step=g_new - gam_prev)
+        base = res["fp64"]
+        fp64 = {}
+        for name, r in res.items():
+            ach = f_s / r["t_s"] / 1e12
+            fp64[name] = dict(s_build_s=r["t_s"], sr_step_s=r["t_sr"], tflops=ach, peak_tflops=peaks[name],
+                              mfma_frac=ach / peaks[name],
+                              s_rel_err=float(np.linalg.norm(r["S"] - base["S"]) / np.linalg.norm(base["S"])),
+                              step_rel_err=float(np.linalg.norm(r["step"] - base["step"]) /
+                                                 np.linalg.norm(base["step"])))
+            log(f"S build {name}: {r['t_s'] * 1e3:.3f} ms ({ach:.1f} TFLOP/s, {100 * ach / peaks[name]:.1f}% of "
+                f"{peaks[name]:.0f}), SR step {r['t_sr'] * 1e3:.2f} ms, S err {fp64[name]['s_rel_err']:.2e}, "
+                f"step err {fp64[name]['step_rel_err']:.2e}")
+        eng.set_s_precision(S_MODES[args.s_precision])
 
     samples = ns_step * args.steps
     acc = sum(r["nacc"] for r in recs) / samples / N
     sweeps_per_launch = args.walkers * sweeps_per_launch_walker
     t_sweep = sum(r["sweep_kernel"] for r in recs) / len(recs)
-    # algorithmic HBM bytes per sweep of one walker: each of the N proposals
-    # reads every copy's X^-1 (8 C N^2 bytes); an accepted one (fraction A)
-    # adds the three passes of calculate_new_InvM_two (pfupdate_two.f90: the
-    # read forming its update vectors, then the read and write of the update):
-    # 8 C N^3 (1 + 3A)
-    b_unit = 8.0 * Cc * N ** 3 * (1.0 + 3.0 * acc)
+    # algorithmic HBM bytes per sweep of one walker.  X^-1 of each copy is
+    # held as its N^2/2 independent entries (folded skew storage): each of
+    # the N proposals reads them once (8 C N^2/2 bytes) and an accepted one
+    # (fraction A) reads and writes them once more: 4 C N^3 (1 + 2A).
+    b_unit = 4.0 * Cc * N ** 3 * (1.0 + 2.0 * acc)
     achieved = b_unit * sweeps_per_launch / t_sweep / 1e9
-    # this kernel's own minimum: the update vectors come from the ratio pass,
-    # so an accepted move adds only the read and write of the update
-    b_min = 8.0 * Cc * N ** 3 * (1.0 + 2.0 * acc)
+    # the same sweep priced with the reference's layout and passes (full
+    # N x N X^-1; calculate_new_InvM_two reads it again to form its update
+    # vectors, pfupdate_two.f90): 8 C N^3 (1 + 3A) -- an equivalent rate
+    b_ref = 8.0 * Cc * N ** 3 * (1.0 + 3.0 * acc)
     # measured HBM bytes per walker sweep of this kernel and workload
     # (rocprofv3 FETCH_SIZE and WRITE_SIZE, each scaled by its reading for a
     # copy of known size), when the committed counter summary covers it
@@ -236,7 +262,7 @@ def main():
                    "walkers": args.walkers, "samples_per_sr_step": ns_step,
                    "sweeps_per_walker_per_step": sweeps_per_launch_walker,
                    "recompute_k": args.recompute_k,
-                   "s_precision": "fp32 products, fp64 centring/shift/solve",
+                   "s_precision": f"{args.s_precision} products on MFMA, fp64 centring/shift/solve",
                    "measurement": "fused into sampler" if args.fused_measure else "recompute per sample",
                    "cond_guard": args.cond_guard,
                    "parallelism": "single_gpu"},
@@ -249,7 +275,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_sweep", "launch_s": t_sweep,
-                     "achieved_min_traffic_GBs": b_min * sweeps_per_launch / t_sweep / 1e9},
+                     "bytes_per_walker_sweep": b_unit,
+                     "reference_layout_equiv_GBs": b_ref * sweeps_per_launch / t_sweep / 1e9},
         "cpu_baseline": cpu,
     }
     if fp64:

@@ -513,13 +513,15 @@ static int build_s(rbmpp_handle* h) {
     return rc;
   const int rows = h->real_only ? n : 2 * n;
   const int nt = syrk_tiles(nv), np = nt * (nt + 1) / 2;
-  int nsplit = std::max(1, std::min(64, (2048 + np - 1) / np));
+  // row splits: about two workgroups per CU (256 CUs)
+  int nsplit = std::max(1, std::min(64, (512 + np - 1) / np));
   nsplit = std::max(1, std::min(nsplit, rows / 64));
-  if ((rc = ensure_part(h, (size_t)nsplit * np * 64 * 64))) return rc;
+  const size_t ts = syrk_tile_size();
+  if ((rc = ensure_part(h, (size_t)nsplit * np * ts * ts))) return rc;
   // obar to device
   HIPCHK(h, hipMemcpyAsync(h->d_vec, h->m_o.data(), sizeof(double) * nv, hipMemcpyHostToDevice, h->st));
   launch_row_kappa(n, h->real_only ? 0 : 1, h->d_mpsi, h->d_rw, h->d_kappa, h->st);
-  const int prec = h->cfg.s_precision == RBMPP_S_FP32 ? 1 : 0;
+  const int prec = h->cfg.s_precision;  // RBMPP_S_FP64 / _FP32 / _BF16X3
   tic(h);
   launch_syrk(prec, rows, nv, h->d_orows, h->d_kappa, h->d_vec, nsplit, h->d_part, 1.0 / h->m_norm, h->d_S,
               h->st);

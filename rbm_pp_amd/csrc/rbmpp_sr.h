@@ -12,6 +12,7 @@ void launch_gvec_partials(int ns, int nv, int cplx_rows, const double* orows, co
 void launch_row_kappa(int ns, int cplx_rows, const double2* psi, const double* rw, double* kappa,
                       hipStream_t st);
 int syrk_tiles(int nv);
+int syrk_tile_size();
 void launch_syrk(int precision, int nrows, int nv, const double* orows, const double* kappa, const double* obar,
                  int nsplit, double* part, double inv_norm, double* S, hipStream_t st);
 void launch_scale_shift(int nv, int nv_rbm, double scal, double shift, const double* S, double* A,

@@ -1,9 +1,10 @@
 // Reductions over the measured samples and the linear algebra of the SR step
-// on gfx950: centred S build (SYRK over the O rows, FP64 centring, FP64 or
-// FP32 products), PP-block scaling and diagonal shift, blocked FP64 Cholesky
+// on gfx950: centred S build (SYRK over the O rows on the matrix cores, FP64
+// centring, FP64 / FP32 / split-BF16x3 products), PP-block scaling and diagonal shift, blocked FP64 Cholesky
 // (replaces ScaLAPACK pdposv, SR_pdposv.f90:232), triangular solves,
 // residual / lambda products.
 #include "rbmpp_sr.h"
+#include <type_traits>
 
 // ---------------------------------------------------------------- sums
 // main.f90:351-361 operands: per-block partials of
@@ -78,83 +79,209 @@ __global__ void k_row_kappa(int ns, int cplx_rows, const double2* __restrict__ p
 }
 
 // ---------------------------------------------------------------- S build
-// Upper-triangle 64x64 tiles of sum_rows c c^T, c = O_row - kappa_row * obar
-// (centring in FP64 before any rounding to the product precision:
-// S = <O O^T>/norm - <O><O>^T, SR_pdposv.f90:11-15).  grid.x enumerates the
-// tile pairs ti <= tj, grid.y splits the rows; each split writes an FP64
-// partial tile.  T = double: FP64 FMA; T = float: FP32 products, FP32
-// accumulation inside a split, FP64 across splits.
-template <typename T>
-__global__ __launch_bounds__(256) void k_syrk_centred(int nrows, int nv, const double* __restrict__ orows,
-                                                      const double* __restrict__ kappa,
-                                                      const double* __restrict__ obar, int ntile,
-                                                      double* part) {
-  constexpr int TS = 64, KC = 32;
-  __shared__ T As[KC][TS + 1];
-  __shared__ T Bs[KC][TS + 1];
-  // decode upper-triangle tile pair
+// S = <O O^T>/norm - <O><O>^T (SR_pdposv.f90:11-15) as a SYRK on the matrix
+// cores over the centred rows c = O_row - kappa_row * obar.  The centring is
+// done in FP64 while a 32-row stage is staged into LDS; the products then run
+// in the configured precision:
+//   MODE 0: FP64 MFMA (v_mfma_f64_16x16x4_f64), FP64 accumulation;
+//   MODE 1: FP32 MFMA (v_mfma_f32_16x16x4_f32), FP32 accumulation;
+//   MODE 2: split BF16 x 3 (v_mfma_f32_16x16x32_bf16): c = hi + lo with
+//           hi = bf16(c), lo = bf16(c - hi); c c' ~ hi hi' + hi lo' + lo hi',
+//           FP32 accumulation.
+// FP32 accumulators are added into FP64 ones every SYRK_FLUSH stages; row
+// splits (grid.y) are summed in FP64 by k_syrk_finish.  grid.x enumerates the
+// 128 x 128 upper-triangle tile pairs; the 4 waves own 64 x 64 quadrants of
+// 16 x 16 MFMA blocks.
+constexpr int ST = 128, SK = 32, SYRK_FLUSH = 32;
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+template <int MODE>
+struct SyrkLds {
+  // MODE 0/1: panels [k][column] (+pad); MODE 2: hi/lo panels [column][k]
+  static constexpr int KP = SK + 8;
+  typedef typename std::conditional<MODE == 0, double, float>::type T;
+  union {
+    struct { T a[SK][ST + 4]; T b[SK][ST + 4]; } kc;
+    struct { __bf16 ahi[ST][KP], alo[ST][KP], bhi[ST][KP], blo[ST][KP]; } bf;
+  };
+};
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_syrk_mfma(int nrows, int nv, const double* __restrict__ orows,
+                                                   const double* __restrict__ kappa,
+                                                   const double* __restrict__ obar, int ntile, double* part) {
+  __shared__ SyrkLds<MODE> L;
   int p = blockIdx.x, ti = 0;
   while (p >= ntile - ti) { p -= ntile - ti; ti++; }
   const int tj = ti + p;
-  const int chunk = (nrows + gridDim.y - 1) / gridDim.y;
+  const bool diag = ti == tj;
+  const int chunk = ((nrows + gridDim.y - 1) / gridDim.y + SK - 1) / SK * SK;
   const int r0 = blockIdx.y * chunk, r1 = min(nrows, r0 + chunk);
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  T acc[4][4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wi = wave >> 1, wj = wave & 1;
+  // staging: column c of both panels, rows hk*16 .. hk*16+15 of the stage
+  const int c = tid & (ST - 1), hk = tid >> 7;
+  const int ci = ti * ST + c, cj = tj * ST + c;
+  const double oa = ci < nv ? obar[ci] : 0.0, ob = cj < nv ? obar[cj] : 0.0;
+
+  f64x4 acc64[4][4];
+  f32x4 acc32[4][4];
 #pragma unroll
-  for (int a = 0; a < 4; a++)
+  for (int x = 0; x < 4; x++)
 #pragma unroll
-    for (int b = 0; b < 4; b++) acc[a][b] = (T)0;
-  for (int k0 = r0; k0 < r1; k0 += KC) {
-    for (int e = threadIdx.x; e < KC * TS; e += 256) {
-      const int kk = e / TS, c = e - kk * TS;
-      const int row = k0 + kk;
-      const int ci = ti * TS + c, cj = tj * TS + c;
-      double va = 0.0, vb = 0.0;
+    for (int y = 0; y < 4; y++) {
+      acc64[x][y] = f64x4{0.0, 0.0, 0.0, 0.0};
+      acc32[x][y] = f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+  int nstage = 0;
+  // the stage's rows are loaded one stage ahead, so the global loads of
+  // stage s+1 are in flight while the matrix cores work on stage s
+  double va[16], vb[16];
+  auto load_stage = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      const int row = k0 + hk * 16 + u;
+      double x = 0.0, y = 0.0;
       if (row < r1) {
         const double kap = kappa[row];
-        if (ci < nv) va = orows[(size_t)row * nv + ci] - kap * obar[ci];
-        if (cj < nv) vb = orows[(size_t)row * nv + cj] - kap * obar[cj];
+        const double* orow = orows + (size_t)row * nv;
+        if (ci < nv) x = orow[ci] - kap * oa;
+        if (!diag && cj < nv) y = orow[cj] - kap * ob;
       }
-      As[kk][c] = (T)va;
-      Bs[kk][c] = (T)vb;
+      va[u] = x;
+      vb[u] = y;
+    }
+  };
+  if (r0 < r1) load_stage(r0);
+  for (int k0 = r0; k0 < r1; k0 += SK) {
+    // ---- stage rows k0..k0+SK-1 (centred in FP64, converted once)
+    if constexpr (MODE == 2) {
+#pragma unroll
+      for (int g = 0; g < 2; g++) {
+        bf16x8 h, l, hb, lb;
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+          const double x = va[8 * g + t], y = vb[8 * g + t];
+          const __bf16 xh = (__bf16)(float)x, yh = (__bf16)(float)y;
+          h[t] = xh;
+          l[t] = (__bf16)(float)(x - (double)(float)xh);
+          hb[t] = yh;
+          lb[t] = (__bf16)(float)(y - (double)(float)yh);
+        }
+        *(bf16x8*)&L.bf.ahi[c][hk * 16 + 8 * g] = h;
+        *(bf16x8*)&L.bf.alo[c][hk * 16 + 8 * g] = l;
+        if (!diag) {
+          *(bf16x8*)&L.bf.bhi[c][hk * 16 + 8 * g] = hb;
+          *(bf16x8*)&L.bf.blo[c][hk * 16 + 8 * g] = lb;
+        }
+      }
+    } else {
+      typedef typename SyrkLds<MODE>::T T;
+#pragma unroll
+      for (int u = 0; u < 16; u++) {
+        L.kc.a[hk * 16 + u][c] = (T)va[u];
+        if (!diag) L.kc.b[hk * 16 + u][c] = (T)vb[u];
+      }
     }
     __syncthreads();
-#pragma unroll 8
-    for (int kk = 0; kk < KC; kk++) {
-      T av[4], bv[4];
+    if (k0 + SK < r1) load_stage(k0 + SK);
+    // ---- products of this wave's 4 x 4 blocks
+    const int ib = wi * 64 + (lane & 15), jb = wj * 64 + (lane & 15);
+    if constexpr (MODE == 2) {
+      const int kk = 8 * (lane >> 4);
+      bf16x8 ah[4], al[4], bh[4], bl[4];
 #pragma unroll
-      for (int a = 0; a < 4; a++) av[a] = As[kk][ty * 4 + a];
+      for (int x = 0; x < 4; x++) {
+        ah[x] = *(const bf16x8*)&L.bf.ahi[ib + 16 * x][kk];
+        al[x] = *(const bf16x8*)&L.bf.alo[ib + 16 * x][kk];
+        bh[x] = diag ? *(const bf16x8*)&L.bf.ahi[jb + 16 * x][kk] : *(const bf16x8*)&L.bf.bhi[jb + 16 * x][kk];
+        bl[x] = diag ? *(const bf16x8*)&L.bf.alo[jb + 16 * x][kk] : *(const bf16x8*)&L.bf.blo[jb + 16 * x][kk];
+      }
 #pragma unroll
-      for (int b = 0; b < 4; b++) bv[b] = Bs[kk][tx * 4 + b];
+      for (int x = 0; x < 4; x++)
 #pragma unroll
-      for (int a = 0; a < 4; a++)
+        for (int y = 0; y < 4; y++) {
+          acc32[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[x], bh[y], acc32[x][y], 0, 0, 0);
+          acc32[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bl[y], acc32[x][y], 0, 0, 0);
+          acc32[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bh[y], acc32[x][y], 0, 0, 0);
+        }
+    } else {
 #pragma unroll
-        for (int b = 0; b < 4; b++) acc[a][b] = fma(av[a], bv[b], acc[a][b]);
+      for (int s = 0; s < SK / 4; s++) {
+        const int kr = 4 * s + (lane >> 4);
+        typename SyrkLds<MODE>::T av[4], bv[4];
+#pragma unroll
+        for (int x = 0; x < 4; x++) {
+          av[x] = L.kc.a[kr][ib + 16 * x];
+          bv[x] = diag ? L.kc.a[kr][jb + 16 * x] : L.kc.b[kr][jb + 16 * x];
+        }
+#pragma unroll
+        for (int x = 0; x < 4; x++)
+#pragma unroll
+          for (int y = 0; y < 4; y++) {
+            if constexpr (MODE == 0)
+              acc64[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[x], bv[y], acc64[x][y], 0, 0, 0);
+            else
+              acc32[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[x], bv[y], acc32[x][y], 0, 0, 0);
+          }
+      }
+    }
+    if constexpr (MODE != 0) {
+      if (++nstage == SYRK_FLUSH) {
+        nstage = 0;
+#pragma unroll
+        for (int x = 0; x < 4; x++)
+#pragma unroll
+          for (int y = 0; y < 4; y++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+              acc64[x][y][r] += (double)acc32[x][y][r];
+              acc32[x][y][r] = 0.f;
+            }
+      }
     }
     __syncthreads();
   }
+  if constexpr (MODE != 0) {
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+      for (int y = 0; y < 4; y++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) acc64[x][y][r] += (double)acc32[x][y][r];
+  }
+  // ---- FP64 partial tile of this split.  Output element (row, col) of a
+  // 16 x 16 block: FP32 MFMA lane l, register r -> (4 (l/16) + r, l % 16);
+  // FP64 MFMA -> (l/16 + 4 r, l % 16).
   const size_t tile = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
-  double* out = part + tile * TS * TS;
+  double* out = part + tile * ST * ST;
 #pragma unroll
-  for (int a = 0; a < 4; a++)
+  for (int x = 0; x < 4; x++)
 #pragma unroll
-    for (int b = 0; b < 4; b++) out[(ty * 4 + a) * TS + tx * 4 + b] = (double)acc[a][b];
+    for (int y = 0; y < 4; y++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = wi * 64 + 16 * x + (MODE == 0 ? (lane >> 4) + 4 * r : 4 * (lane >> 4) + r);
+        const int j = wj * 64 + 16 * y + (lane & 15);
+        out[i * ST + j] = acc64[x][y][r];
+      }
 }
 
 // Sum the split partials of each tile pair, scale by 1/norm and write both
 // triangles of S (row-major nv x nv).
 __global__ void k_syrk_finish(int nv, int ntile, int npairs, int nsplit, const double* __restrict__ part,
                               double inv_norm, double* S) {
-  constexpr int TS = 64;
   int p = blockIdx.x, ti = 0;
   while (p >= ntile - ti) { p -= ntile - ti; ti++; }
   const int tj = ti + p;
-  for (int e = threadIdx.x; e < TS * TS; e += blockDim.x) {
-    const int a = e / TS, b = e - a * TS;
-    const int i = ti * TS + a, j = tj * TS + b;
+  for (int e = threadIdx.x; e < ST * ST; e += blockDim.x) {
+    const int a = e / ST, b = e - a * ST;
+    const int i = ti * ST + a, j = tj * ST + b;
     if (i >= nv || j >= nv) continue;
     double t = 0;
-    for (int z = 0; z < nsplit; z++) t += part[((size_t)z * npairs + blockIdx.x) * TS * TS + e];
+    for (int z = 0; z < nsplit; z++) t += part[((size_t)z * npairs + blockIdx.x) * ST * ST + e];
     t *= inv_norm;
     S[(size_t)i * nv + j] = t;
     S[(size_t)j * nv + i] = t;
@@ -340,16 +467,18 @@ void launch_row_kappa(int ns, int cplx_rows, const double2* psi, const double* r
                       hipStream_t st) {
   hipLaunchKernelGGL(k_row_kappa, dim3((ns + 255) / 256), dim3(256), 0, st, ns, cplx_rows, psi, rw, kappa);
 }
-int syrk_tiles(int nv) { return (nv + 63) / 64; }
+int syrk_tiles(int nv) { return (nv + ST - 1) / ST; }
+int syrk_tile_size() { return ST; }
 void launch_syrk(int precision, int nrows, int nv, const double* orows, const double* kappa, const double* obar,
                  int nsplit, double* part, double inv_norm, double* S, hipStream_t st) {
   const int nt = syrk_tiles(nv), np = nt * (nt + 1) / 2;
-  if (precision == 1)
-    hipLaunchKernelGGL(k_syrk_centred<float>, dim3(np, nsplit), dim3(256), 0, st, nrows, nv, orows, kappa,
-                       obar, nt, part);
+  const dim3 grid(np, nsplit);
+  if (precision == 2)
+    hipLaunchKernelGGL(k_syrk_mfma<2>, grid, dim3(256), 0, st, nrows, nv, orows, kappa, obar, nt, part);
+  else if (precision == 1)
+    hipLaunchKernelGGL(k_syrk_mfma<1>, grid, dim3(256), 0, st, nrows, nv, orows, kappa, obar, nt, part);
   else
-    hipLaunchKernelGGL(k_syrk_centred<double>, dim3(np, nsplit), dim3(256), 0, st, nrows, nv, orows, kappa,
-                       obar, nt, part);
+    hipLaunchKernelGGL(k_syrk_mfma<0>, grid, dim3(256), 0, st, nrows, nv, orows, kappa, obar, nt, part);
   hipLaunchKernelGGL(k_syrk_finish, dim3(np), dim3(256), 0, st, nv, nt, np, nsplit, part, inv_norm, S);
 }
 void launch_scale_shift(int nv, int nv_rbm, double scal, double shift, const double* S, double* A,

@@ -202,21 +202,25 @@ def test_fused_measurement_matches_recompute(oracle_lib, L, ar, ac):
     eng.sr_step(eng.gamma(), s_factor0=1e-3)
 
 
-def test_s_build_fp32_error_bound(oracle_lib):
-    """Mixed S: centred in FP64, products in FP32; stated bound 1e-5
-    normwise relative to the FP64 S on this workload."""
+@pytest.mark.parametrize("prec,bound", [("S_FP32", 1e-5), ("S_BF16X3", 5e-5)])
+def test_s_build_mixed_error_bound(oracle_lib, prec, bound):
+    """Mixed S on the matrix cores: centred in FP64, products in FP32 or in
+    split BF16 x 3 (hi*hi + hi*lo + lo*hi), FP32 accumulation; stated bounds
+    normwise relative to the FP64 S on this workload (nv = 218: a partial
+    128-column tile)."""
     api = _api()
     spec, p = _model(6, 2, 0)
     n = 256
     xs = _configs(spec.N, n, seed=7)
     e64 = api.Engine(spec, n_walkers=1, n_samples=n, params=p, s_precision=api.S_FP64)
-    e32 = api.Engine(spec, n_walkers=1, n_samples=n, params=p, s_precision=api.S_FP32)
-    for e in (e64, e32):
+    emx = api.Engine(spec, n_walkers=1, n_samples=n, params=p, s_precision=getattr(api, prec))
+    for e in (e64, emx):
         e.set_samples(xs)
         e.calc_phys_quantity_opt(shift=1e-3)
-    S64, S32 = e64.smat(), e32.smat()
-    err = np.linalg.norm(S32 - S64) / np.linalg.norm(S64)
-    assert err < 1e-5, err
+    S64, Smx = e64.smat(), emx.smat()
+    err = np.linalg.norm(Smx - S64) / np.linalg.norm(S64)
+    assert err < bound, err
+    assert np.array_equal(Smx, Smx.T)
 
 
 def test_vmc_energy_4x4_against_enumeration(oracle_lib):
