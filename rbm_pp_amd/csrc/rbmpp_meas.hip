@@ -141,7 +141,7 @@ __global__ __launch_bounds__(MNT) void k_meas(LocalArgs a) {
             TILE(x, aa, b) = (ri >= 0 && rj[b] >= 0) ? S[(size_t)ri * 2 * N + rj[b]] : 0.0;
         }
       }
-      const double pfq = reg_skew_pf_inverse(x, N, &s.rowbuf[0][0][0], s.sel, s.selv);
+      const double pfq = reg_skew_pf_inverse<true>(x, N, &s.rowbuf[0][0][0], s.sel, s.selv);
       if (pfq == 0.0) {
         if (tid == 0) s.status = WS_SINGULAR;
         __syncthreads();

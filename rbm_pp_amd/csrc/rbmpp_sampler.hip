@@ -132,7 +132,7 @@ __device__ __attribute__((noinline)) void wg_recompute(const DevModel& m, SweepS
         for (int b = 0; b < RB; b++)
           TILE(x, aa, b) = (ri >= 0 && rj[b] >= 0) ? S[(size_t)ri * 2 * N + rj[b]] : 0.0;
       }
-      pfq = reg_skew_pf_inverse(x, N, s.u, s.gsel, s.gselv);
+      pfq = reg_skew_pf_inverse<false>(x, N, s.u, s.gsel, s.gselv);
       if (pfq == 0.0) {
         if (tid == 0) s.status = WS_SINGULAR;
         __syncthreads();

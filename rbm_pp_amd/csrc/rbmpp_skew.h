@@ -68,16 +68,45 @@ struct Done128 {
   }
 };
 
+// One DPP step of the (|v|, index) arg-max: take the candidate moved in by
+// CTRL when it is larger, or equal with a smaller index.  Lanes whose source
+// lies outside the pattern keep their own candidate.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ void amax_step(double& bv, int& bi) {
+  const long long b = __builtin_bit_cast(long long, bv);
+  const int lo = (int)(b & 0xffffffffll), hi = (int)(b >> 32);
+  const int olo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, ROWMASK, 0xf, false);
+  const int ohi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, ROWMASK, 0xf, false);
+  const int oi = __builtin_amdgcn_update_dpp(bi, bi, CTRL, ROWMASK, 0xf, false);
+  const double ov = __builtin_bit_cast(double, ((long long)ohi << 32) | (unsigned)olo);
+  if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+}
+// Arg-max over the wave, valid in lane 63: row_shr 1, 2, 4, 8 reduce each row
+// of 16 lanes into its last lane, row_bcast 15 and 31 carry rows into the next.
+__device__ __forceinline__ void wave_amax_to63(double& bv, int& bi) {
+  amax_step<0x111, 0xf>(bv, bi);
+  amax_step<0x112, 0xf>(bv, bi);
+  amax_step<0x114, 0xf>(bv, bi);
+  amax_step<0x118, 0xf>(bv, bi);
+  amax_step<0x142, 0xa>(bv, bi);
+  amax_step<0x143, 0xc>(bv, bi);
+}
+
 // Pfaffian and inverse of the skew-symmetric n x n matrix held in the
 // register tile; on return the tile holds X^-1.  Each step eliminates the
-// pivot block {c, r}: with a = X(c, r) the exchange gives
+// pivot block {c, r}, r the first index of the largest |X(c, j)| among the
+// remaining j (the Parlett-Reid rule).  With a = X(c, r) the exchange gives
 //   X'(c,r) = -1/a, X'(r,c) = 1/a, X'(c,j) = -X(r,j)/a, X'(r,j) = X(c,j)/a,
 //   X'(i,c) = -X(i,r)/a, X'(i,r) = X(i,c)/a,
 //   X'(i,j) = X(i,j) + (X(i,c) X(r,j) - X(i,r) X(c,j))/a,
 // and Pf(X) = (-1)^{#remaining between c and r} a Pf(Schur complement).
+// EXACT keeps the tile exactly skew (the two products of each element are
+// rounded apart, so X'(j,i) = -X'(i,j) and the diagonal stays 0); otherwise
+// the update is two FMAs on prescaled pivot rows.
 // Returns 0 for an exactly singular pivot.  Called by all 1024 threads.
+template <bool EXACT>
 __device__ __forceinline__ double reg_skew_pf_inverse(tile_t& x, int n, double* rowbuf, int* sel, double* selv) {
-  const int tid = threadIdx.x, tr = tid >> 6, tc = tid & 63;
+  const int tid = threadIdx.x, tr = uniform_i(tid >> 6), tc = tid & 63;
   Done128 done;
   {
     uint32_t w[4];
@@ -92,9 +121,11 @@ __device__ __forceinline__ double reg_skew_pf_inverse(tile_t& x, int n, double* 
   for (int step = 0; 2 * step < n; step++) {
     const int sb = step & 1;
     const int c = done.first_free();
-    // pivot search along row c (held by wave c & 15), which also publishes row c
+    // pivot search along row c by the wave holding it, which also publishes
+    // row c (with 1 in place of X(c,c) = 0: see the update below)
     if (tr == (c & 15)) {
-      double bv = -1.0, sv = 0.0;
+      const int ac = c >> 4;
+      double bv = -1.0;
       int bi = MPAD;
 #pragma unroll
       for (int b = 0; b < RB; b++) {
@@ -102,23 +133,18 @@ __device__ __forceinline__ double reg_skew_pf_inverse(tile_t& x, int n, double* 
         double v = 0.0;
 #pragma unroll
         for (int aa = 0; aa < RA; aa++)
-          if (tr + 16 * aa == c) v = TILE(x, aa, b);
-        // published with 1 in place of X(c,c) = 0: see the update below
+          if (aa == ac) v = TILE(x, aa, b);
         rowbuf[(2 * sb + 0) * MPAD + j] = j == c ? 1.0 : v;
         const uint32_t w = tc < 32 ? done.word(2 * b) : done.word(2 * b + 1);
         const bool ok = !((w >> (tc & 31)) & 1u) && j != c;
-        if (ok && fabs(v) > bv) { bv = fabs(v); sv = v; bi = j; }
+        if (ok && fabs(v) > bv) { bv = fabs(v); bi = j; }
       }
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) {
-        const double ov = __shfl_xor(bv, o, RBMPP_WAVE);
-        const double osv = __shfl_xor(sv, o, RBMPP_WAVE);
-        const int oi = __shfl_xor(bi, o, RBMPP_WAVE);
-        if (ov > bv || (ov == bv && oi < bi)) { bv = ov; sv = osv; bi = oi; }
-      }
+      wave_amax_to63(bv, bi);
+      const int r = __builtin_amdgcn_readlane(bi, 63);
+      wave_sync();
       if (tc == 0) {
-        sel[sb] = bi;
-        selv[sb] = sv;
+        sel[sb] = r;
+        selv[sb] = r < MPAD ? rowbuf[(2 * sb + 0) * MPAD + r] : 0.0;
       }
     }
     __syncthreads();
@@ -149,43 +175,53 @@ __device__ __forceinline__ double reg_skew_pf_inverse(tile_t& x, int n, double* 
       rci[aa] = rowbuf[(2 * sb + 0) * MPAD + tr + 16 * aa];
       rri[aa] = rowbuf[(2 * sb + 1) * MPAD + tr + 16 * aa];
     }
+    double rcs[RB], rrs[RB];
+#pragma unroll
+    for (int b = 0; b < RB; b++) { rcs[b] = rcj[b] * inva; rrs[b] = rrj[b] * inva; }
     // Schur-complement update of every element.  With the published rows
     // holding 1 at (c,c) and (r,r), the same expression also gives the new
     // pivot columns: X(i,c) + (X(r,i) * 1 - X(c,i) * X(r,c)) / a = X(r,i)/a =
     // -X(i,r)/a, and X(i,r) + (X(r,i) * X(c,r) - X(c,i) * 1) / a = X(i,c)/a.
-    // The diagonal stays exactly zero (both products are the same numbers).
 #pragma unroll
     for (int aa = 0; aa < RA; aa++)
 #pragma unroll
       for (int b = 0; b < RB; b++) {
-        // rounded products kept apart, so X'(j,i) = -X'(i,j) exactly
-        double t;
-        {
+        if constexpr (EXACT) {
+          double t;
+          {
 #pragma clang fp contract(off)
-          t = rri[aa] * rcj[b] - rci[aa] * rrj[b];
-        }
-        TILE(x, aa, b) = fma(t, inva, TILE(x, aa, b));
-      }
-    // The pivot rows are set apart (a wave-uniform test, compile-time tile
-    // indices).
-#pragma unroll
-    for (int aa = 0; aa < RA; aa++) {
-      const int i = tr + 16 * aa;
-      if (i == c) {
-#pragma unroll
-        for (int b = 0; b < RB; b++) {
-          const int j = tc + 64 * b;
-          TILE(x, aa, b) = j == c ? 0.0 : (j == r ? -inva : -rrj[b] * inva);
-        }
-      } else if (i == r) {
-#pragma unroll
-        for (int b = 0; b < RB; b++) {
-          const int j = tc + 64 * b;
-          TILE(x, aa, b) = j == r ? 0.0 : (j == c ? inva : rcj[b] * inva);
+            t = rri[aa] * rcj[b] - rci[aa] * rrj[b];
+          }
+          TILE(x, aa, b) = fma(t, inva, TILE(x, aa, b));
+        } else {
+          TILE(x, aa, b) = fma(rri[aa], rcs[b], fma(-rci[aa], rrs[b], TILE(x, aa, b)));
         }
       }
+    // the pivot rows, set apart by the (at most two) waves holding them
+    if (tr == (c & 15)) {
+      const int ac = c >> 4;
+#pragma unroll
+      for (int aa = 0; aa < RA; aa++)
+        if (aa == ac) {
+#pragma unroll
+          for (int b = 0; b < RB; b++) {
+            const int j = tc + 64 * b;
+            TILE(x, aa, b) = j == c ? 0.0 : (j == r ? -inva : -(EXACT ? rrj[b] * inva : rrs[b]));
+          }
+        }
     }
-
+    if (tr == (r & 15)) {
+      const int ar = r >> 4;
+#pragma unroll
+      for (int aa = 0; aa < RA; aa++)
+        if (aa == ar) {
+#pragma unroll
+          for (int b = 0; b < RB; b++) {
+            const int j = tc + 64 * b;
+            TILE(x, aa, b) = j == r ? 0.0 : (j == c ? inva : (EXACT ? rcj[b] * inva : rcs[b]));
+          }
+        }
+    }
     done.mark(c);
     done.mark(r);
   }
