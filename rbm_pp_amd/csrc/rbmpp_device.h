@@ -128,6 +128,32 @@ __device__ __forceinline__ double wave_max(double v) {
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, RBMPP_WAVE));
   return v;
 }
+// v + (v moved by the DPP pattern CTRL); lanes without a source add 0
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_add(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, ROWMASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWMASK, 0xf, false);
+  return v + __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+// Sum over the wave, valid in lane 63 (row_shr 1, 2, 4, 8 within rows of 16
+// lanes, then row_bcast 15 and 31 across rows).
+__device__ __forceinline__ double wave_sum_to63(double v) {
+  v = dpp_add<0x111, 0xf>(v);
+  v = dpp_add<0x112, 0xf>(v);
+  v = dpp_add<0x114, 0xf>(v);
+  v = dpp_add<0x118, 0xf>(v);
+  v = dpp_add<0x142, 0xa>(v);
+  v = dpp_add<0x143, 0xc>(v);
+  return v;
+}
+// the value of lane `l` (wave-uniform) in every lane
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
 
 // The RBM tables an amplitude update reads: the model's own (global memory)
 // or copies staged in LDS by the kernel

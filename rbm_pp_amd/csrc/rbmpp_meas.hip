@@ -33,7 +33,7 @@ constexpr int MLD = MNMAX + 1;          // LDS leading dimension bound (n + 1)
 
 struct MeasSmem {
   double inv[MNMAX * MLD];       // inv[j * (n+1) + i] = InvM(j, i) = X^-1(i, j)
-  double2 avec[MNW][MNMAX];      // per wave: a vectors of its two bonds
+  double2 avec[MNW][MNMAX];      // a vectors of the bonds in flight (4 per bond wave)
   double rowbuf[2][2][MPAD];     // pivot rows c and r, double buffered by step
   double opp[2][4 * MNMAX];      // pair-product O, re / im
   double2 psib[4 * MNMAX];       // per antiparallel bond: sum over copies of psi(x')
@@ -198,14 +198,17 @@ __global__ __launch_bounds__(MNT) void k_meas(LocalArgs a) {
       }
       __syncthreads();  // inv complete
       // exchange ratios (calculate_new_PfM_two_child, pfupdate_two.f90:27-88):
-      // wave w takes the bond pairs w, w + 16, ...
-      double2* av = s.avec[wid];
-      for (int p = wid; 2 * p < nb; p += MNW) {
-        const bool two = 2 * p + 1 < nb;
-        int bi[2], i1[2], i2[2], msa[2], msb[2], rsa[2], rsb[2];
+      // waves 0..BW-1 take BT bonds per pass (bonds BT*p .. BT*p+BT-1 for
+      // p = w, w + BW, ...); every row of X^-1 read from LDS feeds BT dot
+      // products.  The pass is latency-bound: two bonds on all 16 waves beat
+      // four bonds on 8.
+      constexpr int BT = 2, BW = MNW * 2 / BT;
+      double* av = (double*)&s.avec[0][0] + (size_t)wid * BT * MNMAX;  // a_t(j) at av[BT*j + t]
+      for (int p = wid; wid < BW && BT * p < nb; p += BW) {
+        int bi[BT], i1[BT], i2[BT], msa[BT], msb[BT], rsa[BT], rsb[BT];
 #pragma unroll
-        for (int t = 0; t < 2; t++) {
-          bi[t] = (t == 1 && !two) ? 2 * p : 2 * p + t;
+        for (int t = 0; t < BT; t++) {
+          bi[t] = min(BT * p + t, nb - 1);  // a missing bond repeats the last one (not written)
           i1[t] = s.bi1[bi[t]];
           i2[t] = s.bi2[bi[t]];
           msa[t] = s.ecfg[i1[t]];
@@ -214,17 +217,17 @@ __global__ __launch_bounds__(MNT) void k_meas(LocalArgs a) {
           rsb[t] = i1[t] + (msb[t] >= np ? N : 0);
         }
         // a (to LDS) and b (registers) at electrons j = lane, lane + 64
-        double bv[2][2];
+        double bv[BT][2];
 #pragma unroll
         for (int hh = 0; hh < 2; hh++) {
           const int j = lane + RBMPP_WAVE * hh;
-          double av_[2] = {0.0, 0.0};
-          bv[0][hh] = 0.0;
-          bv[1][hh] = 0.0;
+          double av_[BT];
+#pragma unroll
+          for (int t = 0; t < BT; t++) { av_[t] = 0.0; bv[t][hh] = 0.0; }
           if (j < N) {
             const int off = j >= np ? N : 0;
 #pragma unroll
-            for (int t = 0; t < 2; t++) {
+            for (int t = 0; t < BT; t++) {
               int tt = s.eidx[j];
               if (j == msa[t]) tt = i2[t];
               else if (j == msb[t]) tt = i1[t];
@@ -232,27 +235,32 @@ __global__ __launch_bounds__(MNT) void k_meas(LocalArgs a) {
               av_[t] = S[(size_t)rsa[t] * 2 * N + rr];
               bv[t][hh] = S[(size_t)rsb[t] * 2 * N + rr];
             }
-            av[j] = make_double2(av_[0], av_[1]);
+#pragma unroll
+            for (int t = 0; t < BT; t += 2) *(double2*)&av[BT * j + t] = make_double2(av_[t], av_[t + 1]);
           }
         }
         wave_sync();
         // P_t(i) = sum_j InvM(j, i) a_t(j) for i = lane, lane + 64
-        double P00 = 0.0, P01 = 0.0, P10 = 0.0, P11 = 0.0;  // P[t][half]
+        double Pt[BT][2];
+#pragma unroll
+        for (int t = 0; t < BT; t++) { Pt[t][0] = 0.0; Pt[t][1] = 0.0; }
 #pragma unroll 4
         for (int j = 0; j < N; j++) {
-          const double2 aj = av[j];
           const double* row = s.inv + j * ld;
           const double m0 = h0 ? row[lane] : 0.0;
           const double m1 = h1 ? row[lane + RBMPP_WAVE] : 0.0;
-          P00 = fma(m0, aj.x, P00);
-          P10 = fma(m0, aj.y, P10);
-          P01 = fma(m1, aj.x, P01);
-          P11 = fma(m1, aj.y, P11);
-        }
-        const double Pt[2][2] = {{P00, P01}, {P10, P11}};
 #pragma unroll
-        for (int t = 0; t < 2; t++) {
-          if (t == 1 && !two) break;  // wave-uniform
+          for (int t = 0; t < BT; t += 2) {
+            const double2 aj = *(const double2*)&av[BT * j + t];
+            Pt[t][0] = fma(m0, aj.x, Pt[t][0]);
+            Pt[t + 1][0] = fma(m0, aj.y, Pt[t + 1][0]);
+            Pt[t][1] = fma(m1, aj.x, Pt[t][1]);
+            Pt[t + 1][1] = fma(m1, aj.y, Pt[t + 1][1]);
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < BT; t++) {
+          if (BT * p + t >= nb) break;  // wave-uniform
           // b^T P, and Q(msa), Q(msb) = -sum_j InvM(msa/msb, j) b(j) (skew symmetry)
           const double* ra = s.inv + msa[t] * ld;
           const double* rb = s.inv + msb[t] * ld;
@@ -264,21 +272,17 @@ __global__ __launch_bounds__(MNT) void k_meas(LocalArgs a) {
             qa_ = fma(ra[lane + RBMPP_WAVE], bv[t][1], qa_);
             qb_ = fma(rb[lane + RBMPP_WAVE], bv[t][1], qb_);
           }
-          const double bMa = wave_sum(bp);
-          const double qa = -wave_sum(qa_);
-          const double qb = -wave_sum(qb_);
-          // P(msa), P(msb), b(msa) from their owning lanes
-          const double pa0 = __shfl(Pt[t][0], msa[t] & 63, RBMPP_WAVE);
-          const double pa1 = __shfl(Pt[t][1], msa[t] & 63, RBMPP_WAVE);
-          const double pb0 = __shfl(Pt[t][0], msb[t] & 63, RBMPP_WAVE);
-          const double pb1 = __shfl(Pt[t][1], msb[t] & 63, RBMPP_WAVE);
-          const double ba0 = __shfl(bv[t][0], msa[t] & 63, RBMPP_WAVE);
-          const double ba1 = __shfl(bv[t][1], msa[t] & 63, RBMPP_WAVE);
-          const double pa = msa[t] >= RBMPP_WAVE ? pa1 : pa0;
-          const double pb = msb[t] >= RBMPP_WAVE ? pb1 : pb0;
-          const double vec_ba = msa[t] >= RBMPP_WAVE ? ba1 : ba0;
-          if (lane == 0) {
-            const double invm_ab = s.inv[msb[t] * ld + msa[t]];  // InvM(msb, msa)
+          // wave sums (DPP, valid in lane 63) and the entries of P and b at
+          // the electrons msa, msb read from their owning lanes
+          const double bMa = wave_sum_to63(bp);
+          const double qa = -wave_sum_to63(qa_);
+          const double qb = -wave_sum_to63(qb_);
+          const int la = uniform_i(msa[t]), lb = uniform_i(msb[t]);
+          const double pa = readlane_d(la >= RBMPP_WAVE ? Pt[t][1] : Pt[t][0], la & 63);
+          const double pb = readlane_d(lb >= RBMPP_WAVE ? Pt[t][1] : Pt[t][0], lb & 63);
+          const double vec_ba = readlane_d(la >= RBMPP_WAVE ? bv[t][1] : bv[t][0], la & 63);
+          if (lane == 63) {
+            const double invm_ab = s.inv[lb * ld + la];  // InvM(msb, msa)
             s.rat[bi[t]] = invm_ab * vec_ba + invm_ab * bMa + pa * qb - pb * qa;
           }
         }
