@@ -36,8 +36,10 @@ typedef struct rbmpp_handle rbmpp_handle;
 #define RBMPP_E_SLATER (-5)       /* Slater_Elm not skew-symmetric (slater.f90:51-56) */
 #define RBMPP_E_STATE (-6)        /* call out of order (e.g. sr_step before measure) */
 
-/* S-matrix precision (the sampler, X^-1 chain, centring, shift and solve
-   are FP64 in every mode) */
+/* S-matrix precision: products of the S build on the matrix cores in FP64,
+   FP32, or split BF16 x 3 (c = hi + lo, hi*hi + hi*lo + lo*hi) with FP32
+   accumulation.  The sampler, X^-1 chain, centring, shift and solve are FP64
+   in every mode. */
 #define RBMPP_S_FP64 0
 #define RBMPP_S_FP32 1
 #define RBMPP_S_BF16X3 2
