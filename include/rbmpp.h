@@ -44,6 +44,13 @@ typedef struct rbmpp_handle rbmpp_handle;
 #define RBMPP_S_FP32 1
 #define RBMPP_S_BF16X3 2
 
+/* Spin-projection weights (InitSPWeight): RBMPP_PROJ_NORMAL is
+   normal_code/spin_projection.f90:5-79 (beta = {0, pi}); RBMPP_PROJ_TRANSVERSE
+   is transverse_code/spin_projection.f90:30-38 (beta = {-pi/2, pi/2},
+   cos and sin of beta/2 scaled by sqrt(2)). */
+#define RBMPP_PROJ_NORMAL 0
+#define RBMPP_PROJ_TRANSVERSE 1
+
 typedef struct {
   int32_t L;                 /* linear size; N = L*L sites (main.f90:77,85) */
   int32_t n_sym;             /* N_MP_Trans, number of symmetry operations (main.f90:91) */
@@ -116,6 +123,11 @@ int rbmpp_set_params(rbmpp_handle* h, const double* wcirr, const double* bcirr,
                      const double* wrirr, const double* brirr, const double* slater);
 /* gamma vector in the order of main.f90:263-294 (SR_pdposv.f90:385-412) */
 int rbmpp_set_gamma(rbmpp_handle* h, const double* gamma);
+/* Selects the spin-projection weights (RBMPP_PROJ_*) and rebuilds
+ * Slater_Elm with the current parameters: the InitSPWeight call of
+ * main.f90:202, compiled from the normal_code or the transverse_code tree.
+ * rbmpp_create starts with RBMPP_PROJ_NORMAL. */
+int rbmpp_set_projection(rbmpp_handle* h, int32_t projection);
 int rbmpp_get_gamma(const rbmpp_handle* h, double* gamma);
 
 /* make_sample (make_sample.f90:4-240) for all walkers; the samples stay on

@@ -52,6 +52,7 @@ def lib():
         L.om_nv_rbm.restype = C.c_int; L.om_nv_rbm.argtypes = [P]
         L.om_set_params.argtypes = [P, dp, dp, dp, dp, dp]
         L.om_set_gamma.argtypes = [P, dp]
+        L.om_set_projection.argtypes = [P, C.c_int]
         L.om_get_gamma.argtypes = [P, dp]
         L.om_slater_elm.restype = C.POINTER(C.c_double); L.om_slater_elm.argtypes = [P]
         L.om_sp_weights.argtypes = [P, dp]
@@ -187,6 +188,13 @@ class Oracle:
         n = self.C * 4 * self.N * self.N
         p = lib().om_slater_elm(self.h)
         return np.ctypeslib.as_array(p, shape=(n,)).copy().reshape(self.C, 2 * self.N, 2 * self.N).transpose(0, 2, 1)
+
+    def set_projection(self, proj: int) -> None:
+        """0: spin weights of normal_code; 1: of transverse_code
+        (transverse_code/spin_projection.f90:30-38)."""
+        rc = lib().om_set_projection(self.h, int(proj))
+        if rc:
+            raise RuntimeError(f"om_set_projection: {rc}")
 
     def sp_weights(self) -> np.ndarray:
         out = np.zeros(12)

@@ -177,13 +177,16 @@ int om_bond(int L, int type, int i1) {
 }
 
 /* spin_projection.f90:5-79 (InitSPWeight) with N_SP_Gauss_Leg = 2:
- * beta = {0, pi}; cos/sin of beta/2 and products; weights (1, spin_parity). */
-static void init_sp_weight(om_model* m) {
+ * beta = {0, pi}; cos/sin of beta/2 and products; weights (1, spin_parity).
+ * proj = 1: the transverse_code variant (transverse_code/spin_projection.f90:
+ * 30-38), beta = {-pi/2, pi/2} and sqrt(2) cos/sin of beta/2. */
+static void init_sp_weight(om_model* m, int proj) {
   const double pi = acos(-1.0);
-  const double beta[2] = {0.0, pi};
+  const double beta[2] = {proj ? -0.5 * pi : 0.0, proj ? 0.5 * pi : pi};
+  const double amp = proj ? sqrt(2.0) : 1.0;
   for (int s = 0; s < 2; s++) {
-    m->sp_cos[s] = cos(0.5 * beta[s]);
-    m->sp_sin[s] = sin(0.5 * beta[s]);
+    m->sp_cos[s] = proj ? amp * cos(0.5 * beta[s]) : cos(0.5 * beta[s]);
+    m->sp_sin[s] = proj ? amp * sin(0.5 * beta[s]) : sin(0.5 * beta[s]);
     m->sp_cc[s] = m->sp_cos[s] * m->sp_cos[s];
     m->sp_cs[s] = m->sp_cos[s] * m->sp_sin[s];
     m->sp_ss[s] = m->sp_sin[s] * m->sp_sin[s];
@@ -225,7 +228,7 @@ om_model* om_create(int L, int P, int alphac, int alphar, int n_slater, double J
   /* interface_for_PP.f90:110-136 (define_sublattice) */
   m->subB = (unsigned char*)malloc(N);
   for (int s = 0; s < N; s++) m->subB[s] = (unsigned char)(((s / L) + (s % L)) & 1);
-  init_sp_weight(m);
+  init_sp_weight(m, 0);
   m->Wc = (cplx*)calloc((size_t)alphac * N + 1, sizeof(cplx));
   m->bc = (cplx*)calloc((size_t)alphac + 1, sizeof(cplx));
   m->Wr = (double*)calloc((size_t)alphar * N + 1, sizeof(double));
@@ -302,6 +305,13 @@ int om_get_gamma(const om_model* m, double* gam) {
     for (int iw = 0; iw < N; iw++) gam[k++] = m->Wr[f * N + iw];
   for (int o = 0; o < m->nsl; o++) gam[k++] = m->f[o];
   return k;
+}
+
+/* Spin-projection variant: 0 normal_code, 1 transverse_code (the weights of
+ * InitSPWeight and the Slater_Elm built from them). */
+int om_set_projection(om_model* m, int proj) {
+  init_sp_weight(m, proj);
+  return update_slater_elements(m);
 }
 
 /* SR_pdposv.f90:385-412 (unpacking gamma) */

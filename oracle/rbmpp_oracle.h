@@ -49,6 +49,8 @@ int  om_nv_rbm(const om_model* m);
 int  om_set_params(om_model* m, const double* Wc, const double* bc,
                    const double* Wr, const double* br, const double* f);
 int  om_set_gamma(om_model* m, const double* gamma);
+/* 0: normal_code weights; 1: transverse_code/spin_projection.f90:30-38. */
+int  om_set_projection(om_model* m, int proj);
 int  om_get_gamma(const om_model* m, double* gamma);
 const double* om_slater_elm(const om_model* m);
 void om_sp_weights(const om_model* m, double* out /* cos,sin,cc,cs,ss,w  x2 */);

@@ -23,6 +23,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "librbmpp.so")
 
 S_FP64, S_FP32, S_BF16X3 = 0, 1, 2
+PROJ_NORMAL, PROJ_TRANSVERSE = 0, 1
 
 
 class RbmppError(RuntimeError):
@@ -84,6 +85,7 @@ def lib():
         L.rbmpp_get_smat.argtypes = [vp, dp]
         L.rbmpp_timings.argtypes = [vp, dp]
         L.rbmpp_set_s_precision.argtypes = [vp, C.c_int32]
+        L.rbmpp_set_projection.argtypes = [vp, C.c_int32]
         _lib = L
     return _lib
 
@@ -249,6 +251,12 @@ class Engine:
         if want_o:
             out["oloc"] = o.view(np.complex128).reshape(n, self.nv)
         return out
+
+    def set_projection(self, projection: int) -> None:
+        """Spin-projection weights: PROJ_NORMAL (normal_code) or
+        PROJ_TRANSVERSE (transverse_code/spin_projection.f90:30-38);
+        rebuilds Slater_Elm."""
+        self._check(lib().rbmpp_set_projection(self.h, int(projection)))
 
     def set_s_precision(self, prec: int) -> None:
         self._check(lib().rbmpp_set_s_precision(self.h, prec))

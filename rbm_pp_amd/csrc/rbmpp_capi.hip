@@ -33,6 +33,7 @@ struct rbmpp_handle {
   std::vector<unsigned char> subB;
   std::vector<double> chi, gamma;
   double sp_cs[2], sp_cc[2], sp_ss[2], sp_w[2];
+  int projection = 0;  // RBMPP_PROJ_*
   std::vector<void*> allocs;
   // tables and parameters
   int *d_wmap = nullptr, *d_invmap = nullptr, *d_orb = nullptr;
@@ -105,6 +106,21 @@ static DevModel devmodel(const rbmpp_handle* h) {
   m.wmap = h->d_wmap; m.invmap = h->d_invmap; m.orb = h->d_orb; m.subB = h->d_subB; m.chi = h->d_chi;
   m.se = h->d_se; m.Wr = h->d_Wr; m.br = h->d_br; m.Wc = h->d_Wc; m.bc = h->d_bc;
   return m;
+}
+
+// InitSPWeight (spin_projection.f90:5-79): the two-point rule at beta = {0, pi};
+// transverse_code/spin_projection.f90:30-38 takes beta = {-pi/2, pi/2} and
+// scales cos and sin of beta/2 by sqrt(2).  para_spin_proj is the same in both.
+static void set_sp_weights(rbmpp_handle* h, int projection) {
+  const double pi = std::acos(-1.0);
+  const bool tr = projection == RBMPP_PROJ_TRANSVERSE;
+  const double beta[2] = {tr ? -0.5 * pi : 0.0, tr ? 0.5 * pi : pi};
+  for (int s = 0; s < 2; s++) {
+    double c = std::cos(0.5 * beta[s]), sn = std::sin(0.5 * beta[s]);
+    if (tr) { c *= std::sqrt(2.0); sn *= std::sqrt(2.0); }
+    h->sp_cc[s] = c * c; h->sp_cs[s] = c * sn; h->sp_ss[s] = sn * sn;
+  }
+  h->projection = projection;
 }
 
 static void tic(rbmpp_handle* h) { hipEventRecord(h->e0, h->st); }
@@ -181,13 +197,7 @@ int rbmpp_create(const rbmpp_config* cfg, const int32_t* w_map, const double* pa
     if (h->chi[2 * g + 1] != 0.0) h->real_only = 0;
   h->subB.resize(N);
   for (int s = 0; s < N; s++) h->subB[s] = (unsigned char)(((s / L) + (s % L)) & 1);
-  // InitSPWeight (spin_projection.f90:5-79) with beta = {0, pi}
-  const double pi = std::acos(-1.0);
-  const double beta[2] = {0.0, pi};
-  for (int s = 0; s < 2; s++) {
-    const double c = std::cos(0.5 * beta[s]), sn = std::sin(0.5 * beta[s]);
-    h->sp_cc[s] = c * c; h->sp_cs[s] = c * sn; h->sp_ss[s] = sn * sn;
-  }
+  set_sp_weights(h, RBMPP_PROJ_NORMAL);
   h->sp_w[0] = 1.0;
   h->sp_w[1] = cfg->spin_parity == 1 ? 1.0 : -1.0;
   h->gamma.assign(h->nv, 0.0);
@@ -234,6 +244,15 @@ int rbmpp_create(const rbmpp_config* cfg, const int32_t* w_map, const double* pa
   HIPCHK(h, hipMemcpyAsync(ws.ctr, zc.data(), sizeof(unsigned long long) * W, hipMemcpyHostToDevice, h->st));
   HIPCHK(h, hipStreamSynchronize(h->st));
   return RBMPP_OK;
+}
+
+int rbmpp_set_projection(rbmpp_handle* h, int32_t projection) {
+  if (!h || (projection != RBMPP_PROJ_NORMAL && projection != RBMPP_PROJ_TRANSVERSE)) return RBMPP_E_ARG;
+  set_sp_weights(h, projection);
+  const DevModel hm = devmodel(h);
+  HIPCHK(h, hipMemcpyAsync(h->d_model, &hm, sizeof(DevModel), hipMemcpyHostToDevice, h->st));
+  const std::vector<double> g = h->gamma;  // Slater_Elm depends on the weights
+  return rbmpp_set_gamma(h, g.data());
 }
 
 int rbmpp_set_params(rbmpp_handle* h, const double* wcirr, const double* bcirr, const double* wrirr,

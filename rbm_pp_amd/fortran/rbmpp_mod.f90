@@ -23,6 +23,7 @@ module rbmpp_mod
 
   integer(c_int), parameter, public :: RBMPP_OK = 0
   integer(c_int), parameter, public :: RBMPP_S_FP64 = 0, RBMPP_S_FP32 = 1, RBMPP_S_BF16X3 = 2
+  integer(c_int), parameter, public :: RBMPP_PROJ_NORMAL = 0, RBMPP_PROJ_TRANSVERSE = 1
 
   ! struct rbmpp_config (field order and kinds as in rbmpp.h)
   type, bind(C), public :: rbmpp_config
@@ -45,7 +46,7 @@ module rbmpp_mod
   end type
 
   public :: rbmpp_create, rbmpp_destroy, rbmpp_last_error, rbmpp_nv
-  public :: rbmpp_set_params, rbmpp_set_gamma, rbmpp_get_gamma
+  public :: rbmpp_set_params, rbmpp_set_gamma, rbmpp_get_gamma, rbmpp_set_projection
   public :: rbmpp_make_sample, rbmpp_measure, rbmpp_sr_step, rbmpp_set_s_precision
   public :: rbmpp_measure_energy, rbmpp_set_samples, rbmpp_get_samples, rbmpp_n_samples
   public :: rbmpp_get_walkers, rbmpp_eval, rbmpp_ratio_exchange, rbmpp_local
@@ -122,6 +123,12 @@ module rbmpp_mod
       type(rbmpp_sr_config), intent(in) :: c
       real(c_double), intent(inout) :: gamma(*)
       type(rbmpp_sr_report), intent(out) :: rep
+    end function
+
+    integer(c_int) function rbmpp_set_projection(h, projection) bind(C, name="rbmpp_set_projection")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: projection
     end function
 
     integer(c_int) function rbmpp_set_s_precision(h, prec) bind(C, name="rbmpp_set_s_precision")

@@ -5,7 +5,11 @@
 ! W.input (the reference draws W from a clock-seeded stream when W.input is
 ! absent, parameter.f90:21-41; this driver requires the file).
 !
-!   rbmpp_vmc [walkers [s_precision [recompute_k]]]
+!   rbmpp_vmc [walkers [s_precision [recompute_k [projection]]]]
+!
+! projection 1 runs the transverse_code variant: its spin-projection weights
+! (transverse_code/spin_projection.f90:30-38) and its reweighting-offset
+! factor 0.003 (transverse_code/main.f90:451) in place of 0.001.
 !
 ! Writes Energy_vs_Iteration.txt (main.f90:410), optimized_W.txt and
 ! zqp_orbital_opt.dat (parameter.f90:238-276), energy_after_opt.txt and
@@ -28,11 +32,12 @@ program rbmpp_vmc
   type(c_ptr) :: h
   integer(c_int) :: rc
   integer(c_int64_t) :: naccept
-  integer :: walkers, nspw, prec, kre, it, i, nv, nav_chk
+  integer :: walkers, nspw, prec, kre, proj, it, i, nv, nav_chk
+  real(8) :: shift_factor
   real(8) :: etot, etot2, norm, wf_av, shift, shift_av, wf_av_list(10), acc, e_av, e_er, v_av, v_er
   character(len=64) :: arg
 
-  walkers = 4096; prec = RBMPP_S_FP64; kre = 32
+  walkers = 4096; prec = RBMPP_S_FP64; kre = 32; proj = RBMPP_PROJ_NORMAL
   if (command_argument_count() >= 1) then
     call get_command_argument(1, arg); read(arg, *) walkers
   end if
@@ -42,6 +47,10 @@ program rbmpp_vmc
   if (command_argument_count() >= 3) then
     call get_command_argument(3, arg); read(arg, *) kre
   end if
+  if (command_argument_count() >= 4) then
+    call get_command_argument(4, arg); read(arg, *) proj
+  end if
+  shift_factor = merge(0.003d0, 0.001d0, proj == RBMPP_PROJ_TRANSVERSE)
 
   call read_inputs()
   nspw = (Nsample + walkers - 1) / walkers
@@ -57,6 +66,10 @@ program rbmpp_vmc
   allocate(gvec(nv), ovec(nv), gam(nv), gam_av(nv), e_data(max(Nav, 1)), e2_data(max(Nav, 1)))
   rc = rbmpp_set_params(h, wc, bc, wr, br, slater)
   call check(rc, "rbmpp_set_params")
+  if (proj /= RBMPP_PROJ_NORMAL) then
+    rc = rbmpp_set_projection(h, proj)
+    call check(rc, "rbmpp_set_projection")
+  end if
 
   ! optimisation (main.f90:300-455)
   src%s_factor0 = s_factor0; src%delta_tau = delta_tau; src%max_change = 0.04d0
@@ -90,7 +103,7 @@ program rbmpp_vmc
     ! reweighting offset from the running mean of wf_av (main.f90:441-451)
     wf_av_list(1:9) = wf_av_list(2:10)
     wf_av_list(10) = wf_av
-    shift = sum(wf_av_list) / 10d0 * 0.001d0 / dble(N) / dble(N)
+    shift = sum(wf_av_list) / 10d0 * shift_factor / dble(N) / dble(N)
   end do
   close(10)
   if (Nav <= 0) then

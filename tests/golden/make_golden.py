@@ -2,7 +2,7 @@
 routines (oracle/_ref/libref.so, built by oracle/Makefile from the reference
 sources).  Run in a container that holds the reference sources:
 
-    make -C oracle ref && python tests/golden/make_golden.py
+    make -C oracle ref && python tests/golden/make_golden.py [name ...]
 
 Inputs are generated here from fixed seeds.  Pf and InvM are inputs to the
 reference routines and come from the CPU oracle, because the reference's own
@@ -26,18 +26,24 @@ CASES = {
     "ref_4x4_a1c0": dict(L=4, alphar=1, alphac=0, J2=0.5, nconf=8, seed=11),
     "ref_4x4_a2c1": dict(L=4, alphar=2, alphac=1, J2=0.5, nconf=8, seed=12),
     "ref_6x6_a2c0": dict(L=6, alphar=2, alphac=0, J2=0.5, nconf=4, seed=13),
+    # spin weights of transverse_code/spin_projection.f90
+    "ref_4x4_a2c1_tc": dict(L=4, alphar=2, alphac=1, J2=0.5, nconf=8, seed=14, projection=1),
 }
 
 
-def main():
+def main(names=None):
     for name, c in CASES.items():
+        if names and name not in names:
+            continue
+        proj = c.get("projection", 0)
         spec = lt.j1j2_model(c["L"], alphar=c["alphar"], alphac=c["alphac"], J2=c["J2"])
         p = lt.random_params(spec, seed_w=c["seed"], seed_f=c["seed"] + 100, sigma_w=0.05)
         orc = O.Oracle(spec, p)
-        ref = RH.Ref(spec, p)
+        orc.set_projection(proj)
+        ref = RH.Ref(spec, p, projection=proj)
         rng = np.random.default_rng(c["seed"])
         N = spec.N
-        out = dict(L=c["L"], alphar=c["alphar"], alphac=c["alphac"], J2=c["J2"],
+        out = dict(L=c["L"], alphar=c["alphar"], alphac=c["alphac"], J2=c["J2"], projection=proj,
                    Wc=p.Wc, bc=p.bc, Wr=p.Wr, br=p.br, f=p.f,
                    slater_elm_q0=ref.slater_elm()[0], sp=ref.sp())
         keys = ["x", "thetac", "thetar", "ele_idx", "ele_cfg", "sgn", "pf_in", "psi",
@@ -73,6 +79,8 @@ def main():
             out["invm_two"] = np.array(invm_upd)
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
         print("wrote", name)
+    if names and "ref_mt19937" not in names:
+        return
     # mt.f stream
     spec = lt.j1j2_model(4, alphar=1, J2=0.5)
     ref = RH.Ref(spec, lt.random_params(spec))
@@ -83,4 +91,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -12,23 +12,27 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(ROOT, "oracle", "_ref", "libref.so")
+# the same routines with transverse_code/spin_projection.f90
+LIB_TC = os.path.join(ROOT, "oracle", "_ref", "libref_tc.so")
 
 dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 
 
-def available() -> bool:
-    return os.path.exists(LIB)
+def available(projection: int = 0) -> bool:
+    return os.path.exists(LIB_TC if projection else LIB)
 
 
 class Ref:
-    """One process-wide instance: the reference keeps its state in modules."""
+    """One process-wide instance per library: the reference keeps its state
+    in modules.  projection 1 loads the transverse_code build."""
 
-    _lib = None
+    _libs = {}
 
-    def __init__(self, spec, params):
-        if Ref._lib is None:
-            L = C.CDLL(LIB)
+    def __init__(self, spec, params, projection: int = 0):
+        path = LIB_TC if projection else LIB
+        if path not in Ref._libs:
+            L = C.CDLL(path)
             L.ref_init.argtypes = [C.c_int] * 5 + [C.c_double, C.c_double, C.c_int, ip, dp, ip]
             L.ref_set_params.argtypes = [dp] * 5
             L.ref_get_slater_elm.argtypes = [dp]
@@ -45,8 +49,8 @@ class Ref:
             L.ref_bond_map.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_int)]
             L.ref_sgrnd.argtypes = [C.c_int]
             L.ref_grnd_n.argtypes = [C.c_int, dp]
-            Ref._lib = L
-        self.L = Ref._lib
+            Ref._libs[path] = L
+        self.L = Ref._libs[path]
         self.spec = spec
         s = spec
         chi = np.ascontiguousarray(np.stack([s.chi.real, s.chi.imag], -1).reshape(-1))

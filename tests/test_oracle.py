@@ -79,12 +79,14 @@ def test_pfaffian_brute_force_and_det(oracle_lib):
 
 
 # ------------------------------------------------- reference golden vectors
-@pytest.mark.parametrize("name", ["ref_4x4_a1c0", "ref_4x4_a2c1", "ref_6x6_a2c0"])
+@pytest.mark.parametrize("name", ["ref_4x4_a1c0", "ref_4x4_a2c1", "ref_6x6_a2c0", "ref_4x4_a2c1_tc"])
 def test_oracle_matches_reference_goldens(oracle_lib, name):
     g = np.load(os.path.join(GOLD, name + ".npz"))
     spec = lt.j1j2_model(int(g["L"]), alphar=int(g["alphar"]), alphac=int(g["alphac"]), J2=float(g["J2"]))
     p = lt.Params(Wc=g["Wc"], bc=g["bc"], Wr=g["Wr"], br=g["br"], f=g["f"])
     orc = oracle_lib.Oracle(spec, p)
+    if "projection" in g.files:
+        orc.set_projection(int(g["projection"]))
     assert np.array_equal(orc.slater_elm()[0], g["slater_elm_q0"])
     assert np.array_equal(orc.sp_weights(), g["sp"])
     for k in range(g["x"].shape[0]):
@@ -108,11 +110,16 @@ def test_oracle_matches_reference_goldens(oracle_lib, name):
             np.testing.assert_allclose(inv, g["invm_two"][k], rtol=0, atol=1e-13 * np.abs(inv).max())
 
 
-@pytest.mark.skipif(not RH.available(), reason="reference routines not built here")
-def test_oracle_matches_live_reference(oracle_lib):
+@pytest.mark.parametrize("projection", [0, 1])
+def test_oracle_matches_live_reference(oracle_lib, projection):
+    if not RH.available(projection):
+        pytest.skip("reference routines not built here")
     spec, p = _model(L=6, alphar=2, alphac=1, seed=7)
     orc = oracle_lib.Oracle(spec, p)
-    ref = RH.Ref(spec, p)
+    orc.set_projection(projection)
+    ref = RH.Ref(spec, p, projection=projection)
+    assert np.array_equal(orc.sp_weights(), ref.sp())
+    assert np.array_equal(orc.slater_elm(), ref.slater_elm())
     rng = np.random.default_rng(0)
     for _ in range(3):
         x = _rand_x(rng, spec.N)
