@@ -71,6 +71,9 @@ def lib():
         L.om_measure.argtypes = [P, C.c_int, ip, C.c_double, C.c_int, dp, dp, dp,
                                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.om_measure_energy.argtypes = [P, C.c_int, ip, C.c_double, dp]
+        L.om_bond_sdots.restype = C.c_double; L.om_bond_sdots.argtypes = [P, P, dp, dp]
+        L.om_local_correlation.argtypes = [P, P, dp]
+        L.om_measure_correlation.argtypes = [P, C.c_int, ip, C.c_double, dp, dp, dp]
         L.om_make_sample.argtypes = [P, P, P, C.c_int, C.c_void_p, C.POINTER(C.c_int),
                                      C.c_void_p, C.c_void_p, C.c_void_p,
                                      C.POINTER(C.c_long), C.POINTER(C.c_long)]
@@ -280,6 +283,31 @@ class Oracle:
         if rc:
             raise RuntimeError(f"om_measure_energy: {rc}")
         return dict(Etot=sums[0], Etot2=sums[1], norm=sums[2], wf_av=sums[3])
+
+    def bond_sdots(self):
+        """(E_loc, SdotS[4, N]) of the evaluated configuration."""
+        e = np.zeros(2); sd = np.zeros(8 * self.N)
+        lib().om_bond_sdots(self.h, self.st, e, sd)
+        return complex(e[0], e[1]), sd.view(np.complex128).reshape(4, self.N)
+
+    def local_correlation(self) -> np.ndarray:
+        """Chi_loc_x[Nchi, 2] (SzSz, exchange) of the evaluated configuration."""
+        nchi = self.N * (self.N + 1) // 2
+        out = np.zeros(4 * nchi)
+        lib().om_local_correlation(self.h, self.st, out)
+        return out.view(np.complex128).reshape(nchi, 2)
+
+    def measure_correlation(self, xs, shift=0.0) -> dict:
+        """Un-normalised sums of the correlation run over the samples."""
+        xs = np.ascontiguousarray(xs, np.int32)
+        nchi = self.N * (self.N + 1) // 2
+        sums = np.zeros(6); sp = np.zeros(2 * nchi); vb = np.zeros(4 * nchi)
+        rc = lib().om_measure_correlation(self.h, xs.shape[0], xs.reshape(-1), shift, sums, sp, vb)
+        if rc:
+            raise RuntimeError(f"om_measure_correlation: {rc}")
+        return dict(Etot=sums[0], Etot2=sums[1], norm=sums[2], wf_av=sums[3],
+                    wf_sgn=complex(sums[4], sums[5]), chi_sp=sp.reshape(nchi, 2),
+                    chi_vbs=vb.reshape(4, nchi))
 
     def make_sample(self, rng: Rng, first: bool, nsample: int, nwarmup: int,
                     nupdate: int | None = None, fixed_k: int = 0, ref_draws: bool = True,

@@ -993,6 +993,117 @@ int om_measure_energy(const om_model* m, int ns, const int* xs, double shift, do
   return rc;
 }
 
+/* Calc_Correlation/calc_phys_quantity.f90:348-413 (its
+ * calc_local_energy_correlation): the local energy together with the per-bond
+ * SdotS(i1, t) = x_i1 x_i2 psi + [x_i1 /= x_i2] 2 sgn_t psi(x'), sgn_t = -1
+ * for the J1 bonds and +1 for the J2 bonds.  sdots[2*(t*N + i1) + {0,1}]. */
+double om_bond_sdots(const om_model* m, const om_state* s0, double* eloc_out, double* sdots) {
+  om_state* s = (om_state*)s0;
+  const int N = m->N;
+  cplx psi0 = s->psi[0];
+  cplx e = 0.0;
+  for (int t = 0; t < 4; t++) {
+    double J = (t <= 1) ? m->J1 : m->J2;
+    double sg = (t <= 1) ? -1.0 : 1.0;
+    for (int i1 = 0; i1 < N; i1++) {
+      int i2 = om_bond(m->L, t, i1);
+      cplx sd = (double)s->x[i1] * (double)s->x[i2] * psi0;
+      e = e + J * (double)s->x[i1] * (double)s->x[i2] * psi0;
+      if (s->x[i1] != s->x[i2]) {
+        propose_exchange(m, s, i1, i2, NULL, NULL);
+        e = e + 2.0 * J * sg * s->psi2[0];
+        sd = sd + 2.0 * sg * s->psi2[0];
+      }
+      sdots[2 * (t * N + i1)] = creal(sd);
+      sdots[2 * (t * N + i1) + 1] = cimag(sd);
+    }
+  }
+  if (eloc_out) { eloc_out[0] = creal(e); eloc_out[1] = cimag(e); }
+  return creal(e);
+}
+
+/* Calc_Correlation/calc_phys_quantity.f90:469-550 (calc_local_correlation):
+ * for the pairs i1 <= i2, numbered i0 = i2 (i2 + 1) / 2 + i1 (i2 outer, as
+ * the reference's loops), chi[4*i0 + {0,1}] = Chi_loc_x(1, i0) = x_i1 x_i2 psi
+ * and chi[4*i0 + {2,3}] = Chi_loc_x(2, i0): psi on the diagonal, and
+ * s_i1 s_i2 psi(x') (s = -1 on sublattice B) for an exchange of
+ * antiparallel spins. */
+int om_local_correlation(const om_model* m, const om_state* s0, double* chi) {
+  om_state* s = (om_state*)s0;
+  const int N = m->N;
+  cplx psi0 = s->psi[0];
+  size_t i0 = 0;
+  for (int i2 = 0; i2 < N; i2++)
+    for (int i1 = 0; i1 <= i2; i1++, i0++) {
+      cplx zz = (double)s->x[i1] * (double)s->x[i2] * psi0;
+      cplx pm = (i1 == i2) ? psi0 : 0.0;
+      if (s->x[i1] != s->x[i2]) {
+        propose_exchange(m, s, i1, i2, NULL, NULL);
+        double sg = 1.0;
+        if (m->subB[i1]) sg = -sg;
+        if (m->subB[i2]) sg = -sg;
+        pm = sg * s->psi2[0];
+      }
+      chi[4 * i0] = creal(zz); chi[4 * i0 + 1] = cimag(zz);
+      chi[4 * i0 + 2] = creal(pm); chi[4 * i0 + 3] = cimag(pm);
+    }
+  return 0;
+}
+
+/* Calc_Correlation/calc_phys_quantity.f90:264-341 (calc_phys_quantity_after_opt
+ * of the correlation run): un-normalised sums over the samples.
+ * sums = {Etot, Etot2, norm, wf_av, Re wf_sgn, Im wf_sgn};
+ * chi_sp[2*i0 + k] = Chi_sp(k+1, i0); chi_vbs[t*Nchi + i0] = Chi_vbs(i0, t+1). */
+int om_measure_correlation(const om_model* m, int ns, const int* xs, double shift, double* sums,
+                           double* chi_sp, double* chi_vbs) {
+  const int N = m->N;
+  const size_t nchi = (size_t)N * (N + 1) / 2;
+  om_state* s = om_state_create(m);
+  double* sd = (double*)malloc(sizeof(double) * 8 * N);
+  double* cl = (double*)malloc(sizeof(double) * 4 * nchi);
+  double Et = 0, Et2 = 0, nrm = 0, wf = 0;
+  cplx wsg = 0.0;
+  for (size_t k = 0; k < 2 * nchi; k++) chi_sp[k] = 0.0;
+  for (size_t k = 0; k < 4 * nchi; k++) chi_vbs[k] = 0.0;
+  int rc = 0;
+  for (int is = 0; is < ns; is++) {
+    rc = om_eval(m, xs + (size_t)is * N, s, NULL);
+    if (rc) break;
+    cplx psi0 = s->psi[0];
+    double rw = 1.0 / (cabs(psi0) + shift);
+    double el[2];
+    om_bond_sdots(m, s, el, sd);
+    cplx e = CMPLX(el[0], el[1]) * rw;
+    Et = Et + rw * creal(conj(psi0) * e);
+    Et2 = Et2 + creal(conj(e) * e);
+    nrm = nrm + (cabs(psi0) * rw) * (cabs(psi0) * rw);
+    wf = wf + cabs(psi0);
+    wsg = wsg + psi0 / cabs(psi0);
+    for (int k = 0; k < 8 * N; k++) sd[k] *= rw;
+    for (int t = 0; t < 4; t++) {
+      size_t i0 = 0;
+      for (int i2 = 0; i2 < N; i2++)
+        for (int i1 = 0; i1 <= i2; i1++, i0++) {
+          cplx a = CMPLX(sd[2 * (t * N + i1)], sd[2 * (t * N + i1) + 1]);
+          cplx b = CMPLX(sd[2 * (t * N + i2)], sd[2 * (t * N + i2) + 1]);
+          chi_vbs[t * nchi + i0] += creal(conj(a) * b);
+        }
+    }
+    om_local_correlation(m, s, cl);
+    for (size_t i0 = 0; i0 < nchi; i0++)
+      for (int k = 0; k < 2; k++) {
+        cplx c = CMPLX(cl[4 * i0 + 2 * k], cl[4 * i0 + 2 * k + 1]);
+        chi_sp[2 * i0 + k] += rw * rw * creal(conj(psi0) * c);
+      }
+  }
+  sums[0] = Et; sums[1] = Et2; sums[2] = nrm; sums[3] = wf;
+  sums[4] = creal(wsg); sums[5] = cimag(wsg);
+  free(cl);
+  free(sd);
+  om_state_free(s);
+  return rc;
+}
+
 /* ------------------------------------------------------------------ */
 /* sampler                                                              */
 /* ------------------------------------------------------------------ */

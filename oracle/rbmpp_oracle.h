@@ -80,6 +80,11 @@ int om_measure(const om_model* m, int nsample, const int* xs, double shift, int 
                double* smat, double* eloc_out, double* psi_out, double* ostore);
 int om_measure_energy(const om_model* m, int nsample, const int* xs, double shift,
                       double* sums);
+/* Correlation run (Calc_Correlation/calc_phys_quantity.f90:264-550) */
+double om_bond_sdots(const om_model* m, const om_state* s, double* eloc /*2*/, double* sdots /*8N*/);
+int om_local_correlation(const om_model* m, const om_state* s, double* chi /*4*N(N+1)/2*/);
+int om_measure_correlation(const om_model* m, int nsample, const int* xs, double shift,
+                           double* sums /*6*/, double* chi_sp /*N(N+1)*/, double* chi_vbs /*2N(N+1)*/);
 
 /* periodic recompute with the maintained electron order (make_sample.f90:74-128);
    drift = {max |dPf|/<|Pf|>, max |dInvM|/max|InvM|} */

@@ -59,6 +59,9 @@ contains
     end do
     call InitSPWeight(1)
     call define_sublattice()
+#ifdef REF_CC
+    Nchi = N*(N+1)/2
+#endif
   end subroutine
 
   subroutine ref_set_params(wc, bc, wr, br, f) bind(C, name="ref_set_params")
@@ -167,9 +170,42 @@ contains
     real(c_double), intent(in) :: thr(max(Mr,1)), pf(N_QP_Full), invm(N,N,N_QP_Full)
     complex(c_double_complex), intent(out) :: eloc
     integer :: xx(N), ii(N), cc(N), ss(N_MP_Trans)
+#ifdef REF_CC
+    complex(c_double_complex) :: sdots(N,4)
+    xx = x; ii = idx + 1; cc = cfg + 1; ss = sgn
+    call calc_local_energy_correlation(xx, psi0, thc, thr, ii, cc, ss, pf, invm, eloc, sdots)
+#else
     xx = x; ii = idx + 1; cc = cfg + 1; ss = sgn
     call calc_local_energy_correlation(xx, psi0, thc, thr, ii, cc, ss, pf, invm, eloc)
+#endif
   end subroutine
+
+#ifdef REF_CC
+  ! Calc_Correlation tree only: the per-bond SdotS and Chi_loc_x of one sample
+  subroutine ref_eloc_sdots(x, psi0, thc, thr, idx, cfg, sgn, pf, invm, eloc, sdots) &
+      bind(C, name="ref_eloc_sdots")
+    integer(c_int), intent(in) :: x(N), idx(N), cfg(N), sgn(N_MP_Trans)
+    complex(c_double_complex), intent(in) :: psi0
+    complex(c_double_complex), intent(in) :: thc(max(Mc,1))
+    real(c_double), intent(in) :: thr(max(Mr,1)), pf(N_QP_Full), invm(N,N,N_QP_Full)
+    complex(c_double_complex), intent(out) :: eloc, sdots(N,4)
+    integer :: xx(N), ii(N), cc(N), ss(N_MP_Trans)
+    xx = x; ii = idx + 1; cc = cfg + 1; ss = sgn
+    call calc_local_energy_correlation(xx, psi0, thc, thr, ii, cc, ss, pf, invm, eloc, sdots)
+  end subroutine
+
+  subroutine ref_local_correlation(x, psi0, thc, thr, idx, cfg, sgn, pf, invm, chi) &
+      bind(C, name="ref_local_correlation")
+    integer(c_int), intent(in) :: x(N), idx(N), cfg(N), sgn(N_MP_Trans)
+    complex(c_double_complex), intent(in) :: psi0
+    complex(c_double_complex), intent(in) :: thc(max(Mc,1))
+    real(c_double), intent(in) :: thr(max(Mr,1)), pf(N_QP_Full), invm(N,N,N_QP_Full)
+    complex(c_double_complex), intent(out) :: chi(2,Nchi)
+    integer :: xx(N), ii(N), cc(N), ss(N_MP_Trans)
+    xx = x; ii = idx + 1; cc = cfg + 1; ss = sgn
+    call calc_local_correlation(xx, psi0, thc, thr, ii, cc, ss, pf, invm, chi)
+  end subroutine
+#endif
 
   subroutine ref_deriv_pf(idx, pf, invm, deriv) bind(C, name="ref_deriv_pf")
     integer(c_int), intent(in) :: idx(N)

@@ -54,3 +54,31 @@ def hamiltonian(L: int, J1: float, J2: float, gauge: bool):
 def ground_energy(L: int, J1: float, J2: float) -> float:
     _, H = hamiltonian(L, J1, J2, gauge=False)
     return float(sla.eigsh(H, k=1, which="SA")[0][0])
+
+
+def pair_operators(L: int, gauge: bool):
+    """For every pair i1 <= i2 (i2 outer, i1 inner) on the Sz=0 states:
+    the diagonal of sigma^z_i1 sigma^z_i2 and the sparse exchange operator
+    (sigma^x sigma^x + sigma^y sigma^y) / 2 of the pair, i.e. |x> -> |x'> for
+    antiparallel spins (times s_i1 s_i2, s = -1 on sublattice B, in the
+    gauge of the Hamiltonian above) and the identity for i1 == i2."""
+    N = L * L
+    states = sz0_states(N)
+    weights = 1 << np.arange(N, dtype=np.int64)
+    keys = ((states > 0).astype(np.int64) * weights).sum(1)
+    index = {int(k): i for i, k in enumerate(keys)}
+    sub = np.array([((i // L) + (i % L)) & 1 for i in range(N)])
+    out = []
+    n = len(states)
+    for i2 in range(N):
+        for i1 in range(i2 + 1):
+            zz = (states[:, i1] * states[:, i2]).astype(np.float64)
+            if i1 == i2:
+                ex = sp.identity(n, format="csr")
+            else:
+                sg = -1.0 if (gauge and sub[i1] != sub[i2]) else 1.0
+                anti = np.nonzero(states[:, i1] != states[:, i2])[0]
+                rows = [index[int(keys[s]) ^ (1 << i1) ^ (1 << i2)] for s in anti]
+                ex = sp.csr_matrix((np.full(len(anti), sg), (rows, anti)), shape=(n, n))
+            out.append((i1, i2, zz, ex))
+    return states, out

@@ -79,6 +79,8 @@ def main(names=None):
             out["invm_two"] = np.array(invm_upd)
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
         print("wrote", name)
+    if not names or "ref_4x4_a2c1_cc" in names:
+        correlation_case("ref_4x4_a2c1_cc", L=4, alphar=2, alphac=1, nconf=8, seed=15)
     if names and "ref_mt19937" not in names:
         return
     # mt.f stream
@@ -88,6 +90,28 @@ def main(names=None):
                         seed1=ref.grnd(1, 2000), seed4357=ref.grnd(4357, 2000),
                         seed123456789=ref.grnd(123456789, 2000))
     print("wrote ref_mt19937")
+
+
+def correlation_case(name, L, alphar, alphac, nconf, seed):
+    """Per-configuration SdotS and Chi_loc_x of the Calc_Correlation tree
+    (its calc_local_energy_correlation and calc_local_correlation)."""
+    spec = lt.j1j2_model(L, alphar=alphar, alphac=alphac, J2=0.5)
+    p = lt.random_params(spec, seed_w=seed, seed_f=seed + 100, sigma_w=0.05)
+    orc = O.Oracle(spec, p)
+    ref = RH.Ref(spec, p, tree="cc")
+    rng = np.random.default_rng(seed)
+    N = spec.N
+    xs, el, sd, cl = [], [], [], []
+    for _ in range(nconf):
+        x = np.array([1] * (N // 2) + [-1] * (N // 2), np.int32)
+        rng.shuffle(x)
+        st = orc.eval(x)
+        e, s4 = ref.eloc_sdots(st)
+        xs.append(x); el.append(e); sd.append(s4); cl.append(ref.local_correlation(st))
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), L=L, alphar=alphar, alphac=alphac, J2=0.5,
+                        Wc=p.Wc, bc=p.bc, Wr=p.Wr, br=p.br, f=p.f, x=np.array(xs), eloc=np.array(el),
+                        sdots=np.array(sd), chi_loc=np.array(cl))
+    print("wrote", name)
 
 
 if __name__ == "__main__":

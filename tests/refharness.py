@@ -14,23 +14,32 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(ROOT, "oracle", "_ref", "libref.so")
 # the same routines with transverse_code/spin_projection.f90
 LIB_TC = os.path.join(ROOT, "oracle", "_ref", "libref_tc.so")
+# the Calc_Correlation tree, with the correlation entry points
+LIB_CC = os.path.join(ROOT, "oracle", "_ref", "libref_cc.so")
+
+
+def _path(projection: int, tree: str | None) -> str:
+    if tree == "cc":
+        return LIB_CC
+    return LIB_TC if projection else LIB
 
 dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 
 
-def available(projection: int = 0) -> bool:
-    return os.path.exists(LIB_TC if projection else LIB)
+def available(projection: int = 0, tree: str | None = None) -> bool:
+    return os.path.exists(_path(projection, tree))
 
 
 class Ref:
     """One process-wide instance per library: the reference keeps its state
-    in modules.  projection 1 loads the transverse_code build."""
+    in modules.  projection 1 loads the transverse_code build, tree "cc"
+    the Calc_Correlation build."""
 
     _libs = {}
 
-    def __init__(self, spec, params, projection: int = 0):
-        path = LIB_TC if projection else LIB
+    def __init__(self, spec, params, projection: int = 0, tree: str | None = None):
+        path = _path(projection, tree)
         if path not in Ref._libs:
             L = C.CDLL(path)
             L.ref_init.argtypes = [C.c_int] * 5 + [C.c_double, C.c_double, C.c_int, ip, dp, ip]
@@ -49,6 +58,9 @@ class Ref:
             L.ref_bond_map.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_int)]
             L.ref_sgrnd.argtypes = [C.c_int]
             L.ref_grnd_n.argtypes = [C.c_int, dp]
+            if path == LIB_CC:
+                L.ref_eloc_sdots.argtypes = [ip, dp, dp, dp, ip, ip, ip, dp, dp, dp, dp]
+                L.ref_local_correlation.argtypes = [ip, dp, dp, dp, ip, ip, ip, dp, dp, dp]
             Ref._libs[path] = L
         self.L = Ref._libs[path]
         self.spec = spec
@@ -156,3 +168,25 @@ class Ref:
         out = np.zeros(n)
         self.L.ref_grnd_n(n, out)
         return out
+
+    def _state_args(self, st):
+        thc = np.zeros(2 * max(self.mc, 1)); thc[:2 * self.mc] = st["thetac"].view(np.float64)
+        thr = np.zeros(max(self.mr, 1)); thr[:self.mr] = st["thetar"]
+        psi0 = np.array([st["psi"][0].real, st["psi"][0].imag])
+        return (np.ascontiguousarray(st["x"], np.int32), psi0, thc, thr,
+                np.ascontiguousarray(st["ele_idx"], np.int32), np.ascontiguousarray(st["ele_cfg"], np.int32),
+                np.ascontiguousarray(st["sgn"], np.int32), np.ascontiguousarray(st["pf"], np.float64),
+                self._invm(st["invm"]))
+
+    def eloc_sdots(self, st):
+        """Calc_Correlation tree: (E_loc, SdotS[4, N])."""
+        e = np.zeros(2); sd = np.zeros(8 * self.spec.N)
+        self.L.ref_eloc_sdots(*self._state_args(st), e, sd)
+        return complex(e[0], e[1]), sd.view(np.complex128).reshape(4, self.spec.N)
+
+    def local_correlation(self, st):
+        """Calc_Correlation tree: Chi_loc_x as [Nchi, 2]."""
+        N = self.spec.N
+        out = np.zeros(2 * N * (N + 1))
+        self.L.ref_local_correlation(*self._state_args(st), out)
+        return out.view(np.complex128).reshape(N * (N + 1) // 2, 2)

@@ -196,6 +196,101 @@ def test_local_energy_exact_enumeration_4x4(oracle_lib):
     assert e_h >= e0g
 
 
+# ------------------------------------------------ correlation run (Calc_Correlation)
+def _cc_model(g):
+    spec = lt.j1j2_model(int(g["L"]), alphar=int(g["alphar"]), alphac=int(g["alphac"]), J2=float(g["J2"]))
+    return spec, lt.Params(Wc=g["Wc"], bc=g["bc"], Wr=g["Wr"], br=g["br"], f=g["f"])
+
+
+def test_correlation_matches_reference_goldens(oracle_lib):
+    """SdotS per bond and Chi_loc_x per pair, bit-exact against the
+    Calc_Correlation tree's own routines."""
+    g = np.load(os.path.join(GOLD, "ref_4x4_a2c1_cc.npz"))
+    spec, p = _cc_model(g)
+    orc = oracle_lib.Oracle(spec, p)
+    for k in range(g["x"].shape[0]):
+        orc.eval(g["x"][k])
+        e, sd = orc.bond_sdots()
+        assert e == g["eloc"][k]
+        assert np.array_equal(sd, g["sdots"][k])
+        assert np.array_equal(orc.local_correlation(), g["chi_loc"][k])
+
+
+def test_correlation_matches_live_reference(oracle_lib):
+    if not RH.available(tree="cc"):
+        pytest.skip("reference routines not built here")
+    spec, p = _model(L=6, alphar=2, alphac=1, seed=8)
+    orc = oracle_lib.Oracle(spec, p)
+    ref = RH.Ref(spec, p, tree="cc")
+    rng = np.random.default_rng(3)
+    for _ in range(2):
+        st = orc.eval(_rand_x(rng, spec.N))
+        e, sd = orc.bond_sdots()
+        e_r, sd_r = ref.eloc_sdots(st)
+        assert e == e_r and np.array_equal(sd, sd_r)
+        assert np.array_equal(orc.local_correlation(), ref.local_correlation(st))
+
+
+def test_correlation_exact_enumeration_4x4(oracle_lib):
+    """sum_x conj(psi) Chi_loc_x / sum |psi|^2 equals <psi|O|psi>/<psi|psi>
+    for sigma^z sigma^z and the gauged exchange of every pair, with O built
+    from the state list alone; the bond SdotS sum to E_loc."""
+    spec, p = _model(L=4, alphar=1, alphac=1)
+    orc = oracle_lib.Oracle(spec, p)
+    states, ops = ed.pair_operators(4, gauge=True)
+    nchi = len(ops)
+    psi = np.zeros(len(states), complex)
+    acc = np.zeros((nchi, 2), complex)
+    for k, x in enumerate(states.astype(np.int32)):
+        st = orc.eval(x)
+        psi[k] = st["psi"][0]
+        acc += np.conj(psi[k]) * orc.local_correlation()
+        if k % 997 == 0:
+            e, sd = orc.bond_sdots()
+            J = np.array([1.0, 1.0, 0.5, 0.5])
+            assert (J[:, None] * sd).sum() == pytest.approx(e, rel=1e-13, abs=1e-13 * abs(psi[k]))
+    nrm = (np.abs(psi) ** 2).sum()
+    for i0, (i1, i2, zz, ex) in enumerate(ops):
+        assert acc[i0, 0].real / nrm == pytest.approx((np.abs(psi) ** 2 * zz).sum() / nrm, abs=1e-12)
+        assert acc[i0, 1].real / nrm == pytest.approx((np.conj(psi) @ (ex @ psi)).real / nrm, abs=1e-12)
+        assert abs(acc[i0, 1].imag) / nrm < 1e-12
+
+
+def test_correlation_sums(oracle_lib):
+    """om_measure_correlation against the sums of the reference's own
+    per-sample values (goldens) in the reference's accumulation
+    (Calc_Correlation/calc_phys_quantity.f90:318-338), and the closed forms:
+    Chi_sp(.,ii) = norm, sum over i<j of Chi_sp(1,.) = -N norm / 2 (Sz = 0)."""
+    g = np.load(os.path.join(GOLD, "ref_4x4_a2c1_cc.npz"))
+    spec, p = _cc_model(g)
+    orc = oracle_lib.Oracle(spec, p)
+    N, shift = spec.N, 0.3
+    xs = g["x"]
+    got = orc.measure_correlation(xs, shift=shift)
+    nchi = N * (N + 1) // 2
+    sp_ref = np.zeros((nchi, 2)); vb_ref = np.zeros((4, nchi)); norm = 0.0; wsg = 0.0
+    iu = [(i1, i2) for i2 in range(N) for i1 in range(i2 + 1)]
+    for k, x in enumerate(xs):
+        psi = orc.eval(x)["psi"][0]
+        rw = 1.0 / (abs(psi) + shift)
+        norm += (abs(psi) * rw) ** 2
+        wsg += psi / abs(psi)
+        sd = g["sdots"][k] * rw
+        for t in range(4):
+            vb_ref[t] += np.array([(np.conj(sd[t, a]) * sd[t, b]).real for a, b in iu])
+        sp_ref += rw * rw * (np.conj(psi) * g["chi_loc"][k]).real
+    assert got["norm"] == pytest.approx(norm, rel=1e-14)
+    assert got["wf_sgn"] == pytest.approx(wsg, rel=1e-14)
+    np.testing.assert_allclose(got["chi_sp"], sp_ref, rtol=1e-13, atol=1e-15 * norm)
+    np.testing.assert_allclose(got["chi_vbs"], vb_ref, rtol=1e-13, atol=1e-15 * np.abs(vb_ref).max())
+    diag = [i2 * (i2 + 1) // 2 + i2 for i2 in range(N)]
+    np.testing.assert_allclose(got["chi_sp"][diag], norm, rtol=1e-13)
+    off = np.ones(nchi, bool); off[diag] = False
+    assert got["chi_sp"][off, 0].sum() == pytest.approx(-N * norm / 2, rel=1e-12)
+    e = orc.measure_energy(xs, shift=shift)
+    assert (got["Etot"], got["Etot2"], got["wf_av"]) == (e["Etot"], e["Etot2"], e["wf_av"])
+
+
 def test_oloc_central_finite_differences(oracle_lib):
     """O_loc(k) = d psi / d gamma_k against central differences, h=1e-6."""
     spec, p = _model(L=4, alphar=1, alphac=1, seed=4)
