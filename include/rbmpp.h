@@ -160,6 +160,19 @@ int rbmpp_set_s_precision(rbmpp_handle* h, int32_t s_precision);
 int rbmpp_measure_energy(rbmpp_handle* h, double shift, double* etot, double* etot2,
                          double* norm, double* wf_av);
 
+/* The measurement of the correlation run on the stored samples
+ * (Calc_Correlation/calc_phys_quantity.f90:264-550, its
+ * calc_phys_quantity_after_opt, with the reductions and normalisation of
+ * Calc_Correlation/main.f90:543-581): energy as rbmpp_measure_energy, and
+ *   wf_sgn[2]           = sum psi/|psi| / Nsample (re, im; may be NULL)
+ *   chi_sp[2*(i0-1)+k-1] = Chi_sp(k, i0) / norm   (k = 1: SzSz, 2: SpSm)
+ *   chi_vbs[(t-1)*Nchi+i0-1] = Chi_vbs(i0, t) / norm  (t = 1..4 bond types)
+ * for the Nchi = N(N+1)/2 pairs i1 <= i2 numbered i0 = i2(i2-1)/2 + i1
+ * (1-based, i2 outer), i.e. the Fortran arrays Chi_sp(2,Nchi) and
+ * Chi_vbs(Nchi,4).  Needs L >= 4 and N <= 104. */
+int rbmpp_measure_correlation(rbmpp_handle* h, double shift, double* etot, double* etot2, double* norm,
+                              double* wf_av, double* wf_sgn, double* chi_sp, double* chi_vbs);
+
 /* ---- parity / test entry points on caller-given configurations ---- */
 /* Sample store: x[(s)*N + i] = +-1, s < n (walker-major: s = w*n_samples + k). */
 int rbmpp_set_samples(rbmpp_handle* h, int32_t n, const int32_t* x);

@@ -74,6 +74,7 @@ def lib():
         L.rbmpp_measure.argtypes = [vp, C.c_double] + [dp] * 6
         L.rbmpp_sr_step.argtypes = [vp, C.POINTER(SrConfig), dp, C.POINTER(SrReport)]
         L.rbmpp_measure_energy.argtypes = [vp, C.c_double] + [dp] * 4
+        L.rbmpp_measure_correlation.argtypes = [vp, C.c_double] + [dp] * 7
         L.rbmpp_set_samples.argtypes = [vp, C.c_int32, ip]
         L.rbmpp_get_samples.argtypes = [vp, ip, dp]
         L.rbmpp_get_measured.argtypes = [vp, dp, dp, dp]
@@ -182,6 +183,21 @@ class Engine:
         self._check(lib().rbmpp_measure_energy(self.h, shift, C.byref(e), C.byref(e2), C.byref(nr),
                                                C.byref(wf)))
         return dict(Etot=e.value, Etot2=e2.value, norm=nr.value, wf_av=wf.value)
+
+    def measure_correlation(self, shift: float = 0.0) -> dict:
+        """The correlation run's measurement on the stored samples
+        (Calc_Correlation/calc_phys_quantity.f90:264-550 with the
+        normalisation of Calc_Correlation/main.f90:543-581): chi_sp[i0, k] =
+        Chi_sp(k+1, i0+1), chi_vbs[t, i0] = Chi_vbs(i0+1, t+1), pairs
+        i1 <= i2 numbered i0 = i2 (i2 + 1) / 2 + i1 (0-based)."""
+        N = self.spec.N
+        nchi = N * (N + 1) // 2
+        e, e2, nr, wf = (C.c_double() for _ in range(4))
+        ws = np.zeros(2); sp = np.zeros(2 * nchi); vb = np.zeros(4 * nchi)
+        self._check(lib().rbmpp_measure_correlation(self.h, shift, C.byref(e), C.byref(e2), C.byref(nr),
+                                                    C.byref(wf), _d(ws), _d(sp), _d(vb)))
+        return dict(Etot=e.value, Etot2=e2.value, norm=nr.value, wf_av=wf.value,
+                    wf_sgn=complex(ws[0], ws[1]), chi_sp=sp.reshape(nchi, 2), chi_vbs=vb.reshape(4, nchi))
 
     def sr_step(self, gamma=None, s_factor0: float = 1e-3, delta_tau: float = 0.05,
                 max_change: float = 0.04, slater_abs_av: float = 0.25, refine_iters: int = 1):

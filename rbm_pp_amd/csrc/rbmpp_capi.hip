@@ -55,6 +55,9 @@ struct rbmpp_handle {
   double2 *d_mpsi = nullptr, *d_eloc = nullptr;
   double *d_rw = nullptr, *d_orows = nullptr, *d_kappa = nullptr, *d_part = nullptr;
   double* d_scr = nullptr;  // recompute scratch of the local kernel (N > RBMPP_NLDS)
+  // correlation run: per-workgroup partial sums and their total
+  double *d_cpart = nullptr, *d_csum = nullptr;
+  size_t cpart_cap = 0;
   int* d_status = nullptr;
   int local_grid = 0;
   size_t part_cap = 0;
@@ -401,7 +404,7 @@ static int ensure_part(rbmpp_handle* h, size_t n) {
 
 static int run_local(rbmpp_handle* h, int n, const signed char* d_x, int mode, double shift, const int* ex1,
                      const int* ex2, double* d_invm, double* d_pf, double2* d_psiq, double2* d_expsi,
-                     double* d_expf) {
+                     double* d_expf, double* d_corr = nullptr, int grid_cap = 0) {
   int rc = ensure_measure(h, (size_t)n);
   if (rc) return rc;
   h->fused_valid = 0;  // the buffers are about to be overwritten
@@ -425,7 +428,8 @@ static int run_local(rbmpp_handle* h, int n, const signed char* d_x, int mode, d
   a.ex_psi = d_expsi;
   a.ex_pf = d_expf;
   a.status = h->d_status;
-  const int grid = std::min(n, h->local_grid);
+  a.corr_part = d_corr;
+  const int grid = std::min(n, grid_cap > 0 ? grid_cap : h->local_grid);
   // the kernel reads the descriptor from device memory; real_only may have
   // been switched for this call
   {
@@ -515,6 +519,55 @@ int rbmpp_measure_energy(rbmpp_handle* h, double shift, double* etot, double* et
   rc = reduce_measure(h, n, false);
   if (rc) return rc;
   h->t[2] = toc(h);
+  if (etot) *etot = h->m_etot;
+  if (etot2) *etot2 = h->m_etot2;
+  if (norm) *norm = h->m_norm;
+  if (wf_av) *wf_av = h->m_wfav;
+  return RBMPP_OK;
+}
+
+// workgroups of the correlation run (each keeps its own partial sums)
+static const int CORR_GRID = 1024;
+
+int rbmpp_measure_correlation(rbmpp_handle* h, double shift, double* etot, double* etot2, double* norm,
+                              double* wf_av, double* wf_sgn, double* chi_sp, double* chi_vbs) {
+  if (!h || !chi_sp || !chi_vbs) return RBMPP_E_ARG;
+  if (h->n_stored <= 0) return fail(h, RBMPP_E_STATE, "measure_correlation: no samples");
+  if (h->N > RBMPP_NLDS || h->L < 4)
+    return fail(h, RBMPP_E_ARG, "measure_correlation: needs 4 <= L and N <= 104");
+  const int n = (int)h->n_stored, N = h->N;
+  const int grid = std::min(n, CORR_GRID);
+  const size_t len = (size_t)corr_len(N), nsum = (size_t)corr_sums(N);
+  int rc;
+  if (h->cpart_cap < (size_t)grid * len) {
+    if ((rc = dalloc(h, &h->d_cpart, (size_t)grid * len))) return rc;
+    h->cpart_cap = (size_t)grid * len;
+  }
+  if (!h->d_csum && (rc = dalloc(h, &h->d_csum, nsum))) return rc;
+  tic(h);
+  rc = run_local(h, n, h->d_xs, 1, shift, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                 h->d_cpart, grid);
+  if (rc) return rc;
+  rc = reduce_measure(h, n, false);
+  if (rc) return rc;
+  launch_sum_strided((int)nsum, grid, (int)len, h->d_cpart, h->d_csum, h->st);
+  HIPCHK(h, hipGetLastError());
+  std::vector<double> sums(nsum);
+  HIPCHK(h, hipMemcpyAsync(sums.data(), h->d_csum, sizeof(double) * nsum, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  h->t[2] = toc(h);
+  // normalisation of Calc_Correlation/main.f90:548-581
+  const size_t nchi = (size_t)corr_nchi(N);
+  const double nrm = h->m_norm;
+  for (size_t i0 = 0; i0 < nchi; i0++) {
+    chi_sp[2 * i0] = sums[i0] / nrm;
+    chi_sp[2 * i0 + 1] = sums[nchi + i0] / nrm;
+  }
+  for (size_t k = 0; k < 4 * nchi; k++) chi_vbs[k] = sums[2 * nchi + k] / nrm;
+  if (wf_sgn) {
+    wf_sgn[0] = sums[6 * nchi] / (double)n;
+    wf_sgn[1] = sums[6 * nchi + 1] / (double)n;
+  }
   if (etot) *etot = h->m_etot;
   if (etot2) *etot2 = h->m_etot2;
   if (norm) *norm = h->m_norm;

@@ -78,10 +78,25 @@ struct LocalArgs {
   double2* ex_psi;        // [nconf] amplitude after the custom exchange
   double* ex_pf;          // [nconf][C]
   int* status;            // [nconf]
+  // correlation run (launch_meas_corr): per-workgroup partial sums
+  // [grid][corr_len(N)], laid out by corr_len below
+  double* corr_part;
 };
+
+// Per-workgroup slice of the correlation sums, nchi = N(N+1)/2 pairs i1 <= i2
+// numbered i0 = i2(i2+1)/2 + i1: [0,nchi) Chi_sp(1,i0), [nchi,2nchi)
+// Chi_sp(2,i0), [(2+t)nchi, (3+t)nchi) Chi_vbs(i0,t+1) for t < 4, then
+// wf_sgn (re, im), then 2 (N/2)^2 doubles of scratch for the pair amplitudes.
+__host__ __device__ inline int corr_nchi(int N) { return N * (N + 1) / 2; }
+__host__ __device__ inline int corr_sums(int N) { return 6 * corr_nchi(N) + 2; }
+__host__ __device__ inline int corr_len(int N) { return corr_sums(N) + 2 * (N / 2) * (N / 2); }
 
 void launch_slater(const DevModel& m, const double* f, double* se, int* bad, hipStream_t s);
 int launch_sweep(const DevModel& hm, const SweepArgs& a, hipStream_t s);
 int launch_local(const DevModel& hm, const LocalArgs& a, int grid, hipStream_t s);
 // the N <= RBMPP_NLDS estimator (register Gauss-Jordan, 1024 threads)
 void launch_meas(const LocalArgs& a, int grid, hipStream_t s);
+// the same estimator with the correlation measurement (N <= RBMPP_NLDS)
+void launch_meas_corr(const LocalArgs& a, int grid, hipStream_t s);
+// out[k] = sum_c part[c * stride + k] for k < len, c < nchunk (fixed order)
+void launch_sum_strided(int len, int nchunk, int stride, const double* part, double* out, hipStream_t s);

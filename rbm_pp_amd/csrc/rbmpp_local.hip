@@ -264,7 +264,11 @@ int launch_local(const DevModel& hm, const LocalArgs& a, int grid, hipStream_t s
   const int N = hm.N;
   if (hm.C > RBMPP_CMAX || hm.mr > RBMPP_MRMAX || hm.mc > RBMPP_MCMAX) return -1;
   if (hm.nsl > 4 * N || (N & 1)) return -1;
-  if (N <= RBMPP_NLDS) launch_meas(a, grid, st);
+  if (a.corr_part) {
+    // the correlation run: pair pass of the register estimator only
+    if (N > RBMPP_NLDS || hm.L < 4 || (N & 3)) return -1;
+    launch_meas_corr(a, grid, st);
+  } else if (N <= RBMPP_NLDS) launch_meas(a, grid, st);
   else if (N <= 256) {
     if (!a.scratch) return -1;
     launch_local_t<256, false>(a, grid, st);

@@ -18,6 +18,17 @@
 //  * X^-1 goes to LDS once; each wave then takes two antiparallel bonds per
 //    pass, forms P = X^-1 a for both with one read of X^-1, and evaluates the
 //    exchange ratio of pfupdate_two.f90:27-88 with wave reductions.
+//
+// The correlation run (k_meas<true>, Calc_Correlation/calc_phys_quantity.f90:
+// 264-550) needs the exchange ratio of every antiparallel pair, (N/2)^2 of
+// them.  Instead of P = X^-1 a per pair (N^2 each), the rows G(s, .) of
+// Slater_Elm that an electron moving to site s brings (the row of s for the
+// spin opposite to its occupant, restricted to the occupied columns) are
+// multiplied once per copy: Y = G InvM^T (N^3), and for the pairs
+// Z(u, d) = G(s_u, .) . Y(s_d, .) (N^3 / 4).  Every ratio is then a few
+// entries of Y, Z, InvM and Slater_Elm (the two replaced columns enter as
+// corrections).  The sums of Chi_sp, Chi_vbs and wf_sgn are kept per
+// workgroup.
 #include "rbmpp_kernels.h"
 #include "rbmpp_measure.h"
 #include "rbmpp_skew.h"
@@ -42,6 +53,7 @@ struct MeasSmem {
   double2 thc[RBMPP_MCMAX];
   double pf[RBMPP_CMAX];
   double2 psiq[RBMPP_CMAX], psiwo[RBMPP_CMAX];
+  double2 psi_cur;  // psi(x) of the configuration, for the correlation sums
   double selv[2];
   int sel[2];
   int x[MNMAX], eidx[MNMAX], ecfg[MNMAX], site[MNMAX];
@@ -67,6 +79,72 @@ __device__ __attribute__((noinline)) double2 eloc_of(const DevModel& m, const Me
   return eloc_from_bonds(m, s, p);
 }
 
+// Exchange ratios of the antiparallel pairs of copy q from Y (in s.inv:
+// Y(s, k) at s * ld + k) and InvM(u, np + d) (iud[u * np + d]); lane d takes
+// the down electron np + d, wave w the up electrons u = 4w .. 4w + 3.  The
+// ratio is that of calculate_new_PfM_two_child (pfupdate_two.f90:27-88) for
+// the exchange of the spins at i1 < i2 (electrons a at i1, b at i2), with
+// a = g_rsa + corrections at a, b (rsa = the new row of a), likewise b, and
+// Y = InvM g, Z = g_rsb . Y_rsa.  psi(x') of the copy is added to psip.
+__device__ __attribute__((noinline)) void corr_pairs(const DevModel& m, const MeasSmem& s, const double* S,
+                                                     int q, double pfq, double2* psip, int lane, int wid) {
+  const int N = m.N, np = m.npair, ld = N + 1, two = 2 * N;
+  if (lane >= np || 4 * wid >= np) return;
+  const double* ys = s.inv;
+  const double* iud = (const double*)&s.avec[0][0];
+  const int d = lane, sd = s.eidx[np + d];
+  const double* yd = ys + sd * ld;
+  int uu[4];
+  const double* gu[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    uu[j] = 4 * wid + j;
+    const int su = uu[j] < np ? s.eidx[uu[j]] : 0;
+    gu[j] = S + (size_t)(su + N) * two;  // an up site takes a down electron: row su + N
+  }
+  double z[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int k = 0; k < N; k++) {
+    const double yk = yd[k];
+    const int ck = s.eidx[k] + (k >= np ? N : 0);
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      if (uu[j] < np) z[j] = fma(gu[j][ck], yk, z[j]);
+  }
+  const int g = q >> 1;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int u = uu[j];
+    if (u >= np) break;  // wave-uniform
+    const int su = s.eidx[u];
+    const bool upfirst = su < sd;
+    const int i1 = upfirst ? su : sd, i2 = upfirst ? sd : su;
+    const int ea = upfirst ? u : np + d, eb = upfirst ? np + d : u;
+    const int rsa = i2 + (s.x[i2] == 1 ? N : 0), rsb = i1 + (s.x[i1] == 1 ? N : 0);
+    const int ca = i1 + (ea >= np ? N : 0), cb = i2 + (eb >= np ? N : 0);
+    const double D = upfirst ? z[j] : -z[j];  // Z is skew in its two sites
+    const double iab = upfirst ? iud[u * np + d] : -iud[u * np + d];
+    const double iba = -iab;
+    const double* Sa = S + (size_t)rsa * two;
+    const double* Sb = S + (size_t)rsb * two;
+    const double srr = Sa[rsb];
+    const double daa = -Sa[ca], dab = srr - Sa[cb];
+    const double dbb = -Sb[cb], dba = -srr - Sb[ca];
+    const double y2a = ys[i2 * ld + ea], y2b = ys[i2 * ld + eb];
+    const double y1a = ys[i1 * ld + ea], y1b = ys[i1 * ld + eb];
+    const double pa = -y2a - iab * dab, pb = -y2b - iba * daa;
+    const double qa = -y1a - iab * dbb, qb = -y1b - iba * dba;
+    const double gp = -D + daa * y1a + dab * y1b;
+    const double bma = gp + dba * pa + dbb * pb;
+    const double rat = iba * (-srr) + iba * bma + pa * qb - pb * qa;
+    int sfac = -1;  // make_sample.f90:195-197
+    if (m.subB[i1]) sfac = -sfac;
+    if (m.subB[i2]) sfac = -sfac;
+    const double2 c = exchanged_amp(m, q, s, sfac * s.sgn[g], i1, i2);
+    psip[u * np + d] = cadd(psip[u * np + d], cscale(c, rat * pfq));
+  }
+}
+
+template <bool CORR>
 __global__ __launch_bounds__(MNT) void k_meas(LocalArgs a) {
   __shared__ MeasSmem s;
   const DevModel& m = *a.m;
@@ -80,10 +158,22 @@ __global__ __launch_bounds__(MNT) void k_meas(LocalArgs a) {
   for (int k = tid; k < 2 * 2 * MPAD; k += nth) (&s.rowbuf[0][0][0])[k] = 0.0;
   const bool h0 = lane < N;               // lane owns index lane
   const bool h1 = lane + RBMPP_WAVE < N;  // lane also owns index lane + 64
+  // correlation sums of this workgroup, and the pair amplitudes
+  double* cpart = nullptr;
+  double2* psip = nullptr;
+  int nchi = 0;
+  if constexpr (CORR) {
+    nchi = corr_nchi(N);
+    cpart = a.corr_part + (size_t)blockIdx.x * corr_len(N);
+    psip = (double2*)(cpart + corr_sums(N));
+    for (int k = tid; k < corr_sums(N); k += nth) cpart[k] = 0.0;
+  }
 
   for (int cfg = blockIdx.x; cfg < a.nconf; cfg += gridDim.x) {
     __syncthreads();
     for (int i = tid; i < N; i += nth) s.x[i] = a.xs[(size_t)cfg * N + i];
+    if constexpr (CORR)
+      for (int e = tid; e < np * np; e += nth) psip[e] = make_double2(0.0, 0.0);
     if (tid == 0) s.status = WS_OK;
     for (int k = tid; k < 4 * MNMAX; k += nth) {
       s.opp[0][k] = 0.0;
@@ -300,6 +390,48 @@ __global__ __launch_bounds__(MNT) void k_meas(LocalArgs a) {
         s.psib[b] = cadd(s.psib[b], cscale(c, pfn));
         if (custom && a.ex_pf) a.ex_pf[(size_t)cfg * C + q] = pfn;
       }
+      if constexpr (CORR) {
+        __syncthreads();  // the bond pass is done with avec and inv
+        double* iud = (double*)&s.avec[0][0];  // InvM(u, np + d), up x down block
+        for (int e = tid; e < np * np; e += nth) {
+          const int u = e / np;
+          iud[e] = s.inv[u * ld + np + (e - u * np)];
+        }
+        // Y(s, k) = sum_i InvM(i, k) Slater[col_i][R(s)] = sum_i InvM(k, i) G(s, i)
+        // for sites s = lane, lane + 64 and electrons k = wid + 16 kk; R(s)
+        // is the row of site s for the spin opposite to its occupant and
+        // col_i the column of electron i (G(s, i) = Slater[R(s)][col_i])
+        constexpr int KK = (MNMAX + 15) / 16;
+        double y0[KK], y1[KK];
+#pragma unroll
+        for (int kk = 0; kk < KK; kk++) { y0[kk] = 0.0; y1[kk] = 0.0; }
+        const int R0 = h0 ? lane + (s.x[lane] == 1 ? N : 0) : 0;
+        const int R1 = h1 ? lane + RBMPP_WAVE + (s.x[lane + RBMPP_WAVE] == 1 ? N : 0) : 0;
+        for (int i = 0; i < N; i++) {
+          const double* srow = S + (size_t)(s.eidx[i] + (i >= np ? N : 0)) * 2 * N;
+          const double g0 = h0 ? srow[R0] : 0.0;
+          const double g1 = h1 ? srow[R1] : 0.0;
+          const double* irow = s.inv + i * ld;
+#pragma unroll
+          for (int kk = 0; kk < KK; kk++) {
+            const int k = wid + 16 * kk;
+            const double mv = k < N ? irow[k] : 0.0;
+            y0[kk] = fma(mv, g0, y0[kk]);
+            y1[kk] = fma(mv, g1, y1[kk]);
+          }
+        }
+        __syncthreads();  // every read of InvM is done
+#pragma unroll
+        for (int kk = 0; kk < KK; kk++) {
+          const int k = wid + 16 * kk;
+          if (k < N) {
+            if (h0) s.inv[lane * ld + k] = y0[kk];
+            if (h1) s.inv[(lane + RBMPP_WAVE) * ld + k] = y1[kk];
+          }
+        }
+        __syncthreads();  // Y complete
+        corr_pairs(m, s, S, q, pfq, psip, lane, wid);
+      }
     }
     __syncthreads();
     if (s.status != WS_OK) {
@@ -312,6 +444,7 @@ __global__ __launch_bounds__(MNT) void k_meas(LocalArgs a) {
       for (int q = 0; q < C; q++) p = cadd(p, s.psiq[q]);
       const double rw = a.shift >= 0.0 ? 1.0 / (cmod(p) + a.shift) : 1.0;
       a.psi[cfg] = p;
+      s.psi_cur = p;
       if (a.rw) a.rw[cfg] = rw;
       if (want_eloc && !custom) a.eloc[cfg] = cscale(eloc_of(m, s, p), rw);
       if (custom && a.ex_psi) a.ex_psi[cfg] = s.psib[0];
@@ -327,12 +460,86 @@ __global__ __launch_bounds__(MNT) void k_meas(LocalArgs a) {
       double* ore = a.orows + (size_t)(cplx_rows ? 2 * cfg : cfg) * m.nv;
       o_row(m, s, s.selv[0], ore, ore + m.nv, cplx_rows);
     }
+    if constexpr (CORR) {
+      // Calc_Correlation/calc_phys_quantity.f90:318-338: SdotS per bond
+      // (calc_local_energy_correlation, :348-413) reweighted, then
+      // Chi_vbs += Re(conj SdotS(i1,t) SdotS(i2,t)) and
+      // Chi_sp += rw^2 Re(conj(psi) Chi_loc) (calc_local_correlation, :469-550)
+      const double2 p = s.psi_cur;
+      const double rw = s.selv[0], w2 = rw * rw, ap2 = p.x * p.x + p.y * p.y;
+      double2* sdb = (double2*)&s.opp[0][0];  // SdotS(i1, t) at sdb[t * N + i1]
+      for (int e = tid; e < 4 * N; e += nth) {
+        const int t = e / N, i1 = e - t * N;
+        sdb[e] = cscale(p, (double)(s.x[i1] * s.x[bond_partner(m.L, t, i1)]));
+      }
+      __syncthreads();
+      for (int b = tid; b < nb; b += nth) {
+        const int i1 = s.bi1[b], i2 = s.bi2[b];
+        int t = 0;
+        while (t < 3 && bond_partner(m.L, t, i1) != i2) t++;  // partners are distinct for L >= 3
+        sdb[t * N + i1] = cadd(sdb[t * N + i1], cscale(s.psib[b], t <= 1 ? -2.0 : 2.0));
+      }
+      __syncthreads();
+      for (int e = tid; e < 4 * N; e += nth) sdb[e] = cscale(sdb[e], rw);
+      __syncthreads();
+      double* sp1 = cpart;
+      double* sp2 = cpart + nchi;
+      double* vbs = cpart + 2 * nchi;
+      for (int e = tid; e < N * N; e += nth) {
+        const int i2 = e / N, i1 = e - i2 * N;
+        if (i1 > i2) continue;
+        const int i0 = i2 * (i2 + 1) / 2 + i1;
+        sp1[i0] += w2 * ((double)(s.x[i1] * s.x[i2]) * ap2);
+        if (i1 == i2) sp2[i0] += w2 * ap2;
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+          const double2 u = sdb[t * N + i1], v = sdb[t * N + i2];
+          vbs[t * nchi + i0] += u.x * v.x + u.y * v.y;
+        }
+      }
+      // exchange part of Chi_loc(2, .) for the antiparallel pairs, by the
+      // thread that accumulated the pair: s_i1 s_i2 psi(x')
+      if (lane < np) {
+        const int sd = s.eidx[np + lane];
+        for (int j = 0; j < 4; j++) {
+          const int u = 4 * wid + j;
+          if (u >= np) break;
+          const int su = s.eidx[u];
+          const int i1 = min(su, sd), i2 = max(su, sd);
+          const double sg = (m.subB[i1] != m.subB[i2]) ? -1.0 : 1.0;
+          const double2 c = psip[u * np + lane];
+          sp2[i2 * (i2 + 1) / 2 + i1] += w2 * (sg * (p.x * c.x + p.y * c.y));
+        }
+      }
+      if (tid == 0) {
+        const double ap = cmod(p);
+        cpart[6 * nchi] += p.x / ap;
+        cpart[6 * nchi + 1] += p.y / ap;
+      }
+    }
     if (tid == 0 && a.status) a.status[cfg] = WS_OK;
   }
+}
+
+// out[k] = sum over chunks c (in order) of part[c * stride + k], k < len
+__global__ void k_sum_strided(int len, int nchunk, int stride, const double* __restrict__ part, double* out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= len) return;
+  double t = 0.0;
+  for (int c = 0; c < nchunk; c++) t += part[(size_t)c * stride + k];
+  out[k] = t;
 }
 
 }  // namespace
 
 void launch_meas(const LocalArgs& a, int grid, hipStream_t st) {
-  hipLaunchKernelGGL(k_meas, dim3(grid), dim3(MNT), 0, st, a);
+  hipLaunchKernelGGL(k_meas<false>, dim3(grid), dim3(MNT), 0, st, a);
+}
+
+void launch_meas_corr(const LocalArgs& a, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_meas<true>, dim3(grid), dim3(MNT), 0, st, a);
+}
+
+void launch_sum_strided(int len, int nchunk, int stride, const double* part, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_sum_strided, dim3((len + 255) / 256), dim3(256), 0, st, len, nchunk, stride, part, out);
 }

@@ -10,6 +10,7 @@
 !   reductions of main.f90:347-367      (main.f90:333)     -> rbmpp_measure
 !   include "SR_pdposv.f90"             (main.f90:375)     -> rbmpp_sr_step
 !   calc_phys_quantity_after_opt(...)   (main.f90:518)     -> rbmpp_measure_energy
+!   Calc_Correlation calc_phys_quantity_after_opt(...)     -> rbmpp_measure_correlation
 !
 ! Arrays are passed in the reference's own layouts (column-major, 1-based
 ! index values in the tables), so the reference's arrays go through as they
@@ -48,7 +49,8 @@ module rbmpp_mod
   public :: rbmpp_create, rbmpp_destroy, rbmpp_last_error, rbmpp_nv
   public :: rbmpp_set_params, rbmpp_set_gamma, rbmpp_get_gamma, rbmpp_set_projection
   public :: rbmpp_make_sample, rbmpp_measure, rbmpp_sr_step, rbmpp_set_s_precision
-  public :: rbmpp_measure_energy, rbmpp_set_samples, rbmpp_get_samples, rbmpp_n_samples
+  public :: rbmpp_measure_energy, rbmpp_measure_correlation
+  public :: rbmpp_set_samples, rbmpp_get_samples, rbmpp_n_samples
   public :: rbmpp_get_walkers, rbmpp_eval, rbmpp_ratio_exchange, rbmpp_local
   public :: rbmpp_get_smat, rbmpp_timings, rbmpp_get_measured
   public :: rbmpp_error_message
@@ -143,6 +145,15 @@ module rbmpp_mod
       type(c_ptr), value :: h
       real(c_double), value :: shift
       real(c_double), intent(out) :: etot, etot2, norm, wf_av
+    end function
+
+    ! Calc_Correlation: chi_sp(2,Nchi), chi_vbs(Nchi,4), wf_sgn(2) = (re, im)
+    integer(c_int) function rbmpp_measure_correlation(h, shift, etot, etot2, norm, wf_av, wf_sgn, &
+        chi_sp, chi_vbs) bind(C, name="rbmpp_measure_correlation")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: h
+      real(c_double), value :: shift
+      real(c_double), intent(out) :: etot, etot2, norm, wf_av, wf_sgn(2), chi_sp(*), chi_vbs(*)
     end function
 
     integer(c_int) function rbmpp_set_samples(h, n, x) bind(C, name="rbmpp_set_samples")
