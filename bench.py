@@ -84,6 +84,8 @@ def main():
                          "recomputing Pf and X^-1 per sample as the reference does")
     ap.add_argument("--cond-guard", type=float, default=1e4,
                     help="recompute after an accept whose Pf ratio of a copy leaves (1/g, g); 0: off")
+    ap.add_argument("--no-correlation", action="store_true",
+                    help="skip the correlation-run measurement on the last step's samples")
     ap.add_argument("--sampler-only", action="store_true",
                     help="profiling aid: thermalise, run one make_sample launch and exit")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -222,6 +224,19 @@ def main():
                 f"step err {fp64[name]['step_rel_err']:.2e}")
         eng.set_s_precision(S_MODES[args.s_precision])
 
+    # The measurement of the correlation run (Calc_Correlation: Chi_sp over
+    # all N (N + 1) / 2 pairs, Chi_vbs, wf_sgn) on the last step's samples.
+    corr = None
+    if not args.no_correlation:
+        t0 = time.perf_counter()
+        c = eng.measure_correlation(shift=shift)
+        wall = time.perf_counter() - t0
+        t_dev = eng.timings()["measure"]
+        corr = {"samples": ns_step, "measure_s": t_dev, "wall_s": wall, "samples_per_s": ns_step / t_dev,
+                "exchange_pairs_per_sample": (N // 2) ** 2, "wf_sgn_abs": abs(c["wf_sgn"]),
+                "energy_per_site": c["Etot"] / 4 / N}
+        log(f"correlation measurement: {t_dev:.2f} s for {ns_step} samples ({ns_step / t_dev:.0f} samples/s)")
+
     samples = ns_step * args.steps
     acc = sum(r["nacc"] for r in recs) / samples / N
     sweeps_per_launch = args.walkers * sweeps_per_launch_walker
@@ -281,6 +296,8 @@ def main():
     }
     if fp64:
         out["s_build"] = fp64
+    if corr:
+        out["correlation"] = corr
     print(json.dumps(out))
 
 

@@ -5,11 +5,18 @@
 ! W.input (the reference draws W from a clock-seeded stream when W.input is
 ! absent, parameter.f90:21-41; this driver requires the file).
 !
-!   rbmpp_vmc [walkers [s_precision [recompute_k [projection]]]]
+!   rbmpp_vmc [walkers [s_precision [recompute_k [projection [mode]]]]]
 !
 ! projection 1 runs the transverse_code variant: its spin-projection weights
 ! (transverse_code/spin_projection.f90:30-38) and its reweighting-offset
 ! factor 0.003 (transverse_code/main.f90:451) in place of 0.001.
+!
+! mode 1 is the measurement run of Calc_Correlation (Calc_Correlation/
+! main.f90:311 skips the optimisation, :499-640 measure): Nav iterations of
+! energy, spin correlation Chi_sp and dimer correlation Chi_vbs with the
+! parameters of the input files, written to fort.<60000+it> and
+! fort.<70000+it> in the reference's formats, plus energy_after_opt.txt and
+! Average_Energy.txt.
 !
 ! Writes Energy_vs_Iteration.txt (main.f90:410), optimized_W.txt and
 ! zqp_orbital_opt.dat (parameter.f90:238-276), energy_after_opt.txt and
@@ -32,12 +39,12 @@ program rbmpp_vmc
   type(c_ptr) :: h
   integer(c_int) :: rc
   integer(c_int64_t) :: naccept
-  integer :: walkers, nspw, prec, kre, proj, it, i, nv, nav_chk
+  integer :: walkers, nspw, prec, kre, proj, mode, it, i, nv, nav_chk
   real(8) :: shift_factor
   real(8) :: etot, etot2, norm, wf_av, shift, shift_av, wf_av_list(10), acc, e_av, e_er, v_av, v_er
   character(len=64) :: arg
 
-  walkers = 4096; prec = RBMPP_S_FP64; kre = 32; proj = RBMPP_PROJ_NORMAL
+  walkers = 4096; prec = RBMPP_S_FP64; kre = 32; proj = RBMPP_PROJ_NORMAL; mode = 0
   if (command_argument_count() >= 1) then
     call get_command_argument(1, arg); read(arg, *) walkers
   end if
@@ -49,6 +56,9 @@ program rbmpp_vmc
   end if
   if (command_argument_count() >= 4) then
     call get_command_argument(4, arg); read(arg, *) proj
+  end if
+  if (command_argument_count() >= 5) then
+    call get_command_argument(5, arg); read(arg, *) mode
   end if
   shift_factor = merge(0.003d0, 0.001d0, proj == RBMPP_PROJ_TRANSVERSE)
 
@@ -69,6 +79,11 @@ program rbmpp_vmc
   if (proj /= RBMPP_PROJ_NORMAL) then
     rc = rbmpp_set_projection(h, proj)
     call check(rc, "rbmpp_set_projection")
+  end if
+  if (mode == 1) then
+    call correlation_run()
+    call rbmpp_destroy(h)
+    stop
   end if
 
   ! optimisation (main.f90:300-455)
@@ -158,6 +173,71 @@ contains
     if (code /= RBMPP_OK) then
       write(0, '(a,a,I6,a,a)') what, ' failed: ', code, ' ', rbmpp_error_message(h)
       stop 1
+    end if
+  end subroutine
+
+  ! Calc_Correlation/main.f90:499-624
+  subroutine correlation_run()
+    integer :: nchi, j, u
+    real(c_double) :: wf_sgn(2)
+    real(c_double), allocatable :: chi_sp(:,:), chi_vbs(:,:)
+    character(len=32) :: fname
+    nchi = N * (N + 1) / 2
+    allocate(chi_sp(2, nchi), chi_vbs(nchi, 4))
+    open(unit=10, file='energy_after_opt.txt', status='unknown')
+    ! warm-up sampling; the offset is 1e-5 of the mean |psi| (:515-523)
+    rc = rbmpp_make_sample(h, 1, 0d0, naccept)
+    call check(rc, "rbmpp_make_sample")
+    rc = rbmpp_measure_energy(h, 0d0, etot, etot2, norm, wf_av)
+    call check(rc, "rbmpp_measure_energy")
+    shift = wf_av * 1d-5
+    write(6, '(a,2E15.5)') 'wf_av, shift =', wf_av, shift
+    do it = 1, 7
+      rc = rbmpp_make_sample(h, 0, shift, naccept)
+      call check(rc, "rbmpp_make_sample")
+    end do
+    do it = 1, Nav
+      rc = rbmpp_make_sample(h, 0, shift, naccept)
+      call check(rc, "rbmpp_make_sample")
+      acc = dble(naccept) / dble(walkers * nspw) / dble(N)
+      rc = rbmpp_measure_correlation(h, shift, etot, etot2, norm, wf_av, wf_sgn, chi_sp, chi_vbs)
+      call check(rc, "rbmpp_measure_correlation")
+      write(6, '(a,I6)')      'Iteration (Energy Measuement) :', it
+      write(6, '(a,F20.10)')  '   total energy         :', etot
+      write(6, '(a,F20.10)')  '   energy variance      :', (etot2 - etot*etot) / (etot*etot)
+      write(6, '(a,F20.10)')  '   acceptance ratio     :', acc
+      write(6, '(a,2E20.5)')  '   wf_av, shift         :', wf_av, shift
+      write(6, '(a,F20.10)')  '   norm                 :', norm / dble(walkers * nspw)
+      write(6, '(a,3F20.10)') '   wf_sgn               :', sqrt(wf_sgn(1)**2 + wf_sgn(2)**2), wf_sgn
+      e_data(it) = etot
+      e2_data(it) = etot2 - etot*etot
+      write(fname, '(a,I0)') 'fort.', 60000 + it
+      open(newunit=u, file=trim(fname), status='replace')
+      write(u, '(a)') '# chi index,  SzSz,  SpSm'
+      do i = 1, nchi
+        write(u, '(I6,2F25.15)') i, chi_sp(1, i), chi_sp(2, i)
+      end do
+      close(u)
+      write(fname, '(a,I0)') 'fort.', 70000 + it
+      open(newunit=u, file=trim(fname), status='replace')
+      write(u, '(a)') '# chi index, <SSSS> for (1,0), (0,1), (1,1), (1,-1)'
+      do i = 1, nchi
+        write(u, '(I6,4F25.15)') i, (chi_vbs(i, j), j = 1, 4)
+      end do
+      close(u)
+      write(10, *) it, etot, (etot2 - etot*etot) / (etot*etot)
+    end do
+    close(10)
+    if (Nav > 1) then
+      call calc_av_er(Nav, e_data, e_av, e_er)
+      call calc_av_er(Nav, e2_data, v_av, v_er)
+      v_av = v_av / (e_av*e_av)
+      v_er = v_er / (e_av*e_av)
+      open(unit=10, file='Average_Energy.txt', status='unknown')
+      write(10, '(3F25.15)') e_av, e_er / dsqrt(dble(Nav-1)), e_er
+      write(10, '(3E25.15)') v_av, v_er / dsqrt(dble(Nav-1)), v_er
+      write(10, '(4F20.12)') e_av, e_er / dsqrt(dble(Nav-1)), v_av, v_er / dsqrt(dble(Nav-1))
+      close(10)
     end if
   end subroutine
 

@@ -60,3 +60,41 @@ def test_fortran_driver_matches_python_mirror(tmp_path):
     np.testing.assert_allclose(f, po.f, rtol=0, atol=1e-13)  # e25.15 in the file
     avg = open(tmp_path / "Average_Energy.txt").read().split()
     assert len(avg) == 10 and np.isfinite([float(t) for t in avg]).all()
+
+
+def test_fortran_correlation_run_matches_python_mirror(tmp_path):
+    """Mode 1 of the driver (the Calc_Correlation measurement run) against the
+    same call sequence through the Python mirror: fort.<60000+it> (Chi_sp)
+    and fort.<70000+it> (Chi_vbs).  The two read the parameter files with
+    different parsers and sums (the mean |f| normalisation), so parameters
+    and results agree to rounding (1e-13), not bit for bit."""
+    if not os.path.exists(DRIVER):
+        pytest.skip("Fortran driver not built (no amdflang at build time)")
+    from rbm_pp_amd import api
+    W, nspw, K = 64, 10, 16
+    spec = lt.j1j2_model(4, alphar=2, alphac=1, J2=0.5)
+    spec.nsample, spec.nstep, spec.nav = W * nspw, 1, 2
+    lt.write_run_dir(str(tmp_path), spec, lt.random_params(spec, seed_w=3, seed_f=4, sigma_w=0.05))
+    print(f"fortran driver, correlation run: 4x4 walkers={W} samples/walker={nspw}", flush=True)
+    out = subprocess.run([DRIVER, str(W), "0", str(K), "0", "1"], cwd=tmp_path, capture_output=True,
+                         text=True, timeout=120)
+    assert out.returncode == 0, out.stderr + out.stdout[-2000:]
+    spec2, p = lt.read_run_dir(str(tmp_path))
+    eng = api.Engine(spec2, n_walkers=W, n_samples=nspw, seed=1, recompute_k=K, s_precision=api.S_FP64,
+                     params=p)
+    eng.make_sample(True, shift=0.0)
+    shift = eng.calc_phys_quantity_after_opt(shift=0.0)["wf_av"] * 1e-5
+    for _ in range(7):
+        eng.make_sample(False, shift=shift)
+    nchi = spec.N * (spec.N + 1) // 2
+    for it in (1, 2):
+        eng.make_sample(False, shift=shift)
+        r = eng.measure_correlation(shift=shift)
+        sp = np.loadtxt(tmp_path / f"fort.{60000 + it}", comments="#")
+        vb = np.loadtxt(tmp_path / f"fort.{70000 + it}", comments="#")
+        assert sp.shape == (nchi, 3) and vb.shape == (nchi, 5)
+        np.testing.assert_array_equal(sp[:, 0], np.arange(1, nchi + 1))
+        np.testing.assert_allclose(sp[:, 1:], r["chi_sp"], rtol=0, atol=1e-13)
+        np.testing.assert_allclose(vb[:, 1:], r["chi_vbs"].T, rtol=0, atol=1e-13 * max(1.0, np.abs(vb).max()))
+        e = np.loadtxt(tmp_path / "energy_after_opt.txt")
+        assert e[it - 1, 1] == pytest.approx(r["Etot"], rel=1e-12)
