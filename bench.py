@@ -43,7 +43,8 @@ def _cpu_chain(args):
     orc = oracle.Oracle(spec, p)
     rng = oracle.Rng("mt", seed)
     t0 = time.perf_counter()
-    orc.make_sample(rng, True, nsample=nsweeps, nwarmup=0, fixed_k=32)
+    # the reference's adaptive recompute period (fixed_k = 0), as the GPU run
+    orc.make_sample(rng, True, nsample=nsweeps, nwarmup=0, fixed_k=0)
     return nsweeps, time.perf_counter() - t0
 
 
@@ -72,7 +73,9 @@ def main():
     ap.add_argument("--samples", type=int, default=200000, help="stored samples per SR step")
     ap.add_argument("--therm", type=int, default=200,
                     help="thermalisation sweeps, run as make_sample launches of the step's shape")
-    ap.add_argument("--recompute-k", type=int, default=32)
+    ap.add_argument("--recompute-k", type=int, default=0,
+                    help="recompute Pf and X^-1 every K accepts; 0: the reference's adaptive period "
+                         "(make_sample.f90:74-128, at most 100)")
     ap.add_argument("--cpu-sweeps", type=int, default=600)
     ap.add_argument("--cpu-procs", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -116,7 +119,8 @@ def main():
         rate, wall = cpu_baseline(args.L, args.alphar, args.J2, args.cpu_sweeps, procs)
         cpu = {"value": rate, "unit": "samples/s", "cores": procs, "kind": "port",
                "sample": f"{procs} independent chains x {args.cpu_sweeps} stored sweeps each "
-                         f"(N={args.L * args.L} proposals/sweep, recompute every 32 accepts), "
+                         f"(N={args.L * args.L} proposals/sweep, adaptive recompute period as "
+                         f"make_sample.f90:74-128), "
                          f"{wall:.1f} s wall"}
         log(f"cpu baseline: {rate:.1f} samples/s")
 

@@ -44,7 +44,7 @@ program rbmpp_vmc
   real(8) :: etot, etot2, norm, wf_av, shift, shift_av, wf_av_list(10), acc, e_av, e_er, v_av, v_er
   character(len=64) :: arg
 
-  walkers = 4096; prec = RBMPP_S_FP64; kre = 32; proj = RBMPP_PROJ_NORMAL; mode = 0
+  walkers = 4096; prec = RBMPP_S_FP64; kre = 0; proj = RBMPP_PROJ_NORMAL; mode = 0
   if (command_argument_count() >= 1) then
     call get_command_argument(1, arg); read(arg, *) walkers
   end if

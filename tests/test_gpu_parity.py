@@ -92,10 +92,13 @@ def test_local_energy_and_ok(oracle_lib, L, ar, ac):
         assert _relerr(got["oloc"][s], o) < RTOL
 
 
-@pytest.mark.parametrize("L,ar,ac,W,ns,K", [(4, 1, 0, 4, 6, 8), (6, 2, 1, 3, 4, 20), (10, 4, 0, 2, 3, 32)])
+@pytest.mark.parametrize("L,ar,ac,W,ns,K", [(4, 1, 0, 4, 6, 8), (6, 2, 1, 3, 4, 20), (10, 4, 0, 2, 3, 32),
+                                             (6, 2, 0, 3, 6, 0), (10, 4, 0, 2, 4, 0)])
 def test_sampler_trajectory_bit_exact(oracle_lib, L, ar, ac, W, ns, K):
     """GPU walkers and the oracle chain driven by the same Philox streams
-    visit the same configurations; amplitudes agree to FP64 rounding."""
+    visit the same configurations; amplitudes agree to FP64 rounding.  K = 0
+    is the reference's adaptive recompute period (make_sample.f90:74-128),
+    carried from call to call per walker."""
     api = _api()
     spec, p = _model(L, ar, ac, sigma_w=0.1)
     seed = 1234
@@ -112,9 +115,12 @@ def test_sampler_trajectory_bit_exact(oracle_lib, L, ar, ac, W, ns, K):
         st = orc.new_state()
         out = orc.make_sample(rng, True, nsample=ns, nwarmup=nwarm, fixed_k=K, ref_draws=False,
                               state=st, cond_guard=GUARD)
-        states.append((rng, st))
+        states.append((rng, st, out["nfastup"]))
         assert np.array_equal(x_gpu[w * ns:(w + 1) * ns], out["x"]), f"walker {w}"
-        np.testing.assert_allclose(psi_gpu[w * ns:(w + 1) * ns], out["psi"], rtol=1e-9)
+        # the two fast-update chains round differently; up to 32 accepts
+        # between recomputes they agree to 1e-9, with the adaptive period (up
+        # to 100 accepts) to 1e-8
+        np.testing.assert_allclose(psi_gpu[w * ns:(w + 1) * ns], out["psi"], rtol=1e-9 if K else 1e-8)
         # after many fast updates: normwise over the copies of the walker
         assert _relerr(wk["pf"][w], out["pf"][-1]) < 1e-8
         assert wk["naccept"][w] == out["naccept"]
@@ -122,9 +128,9 @@ def test_sampler_trajectory_bit_exact(oracle_lib, L, ar, ac, W, ns, K):
     eng.make_sample(False, shift=1e-3)
     x2, _ = eng.get_samples()
     for w in range(W):
-        rng, st = states[w]
+        rng, st, nf = states[w]
         out = orc.make_sample(rng, False, nsample=ns, nwarmup=nwarm, fixed_k=K, ref_draws=False,
-                              shift=1e-3, state=st, cond_guard=GUARD)
+                              shift=1e-3, state=st, cond_guard=GUARD, nfastup=nf)
         assert np.array_equal(x2[w * ns:(w + 1) * ns], out["x"]), f"walker {w} (second call)"
 
 
