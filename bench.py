@@ -87,6 +87,9 @@ def main():
                          "recomputing Pf and X^-1 per sample as the reference does")
     ap.add_argument("--cond-guard", type=float, default=1e4,
                     help="recompute after an accept whose Pf ratio of a copy leaves (1/g, g); 0: off")
+    ap.add_argument("--projection", choices=["normal", "normal_exact", "transverse"], default="normal",
+                    help="spin-projection weights: normal (cos(pi/2) = 0, block-form sampler), normal_exact "
+                         "(the reference's cos(pi/2) = 6.1e-17, skew-form sampler), transverse")
     ap.add_argument("--no-correlation", action="store_true",
                     help="skip the correlation-run measurement on the last step's samples")
     ap.add_argument("--sampler-only", action="store_true",
@@ -135,6 +138,8 @@ def main():
                      n_warmup_first=-1, recompute_k=args.recompute_k,
                      s_precision=S_MODES[args.s_precision], params=p, fused_measure=args.fused_measure,
                      cond_guard=args.cond_guard)
+    eng.set_projection({"normal": api.PROJ_NORMAL, "normal_exact": api.PROJ_NORMAL_EXACT,
+                        "transverse": api.PROJ_TRANSVERSE}[args.projection])
     N, Cc = spec.N, spec.C
     log(f"model L={args.L} N={N} copies={Cc} nv={spec.nv} walkers={args.walkers} samples/walker={nspw}")
     # every k_sweep launch has the shape of a step's: nwarm + 1 + nspw sweeps
@@ -249,7 +254,11 @@ def main():
     # held as its N^2/2 independent entries (folded skew storage): each of
     # the N proposals reads them once (8 C N^2/2 bytes) and an accepted one
     # (fraction A) reads and writes them once more: 4 C N^3 (1 + 2A).
-    b_unit = 4.0 * Cc * N ** 3 * (1.0 + 2.0 * acc)
+    # In block form (normal projection: every X = [[0, F], [-F^T, 0]]) the
+    # sampler keeps F^-1, (N/2)^2 entries per copy: half of that,
+    # 2 C N^3 (1 + 2A).
+    block = args.projection == "normal"
+    b_unit = (2.0 if block else 4.0) * Cc * N ** 3 * (1.0 + 2.0 * acc)
     achieved = b_unit * sweeps_per_launch / t_sweep / 1e9
     # the same sweep priced with the reference's layout and passes (full
     # N x N X^-1; calculate_new_InvM_two reads it again to form its update
@@ -259,7 +268,7 @@ def main():
     # (rocprofv3 FETCH_SIZE and WRITE_SIZE, each scaled by its reading for a
     # copy of known size), when the committed counter summary covers it
     traffic = None
-    pmc = os.path.join(ROOT, "profiles", f"k_sweep_j1j2_{args.L}x{args.L}_pmc.md")
+    pmc = os.path.join(ROOT, "profiles", f"k_sweep_{'block' if block else 'skew'}_j1j2_{args.L}x{args.L}_pmc.md")
     if os.path.exists(pmc):
         for line in open(pmc):
             if line.startswith("traffic_bytes_per_walker_sweep:"):
@@ -283,6 +292,8 @@ def main():
                    "recompute_k": args.recompute_k,
                    "s_precision": f"{args.s_precision} products on MFMA, fp64 centring/shift/solve",
                    "measurement": "fused into sampler" if args.fused_measure else "recompute per sample",
+                   "projection": args.projection,
+                   "sampler_layout": "F^-1 per copy (block form)" if block else "folded skew X^-1 per copy",
                    "cond_guard": args.cond_guard,
                    "parallelism": "single_gpu"},
         "sampler_samples_per_s": samples / sum(r["sample"] for r in recs),

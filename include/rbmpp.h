@@ -45,11 +45,15 @@ typedef struct rbmpp_handle rbmpp_handle;
 #define RBMPP_S_BF16X3 2
 
 /* Spin-projection weights (InitSPWeight): RBMPP_PROJ_NORMAL is
-   normal_code/spin_projection.f90:5-79 (beta = {0, pi}); RBMPP_PROJ_TRANSVERSE
-   is transverse_code/spin_projection.f90:30-38 (beta = {-pi/2, pi/2},
-   cos and sin of beta/2 scaled by sqrt(2)). */
+   normal_code/spin_projection.f90:5-79 (beta = {0, pi}) with cos(pi/2) taken
+   as exactly 0 instead of its rounding residue 6.1e-17, so that every X is
+   block off-diagonal and the sampler works with the N/2 x N/2 block;
+   RBMPP_PROJ_NORMAL_EXACT keeps the residue as the reference computes it;
+   RBMPP_PROJ_TRANSVERSE is transverse_code/spin_projection.f90:30-38
+   (beta = {-pi/2, pi/2}, cos and sin of beta/2 scaled by sqrt(2)). */
 #define RBMPP_PROJ_NORMAL 0
 #define RBMPP_PROJ_TRANSVERSE 1
+#define RBMPP_PROJ_NORMAL_EXACT 2
 
 typedef struct {
   int32_t L;                 /* linear size; N = L*L sites (main.f90:77,85) */

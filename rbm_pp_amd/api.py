@@ -23,7 +23,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "librbmpp.so")
 
 S_FP64, S_FP32, S_BF16X3 = 0, 1, 2
-PROJ_NORMAL, PROJ_TRANSVERSE = 0, 1
+PROJ_NORMAL, PROJ_TRANSVERSE, PROJ_NORMAL_EXACT = 0, 1, 2
 
 
 class RbmppError(RuntimeError):
@@ -269,9 +269,10 @@ class Engine:
         return out
 
     def set_projection(self, projection: int) -> None:
-        """Spin-projection weights: PROJ_NORMAL (normal_code) or
-        PROJ_TRANSVERSE (transverse_code/spin_projection.f90:30-38);
-        rebuilds Slater_Elm."""
+        """Spin-projection weights: PROJ_NORMAL (normal_code, cos(pi/2) taken as
+        0: block form), PROJ_NORMAL_EXACT (normal_code with the reference's
+        cos(pi/2) = 6.1e-17: general skew form) or PROJ_TRANSVERSE
+        (transverse_code/spin_projection.f90:30-38); rebuilds Slater_Elm."""
         self._check(lib().rbmpp_set_projection(self.h, int(projection)))
 
     def set_s_precision(self, prec: int) -> None:

@@ -48,6 +48,7 @@ struct rbmpp_handle {
   double2* d_psis = nullptr;
   int64_t n_stored = 0;
   int walkers_ready = 0;
+  int minv_block = 0;  // ws.minv holds F^-1 per copy (block form) instead of folded X^-1
   int fused_valid = 0;     // measurement buffers hold the fused results of the stored samples
   double fused_shift = 0;  // reweighting offset they were made with
   // measurement
@@ -102,6 +103,7 @@ static DevModel devmodel(const rbmpp_handle* h) {
   m.L = h->L; m.N = h->N; m.P = h->P; m.C = h->C; m.ac = h->ac; m.ar = h->ar;
   m.npair = h->npair; m.nsl = h->nsl; m.nv = h->nv; m.nv_rbm = h->nv_rbm; m.mc = h->mc; m.mr = h->mr;
   m.real_only = h->real_only;
+  m.block = (h->sp_cs[0] == 0.0 && h->sp_cs[1] == 0.0 && h->N <= RBMPP_NLDS && h->C == RBMPP_CMAX) ? 1 : 0;
   m.J1 = h->cfg.J1; m.J2 = h->cfg.J2;
   for (int s = 0; s < 2; s++) {
     m.sp_cs[s] = h->sp_cs[s]; m.sp_cc[s] = h->sp_cc[s]; m.sp_ss[s] = h->sp_ss[s]; m.sp_w[s] = h->sp_w[s];
@@ -114,6 +116,11 @@ static DevModel devmodel(const rbmpp_handle* h) {
 // InitSPWeight (spin_projection.f90:5-79): the two-point rule at beta = {0, pi};
 // transverse_code/spin_projection.f90:30-38 takes beta = {-pi/2, pi/2} and
 // scales cos and sin of beta/2 by sqrt(2).  para_spin_proj is the same in both.
+// RBMPP_PROJ_NORMAL takes cos(pi/2) as exactly 0: the reference's value is
+// its rounding residue 6.1e-17, which puts entries of relative size 6e-17 into
+// the same-spin blocks of the beta = pi copies; without them every X is
+// exactly block off-diagonal and the sampler runs in determinant form.
+// RBMPP_PROJ_NORMAL_EXACT keeps the residue (and the general skew path).
 static void set_sp_weights(rbmpp_handle* h, int projection) {
   const double pi = std::acos(-1.0);
   const bool tr = projection == RBMPP_PROJ_TRANSVERSE;
@@ -121,6 +128,7 @@ static void set_sp_weights(rbmpp_handle* h, int projection) {
   for (int s = 0; s < 2; s++) {
     double c = std::cos(0.5 * beta[s]), sn = std::sin(0.5 * beta[s]);
     if (tr) { c *= std::sqrt(2.0); sn *= std::sqrt(2.0); }
+    if (projection == RBMPP_PROJ_NORMAL && s == 1) c = 0.0;
     h->sp_cc[s] = c * c; h->sp_cs[s] = c * sn; h->sp_ss[s] = sn * sn;
   }
   h->projection = projection;
@@ -250,7 +258,9 @@ int rbmpp_create(const rbmpp_config* cfg, const int32_t* w_map, const double* pa
 }
 
 int rbmpp_set_projection(rbmpp_handle* h, int32_t projection) {
-  if (!h || (projection != RBMPP_PROJ_NORMAL && projection != RBMPP_PROJ_TRANSVERSE)) return RBMPP_E_ARG;
+  if (!h || (projection != RBMPP_PROJ_NORMAL && projection != RBMPP_PROJ_TRANSVERSE &&
+             projection != RBMPP_PROJ_NORMAL_EXACT))
+    return RBMPP_E_ARG;
   set_sp_weights(h, projection);
   const DevModel hm = devmodel(h);
   HIPCHK(h, hipMemcpyAsync(h->d_model, &hm, sizeof(DevModel), hipMemcpyHostToDevice, h->st));
@@ -346,7 +356,11 @@ int rbmpp_make_sample(rbmpp_handle* h, int32_t first_iteration, double shift, in
     a.m_psi = h->d_mpsi; a.m_eloc = h->d_eloc; a.m_rw = h->d_rw; a.m_orows = h->d_orows;
   }
   tic(h);
-  if (launch_sweep(devmodel(h), a, h->st) != 0) return fail(h, RBMPP_E_ARG, "sweep: unsupported size");
+  {
+    const DevModel hm = devmodel(h);
+    if (launch_sweep(hm, a, h->st) != 0) return fail(h, RBMPP_E_ARG, "sweep: unsupported size");
+    h->minv_block = hm.block;  // layout of ws.minv written by this launch
+  }
   HIPCHK(h, hipGetLastError());
   h->t[1] = toc(h);
   std::vector<int> stat(h->W);
@@ -825,6 +839,18 @@ int rbmpp_get_walkers(const rbmpp_handle* h0, int32_t* x, double* pf, double* ps
     HIPCHK(h, hipMemcpy(m.data(), h->ws.minv, sizeof(double) * m.size(), hipMemcpyDeviceToHost));
     for (size_t b = 0; b < W * C; b++) {
       double* out = invm + b * N * N;
+      if (h->minv_block) {
+        // block form: F^-1(d, u) at d*H + u is InvM(u, H+d); InvM(H+d, u) is its
+        // negative and the same-spin blocks are zero
+        for (size_t k = 0; k < N * N; k++) out[k] = 0.0;
+        for (size_t d = 0; d < H; d++)
+          for (size_t u = 0; u < H; u++) {
+            const double v = m[b * FS + d * H + u];
+            out[(H + d) * N + u] = v;   // InvM(u, H+d)
+            out[u * N + H + d] = -v;    // InvM(H+d, u)
+          }
+        continue;
+      }
       for (size_t r = 0; r < N; r++) out[r * N + r] = 0.0;
       for (size_t d = 1; d <= H; d++)
         for (size_t r = 0; r < (d == H ? H : N); r++) {

@@ -11,6 +11,10 @@
 struct DevModel {
   int L, N, P, C, ac, ar, npair, nsl, nv, nv_rbm, mc, mr;
   int real_only;  // alphac == 0 and all para_mp_trans real: psi and O are real
+  // spin weights with cos*sin = 0 for both copies of a pair: X is exactly
+  // [[0, F], [-F^T, 0]] (F = up x down block) and the sampler keeps
+  // F^-1 (N/2 x N/2) per copy instead of the folded X^-1
+  int block;
   double J1, J2;
   double sp_cs[2], sp_cc[2], sp_ss[2], sp_w[2];
   const int* wmap;            // [P][N]   image of site i under operation g

@@ -138,23 +138,44 @@ __device__ __attribute__((noinline)) void wg_recompute(const DevModel& m, SweepS
         __syncthreads();
         return;
       }
-      // M(r,c) = InvM(r,c) = X^-1(c,r) = -X^-1(r,c) (exactly skew); the
-      // element of each pair's folded slot is written by its holder
+      if (m.block) {
+        // X^-1 = [[0, -F^-T], [F^-1, 0]]: F^-1(d, u) = X^-1(np + d, u) at
+        // Mq[d * np + u]
+        const int np = m.npair;
 #pragma unroll
-      for (int aa = 0; aa < RA; aa++) {
-        const int r = tr + 16 * aa;
+        for (int aa = 0; aa < RA; aa++) {
+          const int r = tr + 16 * aa;
 #pragma unroll
-        for (int b = 0; b < RB; b++) {
-          const int c = tc + 64 * b;
-          if (r < N && c < N) {
-            const double v = -TILE(x, aa, b);
-            lmax = fmax(lmax, fabs(v));
-            int d = c - r;
-            if (d < 0) d += N;
-            if (d > 0 && (d < H || (d == H && r < H))) {
-              const size_t slot = (size_t)(d - 1) * N + r;
+          for (int b = 0; b < RB; b++) {
+            const int c = tc + 64 * b;
+            if (r >= np && r < N && c < np) {
+              const double v = TILE(x, aa, b);
+              const size_t slot = (size_t)(r - np) * np + c;
+              lmax = fmax(lmax, fabs(v));
               if (check) ldiff = fmax(ldiff, fabs(Mq[slot] - v));
               Mq[slot] = v;
+            }
+          }
+        }
+      } else {
+        // M(r,c) = InvM(r,c) = X^-1(c,r) = -X^-1(r,c) (exactly skew); the
+        // element of each pair's folded slot is written by its holder
+#pragma unroll
+        for (int aa = 0; aa < RA; aa++) {
+          const int r = tr + 16 * aa;
+#pragma unroll
+          for (int b = 0; b < RB; b++) {
+            const int c = tc + 64 * b;
+            if (r < N && c < N) {
+              const double v = -TILE(x, aa, b);
+              lmax = fmax(lmax, fabs(v));
+              int d = c - r;
+              if (d < 0) d += N;
+              if (d > 0 && (d < H || (d == H && r < H))) {
+                const size_t slot = (size_t)(d - 1) * N + r;
+                if (check) ldiff = fmax(ldiff, fabs(Mq[slot] - v));
+                Mq[slot] = v;
+              }
             }
           }
         }
@@ -179,8 +200,20 @@ __device__ __attribute__((noinline)) void wg_recompute(const DevModel& m, SweepS
       for (int r = tid; r < N; r += nth) ip[s.perm[r]] = r;
       __syncthreads();
       // M(r,c) = InvM(r,c) = 0.5 (Xinv(c,r) - Xinv(r,c)) (matrix.f90:97-106),
-      // written to the folded slots
-      for (int e = tid; e < N * H; e += nth) {
+      // written to the folded slots (or, in block form, InvM(u, np + d) =
+      // F^-1(d, u) at d * np + u)
+      if (m.block) {
+        const int np = m.npair;
+        for (int e = tid; e < np * np; e += nth) {
+          const int d = e / np, u = e - d * np;
+          const double xdu = A[(size_t)ip[np + d] * ld + ip[u]];
+          const double xud = A[(size_t)ip[u] * ld + ip[np + d]];
+          const double v = 0.5 * (xdu - xud);
+          if (check) ldiff = fmax(ldiff, fabs(Mq[e] - v));
+          lmax = fmax(lmax, fabs(v));
+          Mq[e] = v;
+        }
+      } else for (int e = tid; e < N * H; e += nth) {
         const int d = e / N + 1, r = e - (d - 1) * N;
         if (d == H && r >= H) continue;
         int c = r + d;
@@ -316,16 +349,27 @@ __device__ __attribute__((noinline)) void wg_measure_sample(const DevModel& m, S
     const double pfq = s.pf[q];
     __syncthreads();
     // unfold this copy's X^-1 into a full N x N block
-    for (int e = tid; e < N * (N / 2); e += nth) {
-      const int d = e / N + 1, r = e - (d - 1) * N;
-      if (d == N / 2 && r >= N / 2) continue;
-      int c = r + d;
-      if (c >= N) c -= N;
-      const double v = Mq[e];
-      Ms[(size_t)r * N + c] = v;
-      Ms[(size_t)c * N + r] = -v;
+    if (m.block) {
+      // InvM(u, np + d) = F^-1(d, u), InvM(np + d, u) = -F^-1(d, u), same-spin blocks 0
+      for (int e = tid; e < N * N; e += nth) {
+        const int r = e / N, c = e - r * N;
+        double v = 0.0;
+        if (r < np && c >= np) v = Mq[(size_t)(c - np) * np + r];
+        else if (r >= np && c < np) v = -Mq[(size_t)(r - np) * np + c];
+        Ms[e] = v;
+      }
+    } else {
+      for (int e = tid; e < N * (N / 2); e += nth) {
+        const int d = e / N + 1, r = e - (d - 1) * N;
+        if (d == N / 2 && r >= N / 2) continue;
+        int c = r + d;
+        if (c >= N) c -= N;
+        const double v = Mq[e];
+        Ms[(size_t)r * N + c] = v;
+        Ms[(size_t)c * N + r] = -v;
+      }
+      for (int r = tid; r < N; r += nth) Ms[(size_t)r * N + r] = 0.0;
     }
-    for (int r = tid; r < N; r += nth) Ms[(size_t)r * N + r] = 0.0;
     __syncthreads();
     const bool pp_wave = (nwv >= 2) ? (wid >= nbw) : true;
     const bool bond_wave = (nwv >= 2) ? (wid < nbw) : true;
@@ -446,7 +490,9 @@ __device__ __attribute__((noinline)) void wg_measure_sample(const DevModel& m, S
 // row covers a whole 100-wide row of X^-1 in one instruction.  MEAS
 // instantiates the measurement fused into the store of each sample (kept
 // out of the plain sampler so it does not weigh on its register allocation).
-template <int NMAX, bool LDSMAT, bool MEAS>
+// BLK: the copies are in block form (DevModel::block) and each wave keeps its
+// copy's F^-1 instead of the folded X^-1.
+template <int NMAX, bool LDSMAT, bool MEAS, bool BLK>
 __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
   constexpr int KD = NMAX >= 128 ? NMAX / 128 : 1;
   constexpr int UN = 8;  // rows in flight per wave
@@ -570,7 +616,64 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
       double2 Pk[KD], Qk[KD], bk[KD];
       double bMa = 0.0, vec_ba = 0.0, invm_ab = 0.0, pa = 0.0, pb = 0.0, qa = 0.0, qb = 0.0;
       double rq = 1.0;  // this copy's Pf ratio
-      if (q < C) {
+      // block form: y = r^T F^-1 (lane u), row db of F^-1, c(u), and the scalars
+      double by = 0.0, brow = 0.0, bcv = 0.0, bgam = 0.0, bdel = 0.0, beps = 0.0, bbet = 0.0;
+      int bua = 0;
+      if constexpr (BLK) {
+        if (q < C) {
+          // X = [[0, F], [-F^T, 0]] with F(u, d) = Slater[site_u][site_(np+d) + N],
+          // Pf X = (-1)^(np(np-1)/2) det F.  The exchange moves the up electron
+          // ua from site s1 to s2 and the down electron db from s2 to s1:
+          // F' = F + e_ua r^T + c e_db^T, r(d) = F'(ua, d) - F(ua, d),
+          // c(u) = F'(u, db) - F(u, db) (c(ua) = 0), and with B = F^-1 and
+          // y = r^T B the ratio of pfupdate_two.f90:27-88 becomes
+          // det F'/det F = (1 + y_ua)(1 + B[db].c) - (y.c) B[db][ua].
+          const int np = m.npair;
+          const bool aup = msa < np;
+          bua = __builtin_amdgcn_readfirstlane(aup ? msa : msb);
+          const int db = __builtin_amdgcn_readfirstlane((aup ? msb : msa) - np);
+          const int s1 = aup ? i1 : i2, s2 = aup ? i2 : i1;
+          const double* S1 = Sq + (size_t)s1 * 2 * N;
+          const double* S2 = Sq + (size_t)s2 * 2 * N;
+          double rv = 0.0;
+          if (lane < np) {
+            const int sd = s.eidx[np + lane];
+            rv = lane == db ? S2[s1 + N] - S1[s2 + N] : S2[sd + N] - S1[sd + N];
+            const double* Su = Sq + (size_t)s.eidx[lane] * 2 * N;
+            bcv = lane == bua ? 0.0 : Su[s1 + N] - Su[s2 + N];
+          }
+          double* rt = wv;  // r(d) for every d, zero past np
+          rt[lane] = rv;
+          wave_sync();
+          const __amdgpu_buffer_rsrc_t rs = block_rsrc(Mq, np * np * 8);
+          const int vo = lane < np ? 8 * lane : RBMPP_OOB;
+          const int rowb = 8 * np;
+          double y = 0.0;
+          int d = 0;
+          for (; d + UN <= np; d += UN) {
+            double g[UN];
+#pragma unroll
+            for (int u = 0; u < UN; u++) g[u] = bload1(rs, vo, (d + u) * rowb);
+#pragma unroll
+            for (int u = 0; u < UN; u++) y = fma(rt[d + u], g[u], y);
+          }
+          for (; d < np; d++) y = fma(rt[d], bload1(rs, vo, d * rowb), y);
+          brow = bload1(rs, vo, db * rowb);
+          by = y;
+          bgam = readlane_d(y, bua);
+          bbet = readlane_d(brow, bua);
+          bdel = readlane_d(wave_sum_to63(brow * bcv), 63);
+          beps = readlane_d(wave_sum_to63(y * bcv), 63);
+          const double ratio = (1.0 + bgam) * (1.0 + bdel) - beps * bbet;
+          rq = ratio;
+          const double pfn = ratio * s.pf[q];
+          const double2 c = psi_wo_pf_exchanged_wave(m, tb, q, s.thc, s.thr, sfac * s.sgn[q >> 1], i1, i2, x1, x2, lane);
+          if (lane == 0) {
+            s.pfn[q] = pfn;
+            s.psiq[q] = cscale(c, pfn);
+          }
+        }
+      } else if (q < C) {
         const double* colA = Sq + (size_t)rsa * 2 * N;
         const double* colB = Sq + (size_t)rsb * 2 * N;
         // (a_j, b_j) for j in [-N/2, 3N/2) at ab[j + N/2] (periodic in N), so
@@ -721,6 +824,37 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
       if (rng.u01() < r1 * r1) {
         // ---- accept: rank-4 update of this copy's X^-1 (calculate_new_InvM_two_child)
         if (q < C) {
+          if constexpr (BLK) {
+            // Sherman-Morrison-Woodbury for F' = F + e_ua r^T + c e_db^T:
+            // B'[d][u] = B[d][u] + B[d][ua] al_u + x_d be_u with x = B c (old B),
+            // al = -((1 + B[db].c) y - (y.c) B[db]) / R,
+            // be = -((1 + y_ua) B[db] - B[db][ua] y) / R
+            const int np = m.npair;
+            const double ia = -1.0 / rq;
+            const double al = ia * ((1.0 + bdel) * by - beps * brow);
+            const double be = ia * ((1.0 + bgam) * brow - bbet * by);
+            const __amdgpu_buffer_rsrc_t rs = block_rsrc(Mq, np * np * 8);
+            const int vo = lane < np ? 8 * lane : RBMPP_OOB;
+            const int rowb = 8 * np;
+            int d = 0;
+            for (; d + UN <= np; d += UN) {
+              double g[UN];
+#pragma unroll
+              for (int u = 0; u < UN; u++) g[u] = bload1(rs, vo, (d + u) * rowb);
+#pragma unroll
+              for (int u = 0; u < UN; u++) {
+                const double xd = readlane_d(wave_sum_to63(g[u] * bcv), 63);
+                const double cd = readlane_d(g[u], bua);
+                bstore1(fma(cd, al, fma(xd, be, g[u])), rs, vo, (d + u) * rowb);
+              }
+            }
+            for (; d < np; d++) {
+              const double g = bload1(rs, vo, d * rowb);
+              const double xd = readlane_d(wave_sum_to63(g * bcv), 63);
+              const double cd = readlane_d(g, bua);
+              bstore1(fma(cd, al, fma(xd, be, g)), rs, vo, d * rowb);
+            }
+          } else {
           const double a_ = -pa, b_ = pb, c_ = qa, d_ = -qb;
           const double e_ = -bMa - vec_ba;
           const double f_ = invm_ab;
@@ -823,6 +957,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
               }
             }
           }
+          }  // folded form
           // conditioning guard: a Pf ratio outside (1/g, g) means this copy
           // passes close to a singular configuration
           if (a.cond_guard > 0.0 && lane == 0) {
@@ -908,24 +1043,33 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
 }
 
 template <int NMAX, bool LDSMAT>
-static void launch_sweep_t(int C, const SweepArgs& a, hipStream_t st) {
+static void launch_sweep_t(int C, bool blk, const SweepArgs& a, hipStream_t st) {
   // one wave per projection copy
   if constexpr (LDSMAT) {
+    if (blk) {
+      if (a.measure)
+        hipLaunchKernelGGL((k_sweep<NMAX, LDSMAT, true, true>), dim3(a.nwalk), dim3(64 * C), 0, st, a);
+      else
+        hipLaunchKernelGGL((k_sweep<NMAX, LDSMAT, false, true>), dim3(a.nwalk), dim3(64 * C), 0, st, a);
+      return;
+    }
     if (a.measure) {
-      hipLaunchKernelGGL((k_sweep<NMAX, LDSMAT, true>), dim3(a.nwalk), dim3(64 * C), 0, st, a);
+      hipLaunchKernelGGL((k_sweep<NMAX, LDSMAT, true, false>), dim3(a.nwalk), dim3(64 * C), 0, st, a);
       return;
     }
   }
-  hipLaunchKernelGGL((k_sweep<NMAX, LDSMAT, false>), dim3(a.nwalk), dim3(64 * C), 0, st, a);
+  hipLaunchKernelGGL((k_sweep<NMAX, LDSMAT, false, false>), dim3(a.nwalk), dim3(64 * C), 0, st, a);
 }
 
 int launch_sweep(const DevModel& hm, const SweepArgs& a, hipStream_t st) {
   const int N = hm.N;
   if (hm.C > RBMPP_CMAX || hm.mr > RBMPP_MRMAX || hm.mc > RBMPP_MCMAX || (N & 1)) return -1;
   if (N > RBMPP_NLDS && !a.ws.scratch) return -1;
-  if (N <= 64) launch_sweep_t<64, true>(hm.C, a, st);
-  else if (N <= RBMPP_NLDS) launch_sweep_t<128, true>(hm.C, a, st);
-  else if (N <= 256) launch_sweep_t<256, false>(hm.C, a, st);
+  // block form needs the register recompute (16 waves, N <= RBMPP_NLDS)
+  const bool blk = hm.block && hm.C == RBMPP_CMAX && N <= RBMPP_NLDS;
+  if (N <= 64) launch_sweep_t<64, true>(hm.C, blk, a, st);
+  else if (N <= RBMPP_NLDS) launch_sweep_t<128, true>(hm.C, blk, a, st);
+  else if (N <= 256) launch_sweep_t<256, false>(hm.C, false, a, st);
   else return -1;
   return 0;
 }

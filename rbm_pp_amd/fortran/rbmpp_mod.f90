@@ -24,7 +24,8 @@ module rbmpp_mod
 
   integer(c_int), parameter, public :: RBMPP_OK = 0
   integer(c_int), parameter, public :: RBMPP_S_FP64 = 0, RBMPP_S_FP32 = 1, RBMPP_S_BF16X3 = 2
-  integer(c_int), parameter, public :: RBMPP_PROJ_NORMAL = 0, RBMPP_PROJ_TRANSVERSE = 1
+  integer(c_int), parameter, public :: RBMPP_PROJ_NORMAL = 0, RBMPP_PROJ_TRANSVERSE = 1, &
+                                        RBMPP_PROJ_NORMAL_EXACT = 2
 
   ! struct rbmpp_config (field order and kinds as in rbmpp.h)
   type, bind(C), public :: rbmpp_config

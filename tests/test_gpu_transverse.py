@@ -84,7 +84,7 @@ def test_switching_back_restores_the_normal_weights(oracle_lib):
         assert _relerr(nm[s], orc.eval(x)["psi"]) < RTOL
     assert _relerr(tr, nm) > 1e-3  # the two variants are different wave functions
     with pytest.raises(RuntimeError):
-        eng.set_projection(2)
+        eng.set_projection(3)
 
 
 @pytest.mark.parametrize("L,ar,ac,W,ns,K", [(4, 1, 0, 4, 6, 8), (6, 2, 1, 3, 4, 20), (10, 4, 0, 2, 3, 32)])
