@@ -59,6 +59,7 @@ struct rbmpp_handle {
   // correlation run: per-workgroup partial sums and their total
   double *d_cpart = nullptr, *d_csum = nullptr;
   size_t cpart_cap = 0;
+  double* d_bscr = nullptr;  // F^-1 of every copy per workgroup of the block-form estimator
   int* d_status = nullptr;
   int local_grid = 0;
   size_t part_cap = 0;
@@ -416,6 +417,9 @@ static int ensure_part(rbmpp_handle* h, size_t n) {
   return 0;
 }
 
+// workgroups of the block-form estimator (each holds C (N/2)^2 doubles of scratch)
+static const int BLK_GRID = 1024;
+
 static int run_local(rbmpp_handle* h, int n, const signed char* d_x, int mode, double shift, const int* ex1,
                      const int* ex2, double* d_invm, double* d_pf, double2* d_psiq, double2* d_expsi,
                      double* d_expf, double* d_corr = nullptr, int grid_cap = 0) {
@@ -443,7 +447,15 @@ static int run_local(rbmpp_handle* h, int n, const signed char* d_x, int mode, d
   a.ex_pf = d_expf;
   a.status = h->d_status;
   a.corr_part = d_corr;
-  const int grid = std::min(n, grid_cap > 0 ? grid_cap : h->local_grid);
+  int cap = grid_cap > 0 ? grid_cap : h->local_grid;
+  const DevModel hm0 = devmodel(h);
+  if (!d_corr && hm0.block) {
+    // the block-form estimator keeps F^-1 of every copy per workgroup
+    cap = std::min(cap, BLK_GRID);
+    if (!h->d_bscr && (rc = dalloc(h, &h->d_bscr, (size_t)BLK_GRID * h->C * (h->N / 2) * (h->N / 2)))) return rc;
+    a.scratch = h->d_bscr;
+  }
+  const int grid = std::min(n, cap);
   // the kernel reads the descriptor from device memory; real_only may have
   // been switched for this call
   {

@@ -96,6 +96,9 @@ int launch_sweep(const DevModel& hm, const SweepArgs& a, hipStream_t s);
 int launch_local(const DevModel& hm, const LocalArgs& a, int grid, hipStream_t s);
 // the N <= RBMPP_NLDS estimator (register Gauss-Jordan, 1024 threads)
 void launch_meas(const LocalArgs& a, int grid, hipStream_t s);
+// block form (DevModel::block, C == RBMPP_CMAX, N <= RBMPP_NLDS): a.scratch
+// holds grid x C x (N/2)^2 doubles for the F^-1 of every copy
+void launch_meas_blk(const LocalArgs& a, int grid, hipStream_t s);
 // the same estimator with the correlation measurement (N <= RBMPP_NLDS)
 void launch_meas_corr(const LocalArgs& a, int grid, hipStream_t s);
 // out[k] = sum_c part[c * stride + k] for k < len, c < nchunk (fixed order)

@@ -268,6 +268,9 @@ int launch_local(const DevModel& hm, const LocalArgs& a, int grid, hipStream_t s
     // the correlation run: pair pass of the register estimator only
     if (N > RBMPP_NLDS || hm.L < 4 || (N & 3)) return -1;
     launch_meas_corr(a, grid, st);
+  } else if (hm.block && hm.C == RBMPP_CMAX && N <= RBMPP_NLDS) {
+    if (!a.scratch) return -1;
+    launch_meas_blk(a, grid, st);
   } else if (N <= RBMPP_NLDS) launch_meas(a, grid, st);
   else if (N <= 256) {
     if (!a.scratch) return -1;

@@ -1,0 +1,346 @@
+// Per-configuration estimators in block form (DevModel::block, normal
+// projection): every X is [[0, F], [-F^T, 0]] with F the np x np block of up
+// rows and down columns, so Pf X = (-1)^(np(np-1)/2) det F and
+// X^-1 = [[0, -F^-T], [F^-1, 0]] (calculate_MAll, matrix.f90:6-110, for this
+// structure).  One workgroup of 1024 threads per configuration (grid-stride):
+//  * each of the 16 waves inverts one copy's F by Gauss-Jordan in registers
+//    (lane i holds row i; partial pivoting over the rows not yet used, chosen
+//    logically), with no workgroup barrier, and writes F^-1 to the
+//    workgroup's global scratch;
+//  * then copy by copy F^-1 is staged in LDS for the exchange ratio of every
+//    antiparallel bond (calculate_new_PfM_two_child, pfupdate_two.f90:27-88,
+//    as det F'/det F for F' = F + e_ua r^T + c e_db^T) and for the
+//    pair-product log-derivatives (slater.f90:136-167).
+// The energy and O rows follow calc_phys_quantity.f90:20-246 as in the skew
+// estimator (rbmpp_meas_impl.h).
+#include "rbmpp_kernels.h"
+#include "rbmpp_measure.h"
+#include "rbmpp_skew.h"
+
+namespace {
+
+using rbmpp_skew::wave_amax_to63;
+
+constexpr int BNT = 1024;                // threads per workgroup
+constexpr int BNW = BNT / RBMPP_WAVE;    // waves: one per projection copy
+constexpr int NMX = RBMPP_NLDS;          // largest N
+constexpr int HMX = NMX / 2;             // largest np = N/2
+constexpr int BT = 2;                    // bonds per wave and pass
+
+struct BlkSmem {
+  double binv[HMX * HMX];        // F^-1 of the current copy: B[d][u] at d * np + u
+  double prow[BNW][HMX];         // pivot row of each wave's elimination
+  int piv[BNW][HMX];             // pivot row (lane) of each elimination step
+  double vr[BNW][BT][HMX];       // r vectors of the bonds in flight
+  double opp[2][4 * NMX];        // pair-product O, re / im
+  double2 psib[4 * NMX];         // per antiparallel bond: sum over copies of psi(x')
+  double rat[4 * NMX];           // per antiparallel bond: Pf ratio of the current copy
+  double thr[RBMPP_MRMAX];
+  double2 thc[RBMPP_MCMAX];
+  double pf[RBMPP_CMAX];
+  double2 psiq[RBMPP_CMAX], psiwo[RBMPP_CMAX];
+  double selv[2];
+  int x[NMX], eidx[NMX], ecfg[NMX], site[NMX];
+  short bi1[4 * NMX], bi2[4 * NMX];
+  int sgn[RBMPP_CMAX / 2];
+  unsigned sgnbits[RBMPP_CMAX / 2];
+  int nb, status;
+};
+
+// cold helpers kept out of line (their transcendental temporaries stay out of
+// the register allocation of the elimination and bond loops)
+__device__ __attribute__((noinline)) double2 exchanged_amp(const DevModel& m, int q, const BlkSmem& s, int sgn,
+                                                           int i1, int i2) {
+  return psi_wo_pf_exchanged(m, q, s.thc, s.thr, sgn, i1, i2, s.x[i1], s.x[i2]);
+}
+__device__ __attribute__((noinline)) void theta_sign(const DevModel& m, BlkSmem& s) { wg_theta_sign(m, s, 0); }
+__device__ __attribute__((noinline)) void o_row(const DevModel& m, const BlkSmem& s, double rw, double* ore,
+                                                double* oim, bool cplx_rows) {
+  wg_orow(m, s, rw, ore, oim, cplx_rows);
+}
+__device__ __attribute__((noinline)) int bonds_of(const DevModel& m, BlkSmem& s) { return bond_list(m, s); }
+__device__ __attribute__((noinline)) double2 eloc_of(const DevModel& m, const BlkSmem& s, double2 p) {
+  return eloc_from_bonds(m, s, p);
+}
+
+// det F and F^-1 of copy S by one wave (NB >= np registers per lane).
+// F(i, j) = Slater[site_i][site_(np+j) + N]; with the block weights (0 and 1)
+// Slater_Elm is exactly skew there, so the lanes read it as
+// -Slater[site_(np+j) + N][site_i], one row per j.  In-place Gauss-Jordan
+// exchange steps on the pivots (p_k, k), p_k the remaining row with the
+// largest |F(., k)| (smallest index on ties); afterwards row p_k, column m of
+// the registers is F^-1(k, p_m), written to Bout[k * np + p_m].  det F is the
+// product of the pivots times the sign of the row order.  Returns 0 for an
+// exactly singular pivot (uniform).
+template <int NB>
+__device__ __forceinline__ double wave_block_inverse(const double* __restrict__ S, int N, int np,
+                                                     const int* eidx, double* prow, int* piv, double* Bout,
+                                                     int lane) {
+  double a[NB];
+  const int si = lane < np ? eidx[lane] : 0;
+#pragma unroll
+  for (int j = 0; j < NB; j++) {
+    double v = 0.0;
+    if (j < np) {
+      const double* row = S + (size_t)(eidx[np + j] + N) * 2 * N;
+      if (lane < np) v = -row[si];
+    }
+    a[j] = v;
+  }
+  unsigned long long rem = np >= 64 ? ~0ull : ((1ull << np) - 1ull);
+  double det = 1.0;
+  int neg = 0, myk = 0;
+  for (int k = 0; k < np; k++) {
+    double ak = 0.0;
+#pragma unroll
+    for (int j = 0; j < NB; j++)
+      if (j == k) ak = a[j];
+    double bv = ((rem >> lane) & 1ull) ? fabs(ak) : -1.0;
+    int bi = lane;
+    wave_amax_to63(bv, bi);
+    const int p = __builtin_amdgcn_readlane(bi, 63);
+    const double pv = readlane_d(ak, p);
+    if (pv == 0.0) return 0.0;
+    det *= pv;
+    neg ^= __builtin_popcountll(rem & ((1ull << p) - 1ull)) & 1;
+    rem &= ~(1ull << p);
+    const bool isp = lane == p;
+    if (isp) {
+      myk = k;
+#pragma unroll
+      for (int j = 0; j < NB; j++) prow[j] = a[j];
+    }
+    if (lane == 0) piv[k] = p;
+    wave_sync();
+    const double inv = 1.0 / pv;
+    const double t = ak * inv;
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+      double v = isp ? a[j] * inv : fma(-t, prow[j], a[j]);
+      if (j == k) v = isp ? inv : -t;
+      a[j] = v;
+    }
+    wave_sync();  // prow is rewritten by the next step
+  }
+  if (lane < np) {
+#pragma unroll
+    for (int m = 0; m < NB; m++)
+      if (m < np) Bout[(size_t)myk * np + piv[m]] = a[m];
+  }
+  return neg ? -det : det;
+}
+
+// the elimination of one copy, out of line with its own register budget; NB
+// is the smallest register count that holds np
+template <int NB>
+__device__ __attribute__((noinline)) double block_inverse_nb(const double* S, int N, int np, const int* eidx,
+                                                             double* prow, int* piv, double* Bout, int lane) {
+  return wave_block_inverse<NB>(S, N, np, eidx, prow, piv, Bout, lane);
+}
+__device__ __forceinline__ double copy_block_inverse(const double* S, int N, int np, const int* eidx,
+                                                     double* prow, int* piv, double* Bout, int lane) {
+  if (np <= 8) return block_inverse_nb<8>(S, N, np, eidx, prow, piv, Bout, lane);
+  if (np <= 18) return block_inverse_nb<18>(S, N, np, eidx, prow, piv, Bout, lane);
+  if (np <= 32) return block_inverse_nb<32>(S, N, np, eidx, prow, piv, Bout, lane);
+  if (np <= 50) return block_inverse_nb<50>(S, N, np, eidx, prow, piv, Bout, lane);
+  return block_inverse_nb<HMX>(S, N, np, eidx, prow, piv, Bout, lane);
+}
+
+__global__ __launch_bounds__(BNT) void k_meas_blk(LocalArgs a) {
+  __shared__ BlkSmem s;
+  const DevModel& m = *a.m;
+  const int tid = threadIdx.x, nth = BNT;
+  const int lane = tid & (RBMPP_WAVE - 1), wid = tid >> 6;
+  const int N = m.N, C = m.C, np = m.npair;
+  const bool want_eloc = a.mode & 1, want_o = a.mode & 2, want_inv = a.mode & 4, custom = a.mode & 8;
+  const double pfsign = ((np * (np - 1) / 2) & 1) ? -1.0 : 1.0;  // Pf X = pfsign det F
+  double* Bw = a.scratch + (size_t)blockIdx.x * C * np * np;  // F^-1 of every copy
+
+  for (int cfg = blockIdx.x; cfg < a.nconf; cfg += gridDim.x) {
+    __syncthreads();
+    for (int i = tid; i < N; i += nth) s.x[i] = a.xs[(size_t)cfg * N + i];
+    if (tid == 0) s.status = WS_OK;
+    for (int k = tid; k < 4 * NMX; k += nth) {
+      s.opp[0][k] = 0.0;
+      s.opp[1][k] = 0.0;
+      s.psib[k] = make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      // x_to_Ele_Idx_Ele_Cfg (interface_for_PP.f90:5-33)
+      int u = 0, d = np;
+      for (int i = 0; i < N; i++) {
+        if (s.x[i] == 1) { s.eidx[u] = i; s.ecfg[i] = u; u++; }
+        else { s.eidx[d] = i; s.ecfg[i] = d; d++; }
+      }
+      if (u != np) s.status = WS_INIT;
+      int nb = 0;
+      if (custom) {
+        s.bi1[0] = (short)a.ex_i1[cfg];
+        s.bi2[0] = (short)a.ex_i2[cfg];
+        nb = 1;
+      } else if (want_eloc) {
+        nb = bonds_of(m, s);
+      }
+      s.nb = nb;
+    }
+    __syncthreads();
+    if (s.status != WS_OK) {
+      if (tid == 0 && a.status) a.status[cfg] = s.status;
+      continue;
+    }
+    theta_sign(m, s);
+    for (int q = tid; q < C; q += nth) s.psiwo[q] = psi_wo_pf(m, q, s.thc, s.thr, s.sgn[q >> 1]);
+    // ---- det and F^-1 of every copy, one wave each
+    if (wid < C) {
+      const int q = wid;
+      const double det = copy_block_inverse(m.se + (size_t)q * 4 * N * N, N, np, s.eidx, s.prow[wid],
+                                            s.piv[wid], Bw + (size_t)q * np * np, lane);
+      if (lane == 0) {
+        s.pf[q] = pfsign * det;
+        if (det == 0.0) s.status = WS_SINGULAR;
+      }
+    }
+    __syncthreads();
+    if (s.status != WS_OK) {
+      if (tid == 0 && a.status) a.status[cfg] = s.status;
+      continue;
+    }
+    const int nb = s.nb;
+    for (int q = 0; q < C; q++) {
+      const int g = q >> 1, sidx = q & 1;
+      const double* S = m.se + (size_t)q * 4 * N * N;
+      const double pfq = s.pf[q];
+      __syncthreads();  // the previous copy's readers of binv and site are done
+      for (int e = tid; e < np * np; e += nth) s.binv[e] = Bw[(size_t)q * np * np + e];
+      for (int e = tid; e < N; e += nth) s.site[e] = m.wmap[g * N + s.eidx[e]];
+      if (tid == 0) s.psiq[q] = cscale(s.psiwo[q], pfq);
+      __syncthreads();
+      if (want_inv) {
+        // reference layout out[i*N + j] = X^-1(i, j): X^-1(u, np+d) = -F^-1(d, u),
+        // X^-1(np+d, u) = F^-1(d, u), zero same-spin blocks
+        double* out = a.invm + ((size_t)cfg * C + q) * N * N;
+        for (int e = tid; e < N * N; e += nth) {
+          const int i = e / N, j = e - i * N;
+          double v = 0.0;
+          if (i < np && j >= np) v = -s.binv[(j - np) * np + i];
+          else if (i >= np && j < np) v = s.binv[(i - np) * np + j];
+          out[e] = v;
+        }
+      }
+      if (want_o) {
+        // calculate_derivative_Pfaffian_child (slater.f90:136-167) weighted by
+        // psi_wo_pf(q): X^-1(u, np+d) = -F^-1(d,u) lands in the orbital of
+        // (site(np+d), site(u)) with weight -ss, X^-1(np+d, u) = F^-1(d,u) in
+        // that of (site(u), site(np+d)) with weight cc
+        const double cc = pfq * m.sp_cc[sidx], ss = pfq * m.sp_ss[sidx];
+        const double2 wq = s.psiwo[q];
+        for (int e = tid; e < np * np; e += nth) {
+          const int d = e / np, u = e - d * np;
+          const double bv = s.binv[e];
+          const int su = s.site[u], sd = s.site[np + d];
+          const int o1 = m.orb[sd * N + su], o2 = m.orb[su * N + sd];
+          const double d1 = ss * bv, d2 = cc * bv;
+          atomicAdd(&s.opp[0][o1], d1 * wq.x);
+          atomicAdd(&s.opp[0][o2], d2 * wq.x);
+          if (!m.real_only) {
+            atomicAdd(&s.opp[1][o1], d1 * wq.y);
+            atomicAdd(&s.opp[1][o2], d2 * wq.y);
+          }
+        }
+      }
+      // exchange ratios, BT bonds per wave and pass: the up electron ua moves
+      // from s1 to s2, the down electron db from s2 to s1; with B = F^-1,
+      // y = r^T B: det F'/det F = (1 + y_ua)(1 + B[db].c) - (y.c) B[db][ua]
+      for (int p = wid; BT * p < nb; p += BNW) {
+        int bb[BT], ua[BT], db[BT];
+        double cv[BT];
+#pragma unroll
+        for (int t = 0; t < BT; t++) {
+          bb[t] = min(BT * p + t, nb - 1);  // a missing bond repeats the last one (not written)
+          const int i1 = s.bi1[bb[t]], i2 = s.bi2[bb[t]];
+          const int msa = s.ecfg[i1], msb = s.ecfg[i2];
+          const bool aup = msa < np;
+          ua[t] = __builtin_amdgcn_readfirstlane(aup ? msa : msb);
+          db[t] = __builtin_amdgcn_readfirstlane((aup ? msb : msa) - np);
+          const int s1 = aup ? i1 : i2, s2 = aup ? i2 : i1;
+          const double* S1 = S + (size_t)s1 * 2 * N;
+          const double* S2 = S + (size_t)s2 * 2 * N;
+          double rv = 0.0;
+          cv[t] = 0.0;
+          if (lane < np) {
+            const int sd = s.eidx[np + lane];
+            rv = lane == db[t] ? S2[s1 + N] - S1[s2 + N] : S2[sd + N] - S1[sd + N];
+            const double* Su = S + (size_t)s.eidx[lane] * 2 * N;
+            cv[t] = lane == ua[t] ? 0.0 : Su[s1 + N] - Su[s2 + N];
+          }
+          if (lane < np) s.vr[wid][t][lane] = rv;
+        }
+        wave_sync();
+        double y[BT];
+#pragma unroll
+        for (int t = 0; t < BT; t++) y[t] = 0.0;
+        if (lane < np) {
+#pragma unroll 4
+          for (int d = 0; d < np; d++) {
+            const double bv = s.binv[d * np + lane];
+#pragma unroll
+            for (int t = 0; t < BT; t++) y[t] = fma(s.vr[wid][t][d], bv, y[t]);
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < BT; t++) {
+          if (BT * p + t >= nb) break;  // wave-uniform
+          const double brow = lane < np ? s.binv[db[t] * np + lane] : 0.0;
+          const double gam = readlane_d(y[t], ua[t]);
+          const double bet = readlane_d(brow, ua[t]);
+          const double del = wave_sum_to63(brow * cv[t]);
+          const double eps = wave_sum_to63(y[t] * cv[t]);
+          if (lane == 63) s.rat[bb[t]] = (1.0 + gam) * (1.0 + del) - eps * bet;
+        }
+        wave_sync();  // vr is rewritten by the next pass
+      }
+      __syncthreads();
+      // psi(x') of this copy for every bond, one thread per bond
+      for (int b = tid; b < nb; b += nth) {
+        const int j1 = s.bi1[b], j2 = s.bi2[b];
+        int sfac = -1;
+        if (m.subB[j1]) sfac = -sfac;
+        if (m.subB[j2]) sfac = -sfac;
+        const double pfn = s.rat[b] * pfq;
+        const double2 c = exchanged_amp(m, q, s, sfac * s.sgn[g], j1, j2);
+        s.psib[b] = cadd(s.psib[b], cscale(c, pfn));
+        if (custom && a.ex_pf) a.ex_pf[(size_t)cfg * C + q] = pfn;
+      }
+    }
+    __syncthreads();
+    // amplitude, local energy, reweighting
+    if (tid == 0) {
+      double2 p = make_double2(0.0, 0.0);
+      for (int q = 0; q < C; q++) p = cadd(p, s.psiq[q]);
+      const double rw = a.shift >= 0.0 ? 1.0 / (cmod(p) + a.shift) : 1.0;
+      a.psi[cfg] = p;
+      if (a.rw) a.rw[cfg] = rw;
+      if (want_eloc && !custom) a.eloc[cfg] = cscale(eloc_of(m, s, p), rw);
+      if (custom && a.ex_psi) a.ex_psi[cfg] = s.psib[0];
+      s.selv[0] = rw;
+    }
+    for (int q = tid; q < C; q += nth) {
+      if (a.pf) a.pf[(size_t)cfg * C + q] = s.pf[q];
+      if (a.psi_q) a.psi_q[(size_t)cfg * C + q] = s.psiq[q];
+    }
+    __syncthreads();
+    if (want_o) {
+      const bool cplx_rows = !m.real_only;
+      double* ore = a.orows + (size_t)(cplx_rows ? 2 * cfg : cfg) * m.nv;
+      o_row(m, s, s.selv[0], ore, ore + m.nv, cplx_rows);
+    }
+    if (tid == 0 && a.status) a.status[cfg] = WS_OK;
+  }
+}
+
+}  // namespace
+
+void launch_meas_blk(const LocalArgs& a, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_meas_blk, dim3(grid), dim3(BNT), 0, st, a);
+}
