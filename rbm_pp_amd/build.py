@@ -14,6 +14,16 @@ SOURCES = ["rbmpp_capi.hip", "rbmpp_sampler.hip", "rbmpp_local.hip", "rbmpp_meas
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
          # packed FP32 instructions are kept out of every kernel
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+# per-source additions.  The sampler is built without interprocedural
+# register allocation: with it the cold out-of-line helpers (recompute,
+# amplitudes) shape the register assignment of the proposal loop that calls
+# them (97 spilled registers instead of 36, 1.3x the sweep time); the other
+# kernels are slightly faster with it.
+EXTRA_FLAGS = {"rbmpp_sampler.hip": ["-mllvm", "-enable-ipra=0"]}
+
+
+def flags_for(src: str) -> list:
+    return FLAGS + EXTRA_FLAGS.get(src, [])
 
 
 def build(verbose: bool = False) -> str:
@@ -22,7 +32,7 @@ def build(verbose: bool = False) -> str:
     procs = []
     for src in SOURCES:
         obj = os.path.join(objdir, src.replace(".hip", ".o"))
-        cmd = ["hipcc", *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = ["hipcc", *flags_for(src), "-c", os.path.join(CSRC, src), "-o", obj]
         procs.append((src, obj, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
     objs = []
     for src, obj, p in procs:

@@ -4,33 +4,33 @@
 // X^-1 = [[0, -F^-T], [F^-1, 0]] (calculate_MAll, matrix.f90:6-110, for this
 // structure).  One workgroup of 1024 threads per configuration (grid-stride):
 //  * each of the 16 waves inverts one copy's F by Gauss-Jordan in registers
-//    (lane i holds row i; partial pivoting over the rows not yet used, chosen
-//    logically), with no workgroup barrier, and writes F^-1 to the
-//    workgroup's global scratch;
+//    (rbmpp_block.h: lane i holds row i; partial pivoting over the rows not
+//    yet used, chosen logically), with no workgroup barrier, and writes F^-1
+//    to the workgroup's global scratch;
 //  * then copy by copy F^-1 is staged in LDS for the exchange ratio of every
 //    antiparallel bond (calculate_new_PfM_two_child, pfupdate_two.f90:27-88,
 //    as det F'/det F for F' = F + e_ua r^T + c e_db^T) and for the
 //    pair-product log-derivatives (slater.f90:136-167).
 // The energy and O rows follow calc_phys_quantity.f90:20-246 as in the skew
 // estimator (rbmpp_meas_impl.h).
+#include "rbmpp_block.h"
 #include "rbmpp_kernels.h"
 #include "rbmpp_measure.h"
-#include "rbmpp_skew.h"
 
 namespace {
 
-using rbmpp_skew::wave_amax_to63;
+using rbmpp_block::copy_block_inverse;
 
 constexpr int BNT = 1024;                // threads per workgroup
 constexpr int BNW = BNT / RBMPP_WAVE;    // waves: one per projection copy
 constexpr int NMX = RBMPP_NLDS;          // largest N
-constexpr int HMX = NMX / 2;             // largest np = N/2
+constexpr int HMX = rbmpp_block::HMX;    // largest np = N/2
 constexpr int BT = 2;                    // bonds per wave and pass
 
 struct BlkSmem {
   double binv[HMX * HMX];        // F^-1 of the current copy: B[d][u] at d * np + u
-  double prow[BNW][HMX];         // pivot row of each wave's elimination
   int piv[BNW][HMX];             // pivot row (lane) of each elimination step
+  double stage[BNW][8 * HMX];    // write-back stage of each wave's elimination
   double vr[BNW][BT][HMX];       // r vectors of the bonds in flight
   double opp[2][4 * NMX];        // pair-product O, re / im
   double2 psib[4 * NMX];         // per antiparallel bond: sum over copies of psi(x')
@@ -61,89 +61,6 @@ __device__ __attribute__((noinline)) void o_row(const DevModel& m, const BlkSmem
 __device__ __attribute__((noinline)) int bonds_of(const DevModel& m, BlkSmem& s) { return bond_list(m, s); }
 __device__ __attribute__((noinline)) double2 eloc_of(const DevModel& m, const BlkSmem& s, double2 p) {
   return eloc_from_bonds(m, s, p);
-}
-
-// det F and F^-1 of copy S by one wave (NB >= np registers per lane).
-// F(i, j) = Slater[site_i][site_(np+j) + N]; with the block weights (0 and 1)
-// Slater_Elm is exactly skew there, so the lanes read it as
-// -Slater[site_(np+j) + N][site_i], one row per j.  In-place Gauss-Jordan
-// exchange steps on the pivots (p_k, k), p_k the remaining row with the
-// largest |F(., k)| (smallest index on ties); afterwards row p_k, column m of
-// the registers is F^-1(k, p_m), written to Bout[k * np + p_m].  det F is the
-// product of the pivots times the sign of the row order.  Returns 0 for an
-// exactly singular pivot (uniform).
-template <int NB>
-__device__ __forceinline__ double wave_block_inverse(const double* __restrict__ S, int N, int np,
-                                                     const int* eidx, double* prow, int* piv, double* Bout,
-                                                     int lane) {
-  double a[NB];
-  const int si = lane < np ? eidx[lane] : 0;
-#pragma unroll
-  for (int j = 0; j < NB; j++) {
-    double v = 0.0;
-    if (j < np) {
-      const double* row = S + (size_t)(eidx[np + j] + N) * 2 * N;
-      if (lane < np) v = -row[si];
-    }
-    a[j] = v;
-  }
-  unsigned long long rem = np >= 64 ? ~0ull : ((1ull << np) - 1ull);
-  double det = 1.0;
-  int neg = 0, myk = 0;
-  for (int k = 0; k < np; k++) {
-    double ak = 0.0;
-#pragma unroll
-    for (int j = 0; j < NB; j++)
-      if (j == k) ak = a[j];
-    double bv = ((rem >> lane) & 1ull) ? fabs(ak) : -1.0;
-    int bi = lane;
-    wave_amax_to63(bv, bi);
-    const int p = __builtin_amdgcn_readlane(bi, 63);
-    const double pv = readlane_d(ak, p);
-    if (pv == 0.0) return 0.0;
-    det *= pv;
-    neg ^= __builtin_popcountll(rem & ((1ull << p) - 1ull)) & 1;
-    rem &= ~(1ull << p);
-    const bool isp = lane == p;
-    if (isp) {
-      myk = k;
-#pragma unroll
-      for (int j = 0; j < NB; j++) prow[j] = a[j];
-    }
-    if (lane == 0) piv[k] = p;
-    wave_sync();
-    const double inv = 1.0 / pv;
-    const double t = ak * inv;
-#pragma unroll
-    for (int j = 0; j < NB; j++) {
-      double v = isp ? a[j] * inv : fma(-t, prow[j], a[j]);
-      if (j == k) v = isp ? inv : -t;
-      a[j] = v;
-    }
-    wave_sync();  // prow is rewritten by the next step
-  }
-  if (lane < np) {
-#pragma unroll
-    for (int m = 0; m < NB; m++)
-      if (m < np) Bout[(size_t)myk * np + piv[m]] = a[m];
-  }
-  return neg ? -det : det;
-}
-
-// the elimination of one copy, out of line with its own register budget; NB
-// is the smallest register count that holds np
-template <int NB>
-__device__ __attribute__((noinline)) double block_inverse_nb(const double* S, int N, int np, const int* eidx,
-                                                             double* prow, int* piv, double* Bout, int lane) {
-  return wave_block_inverse<NB>(S, N, np, eidx, prow, piv, Bout, lane);
-}
-__device__ __forceinline__ double copy_block_inverse(const double* S, int N, int np, const int* eidx,
-                                                     double* prow, int* piv, double* Bout, int lane) {
-  if (np <= 8) return block_inverse_nb<8>(S, N, np, eidx, prow, piv, Bout, lane);
-  if (np <= 18) return block_inverse_nb<18>(S, N, np, eidx, prow, piv, Bout, lane);
-  if (np <= 32) return block_inverse_nb<32>(S, N, np, eidx, prow, piv, Bout, lane);
-  if (np <= 50) return block_inverse_nb<50>(S, N, np, eidx, prow, piv, Bout, lane);
-  return block_inverse_nb<HMX>(S, N, np, eidx, prow, piv, Bout, lane);
 }
 
 __global__ __launch_bounds__(BNT) void k_meas_blk(LocalArgs a) {
@@ -194,8 +111,8 @@ __global__ __launch_bounds__(BNT) void k_meas_blk(LocalArgs a) {
     // ---- det and F^-1 of every copy, one wave each
     if (wid < C) {
       const int q = wid;
-      const double det = copy_block_inverse(m.se + (size_t)q * 4 * N * N, N, np, s.eidx, s.prow[wid],
-                                            s.piv[wid], Bw + (size_t)q * np * np, lane);
+      const double det = copy_block_inverse(m.se + (size_t)q * 4 * N * N, N, np, s.eidx, s.piv[wid],
+                                            s.stage[wid], Bw + (size_t)q * np * np, lane);
       if (lane == 0) {
         s.pf[q] = pfsign * det;
         if (det == 0.0) s.status = WS_SINGULAR;

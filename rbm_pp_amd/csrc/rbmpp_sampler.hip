@@ -13,6 +13,7 @@
 // same Philox numbers, so the control flow is uniform across the workgroup.
 #include "rbmpp_kernels.h"
 #include "rbmpp_measure.h"
+#include "rbmpp_block.h"
 #include "rbmpp_skew.h"
 
 // ------------------------------------------------------------------
@@ -109,6 +110,35 @@ __device__ __attribute__((noinline)) void wg_recompute(const DevModel& m, SweepS
     for (int k = tid; k < 4 * rbmpp_skew::MPAD; k += nth) s.u[k] = 0.0;  // pivot-row buffers
   }
   const int H = N >> 1;
+  if (regpath && m.block) {
+    // block form: every wave inverts its copy's F (rbmpp_block.h) into
+    // Mq[d * np + u] = F^-1(d, u), all copies at once
+    const int lane = tid & 63, wid = tid >> 6, np = m.npair;
+    // pivot rows in gjbuf (4 NMAX doubles = 16 x NMAX/2 ints), the write-back
+    // stages in u (16 x 8 x NMAX/2 doubles)
+    static_assert(!LDSMAT || (SweepSmem<NMAX, LDSMAT>::UNION >= RBMPP_CMAX * 8 * (NMAX / 2) &&
+                              4 * NMAX * 2 >= RBMPP_CMAX * (NMAX / 2)),
+                  "stages and pivot rows of the block recompute fit u and gjbuf");
+    int* piv = (int*)s.gjbuf + (size_t)wid * (NMAX / 2);
+    double* stage = s.u + (size_t)wid * 8 * (NMAX / 2);
+    __syncthreads();  // s.u is free
+    if (wid < C) {
+      const int q = wid;
+      double dr[2] = {0.0, 0.0};
+      const double det = rbmpp_block::copy_block_inverse(m.se + (size_t)q * 4 * N * N, N, np, s.eidx, piv, stage,
+                                                         Mw + (size_t)q * fold_size(N), lane, check ? dr : nullptr);
+      if (lane == 0) {
+        s.pfn[q] = s.pf[q];  // old value, for the drift measure
+        s.pf[q] = (((np * (np - 1) / 2) & 1) ? -1.0 : 1.0) * det;
+        s.red[q] = dr[1] > 0.0 ? dr[0] / dr[1] : 0.0;
+        if (det == 0.0) s.status = WS_SINGULAR;
+      }
+    }
+    __syncthreads();
+    if (s.status) return;
+    if (check)
+      for (int q = 0; q < C; q++) dinv = fmax(dinv, s.red[q]);
+  } else
   for (int q = 0; q < C; q++) {
     const double* S = m.se + (size_t)q * 4 * N * N;
     double* Mq = Mw + (size_t)q * fold_size(N);
@@ -138,44 +168,23 @@ __device__ __attribute__((noinline)) void wg_recompute(const DevModel& m, SweepS
         __syncthreads();
         return;
       }
-      if (m.block) {
-        // X^-1 = [[0, -F^-T], [F^-1, 0]]: F^-1(d, u) = X^-1(np + d, u) at
-        // Mq[d * np + u]
-        const int np = m.npair;
+      // M(r,c) = InvM(r,c) = X^-1(c,r) = -X^-1(r,c) (exactly skew); the
+      // element of each pair's folded slot is written by its holder
 #pragma unroll
-        for (int aa = 0; aa < RA; aa++) {
-          const int r = tr + 16 * aa;
+      for (int aa = 0; aa < RA; aa++) {
+        const int r = tr + 16 * aa;
 #pragma unroll
-          for (int b = 0; b < RB; b++) {
-            const int c = tc + 64 * b;
-            if (r >= np && r < N && c < np) {
-              const double v = TILE(x, aa, b);
-              const size_t slot = (size_t)(r - np) * np + c;
-              lmax = fmax(lmax, fabs(v));
+        for (int b = 0; b < RB; b++) {
+          const int c = tc + 64 * b;
+          if (r < N && c < N) {
+            const double v = -TILE(x, aa, b);
+            lmax = fmax(lmax, fabs(v));
+            int d = c - r;
+            if (d < 0) d += N;
+            if (d > 0 && (d < H || (d == H && r < H))) {
+              const size_t slot = (size_t)(d - 1) * N + r;
               if (check) ldiff = fmax(ldiff, fabs(Mq[slot] - v));
               Mq[slot] = v;
-            }
-          }
-        }
-      } else {
-        // M(r,c) = InvM(r,c) = X^-1(c,r) = -X^-1(r,c) (exactly skew); the
-        // element of each pair's folded slot is written by its holder
-#pragma unroll
-        for (int aa = 0; aa < RA; aa++) {
-          const int r = tr + 16 * aa;
-#pragma unroll
-          for (int b = 0; b < RB; b++) {
-            const int c = tc + 64 * b;
-            if (r < N && c < N) {
-              const double v = -TILE(x, aa, b);
-              lmax = fmax(lmax, fabs(v));
-              int d = c - r;
-              if (d < 0) d += N;
-              if (d > 0 && (d < H || (d == H && r < H))) {
-                const size_t slot = (size_t)(d - 1) * N + r;
-                if (check) ldiff = fmax(ldiff, fabs(Mq[slot] - v));
-                Mq[slot] = v;
-              }
             }
           }
         }

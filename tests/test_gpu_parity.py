@@ -172,7 +172,9 @@ def test_measure_s_and_sr_step(oracle_lib, L, ar, ac):
 def test_fused_measurement_matches_recompute(oracle_lib, L, ar, ac):
     """Measurement fused into the sampler (fast-updated X^-1) against the
     oracle's calc_phys_quantity_opt (X^-1 recomputed per sample) on the same
-    samples: equal to the fast-update drift (bound 1e-8)."""
+    samples: equal to the fast-update drift over up to 16 accepts (bounds
+    1e-8 for norm and energy, 5e-8 normwise for g, <O> and S, whose
+    O entries carry X^-1 itself)."""
     api = _api()
     spec, p = _model(L, ar, ac, sigma_w=0.1)
     W, ns = 8, 6
@@ -187,12 +189,12 @@ def test_fused_measurement_matches_recompute(oracle_lib, L, ar, ac):
     norm = ref["norm"]
     assert got["norm"] == pytest.approx(norm, rel=1e-8)
     assert got["Etot"] == pytest.approx(ref["Etot"] / norm, rel=1e-8)
-    assert _relerr(got["gvec"], ref["gvec"] / norm) < 1e-8
-    assert _relerr(got["ovec"], ref["ovec"] / norm) < 1e-8
+    assert _relerr(got["gvec"], ref["gvec"] / norm) < 5e-8
+    assert _relerr(got["ovec"], ref["ovec"] / norm) < 5e-8
     ob = ref["ovec"] / norm
     S_ref = ref["smat"] / norm - np.outer(ob, ob)
     S = eng.smat()
-    assert _relerr(S, S_ref) < 1e-8
+    assert _relerr(S, S_ref) < 5e-8
     # the f_ii rows of S are exactly zero, as the SR step's checks require
     # (SR_pdposv.f90:18-40): the fast-updated X^-1 keeps a zero diagonal
     d = np.diag(S)

@@ -71,10 +71,10 @@ def test_scanner_flags_the_pattern():
 
 @pytest.mark.skipif(shutil.which("hipcc") is None, reason="hipcc not available")
 def test_kernels_have_no_wide_store_data_hazard_and_no_packed_fp32():
-    flags = [f for f in build.FLAGS if f != "-fPIC"]
     with tempfile.TemporaryDirectory() as tmp:
         procs = []
         for src in build.SOURCES:
+            flags = [f for f in build.flags_for(src) if f != "-fPIC"]
             out = os.path.join(tmp, src.replace(".hip", ".s"))
             cmd = ["hipcc", *flags, "--cuda-device-only", "-S", "-o", out, os.path.join(build.CSRC, src)]
             procs.append((src, out, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
