@@ -23,6 +23,19 @@ def _model(L, alphar, alphac=0, J2=0.5, seed=1, sigma_w=0.05):
     return spec, lt.random_params(spec, seed_w=seed, seed_f=seed + 100, sigma_w=sigma_w)
 
 
+def _engine(api, spec, p, proj, **kw):
+    """proj "block": the default normal projection (cos(pi/2) = 0, block-form
+    kernels); "exact": the reference's weights bit for bit (skew-form
+    kernels).  The oracle keeps the reference's weights in both cases."""
+    eng = api.Engine(spec, params=p, **kw)
+    if proj == "exact":
+        eng.set_projection(api.PROJ_NORMAL_EXACT)
+    return eng
+
+
+PROJ = pytest.mark.parametrize("proj", ["block", "exact"])
+
+
 def _configs(N, n, seed):
     rng = np.random.default_rng(seed)
     out = []
@@ -40,12 +53,13 @@ def _relerr(a, b):
 CASES = [(4, 1, 0), (4, 2, 1), (6, 2, 0), (10, 4, 0)]
 
 
+@PROJ
 @pytest.mark.parametrize("L,ar,ac", CASES)
-def test_eval_psi_pf_invm(oracle_lib, L, ar, ac):
+def test_eval_psi_pf_invm(oracle_lib, L, ar, ac, proj):
     api = _api()
     spec, p = _model(L, ar, ac)
     orc = oracle_lib.Oracle(spec, p)
-    eng = api.Engine(spec, n_walkers=1, n_samples=1, params=p)
+    eng = _engine(api, spec, p, proj, n_walkers=1, n_samples=1)
     xs = _configs(spec.N, 6, seed=L)
     got = eng.eval(xs, with_invm=True)
     for s, x in enumerate(xs):
@@ -58,12 +72,13 @@ def test_eval_psi_pf_invm(oracle_lib, L, ar, ac):
             assert _relerr(got["invm"][s, q], inv_ref) < max(RTOL, 1e-15 * cond), (s, q, cond)
 
 
+@PROJ
 @pytest.mark.parametrize("L,ar,ac", CASES)
-def test_ratio_exchange(oracle_lib, L, ar, ac):
+def test_ratio_exchange(oracle_lib, L, ar, ac, proj):
     api = _api()
     spec, p = _model(L, ar, ac)
     orc = oracle_lib.Oracle(spec, p)
-    eng = api.Engine(spec, n_walkers=1, n_samples=1, params=p)
+    eng = _engine(api, spec, p, proj, n_walkers=1, n_samples=1)
     xs = _configs(spec.N, 8, seed=100 + L)
     rng = np.random.default_rng(L)
     i1 = np.array([int(rng.integers(spec.N)) for _ in xs], np.int32)
@@ -76,12 +91,13 @@ def test_ratio_exchange(oracle_lib, L, ar, ac):
         assert abs(psi[s] - psin[0]) <= RTOL * abs(psin[0])
 
 
+@PROJ
 @pytest.mark.parametrize("L,ar,ac", CASES)
-def test_local_energy_and_ok(oracle_lib, L, ar, ac):
+def test_local_energy_and_ok(oracle_lib, L, ar, ac, proj):
     api = _api()
     spec, p = _model(L, ar, ac)
     orc = oracle_lib.Oracle(spec, p)
-    eng = api.Engine(spec, n_walkers=1, n_samples=1, params=p)
+    eng = _engine(api, spec, p, proj, n_walkers=1, n_samples=1)
     xs = _configs(spec.N, 6, seed=200 + L)
     got = eng.local(xs, want_o=True)
     for s, x in enumerate(xs):
@@ -94,7 +110,8 @@ def test_local_energy_and_ok(oracle_lib, L, ar, ac):
 
 @pytest.mark.parametrize("L,ar,ac,W,ns,K", [(4, 1, 0, 4, 6, 8), (6, 2, 1, 3, 4, 20), (10, 4, 0, 2, 3, 32),
                                              (6, 2, 0, 3, 6, 0), (10, 4, 0, 2, 4, 0)])
-def test_sampler_trajectory_bit_exact(oracle_lib, L, ar, ac, W, ns, K):
+@PROJ
+def test_sampler_trajectory_bit_exact(oracle_lib, L, ar, ac, W, ns, K, proj):
     """GPU walkers and the oracle chain driven by the same Philox streams
     visit the same configurations; amplitudes agree to FP64 rounding.  K = 0
     is the reference's adaptive recompute period (make_sample.f90:74-128),
@@ -103,8 +120,7 @@ def test_sampler_trajectory_bit_exact(oracle_lib, L, ar, ac, W, ns, K):
     spec, p = _model(L, ar, ac, sigma_w=0.1)
     seed = 1234
     nwarm = 4
-    eng = api.Engine(spec, n_walkers=W, n_samples=ns, seed=seed, n_warmup=nwarm, recompute_k=K,
-                     params=p)
+    eng = _engine(api, spec, p, proj, n_walkers=W, n_samples=ns, seed=seed, n_warmup=nwarm, recompute_k=K)
     eng.make_sample(True, shift=0.0)
     x_gpu, psi_gpu = eng.get_samples()
     wk = eng.get_walkers()
@@ -135,13 +151,14 @@ def test_sampler_trajectory_bit_exact(oracle_lib, L, ar, ac, W, ns, K):
 
 
 @pytest.mark.parametrize("L,ar,ac", [(4, 2, 1), (6, 2, 0), (10, 4, 0)])
-def test_measure_s_and_sr_step(oracle_lib, L, ar, ac):
+@PROJ
+def test_measure_s_and_sr_step(oracle_lib, L, ar, ac, proj):
     api = _api()
     spec, p = _model(L, ar, ac)
     orc = oracle_lib.Oracle(spec, p)
     n = 96
     xs = _configs(spec.N, n, seed=300 + L)
-    eng = api.Engine(spec, n_walkers=1, n_samples=n, params=p)
+    eng = _engine(api, spec, p, proj, n_walkers=1, n_samples=n)
     eng.set_samples(xs)
     shift = 2e-3
     got = eng.calc_phys_quantity_opt(shift=shift)
