@@ -183,13 +183,16 @@ __global__ __launch_bounds__(BNT) void k_meas_blk(LocalArgs a) {
           const int s1 = aup ? i1 : i2, s2 = aup ? i2 : i1;
           const double* S1 = S + (size_t)s1 * 2 * N;
           const double* S2 = S + (size_t)s2 * 2 * N;
+          // c(u) from rows s1+N and s2+N: the off-diagonal blocks of the
+          // block form are exact negations of each other (one weight is zero)
+          const double* T1 = S + (size_t)(s1 + N) * 2 * N;
+          const double* T2 = S + (size_t)(s2 + N) * 2 * N;
           double rv = 0.0;
           cv[t] = 0.0;
           if (lane < np) {
-            const int sd = s.eidx[np + lane];
+            const int sd = s.eidx[np + lane], su = s.eidx[lane];
             rv = lane == db[t] ? S2[s1 + N] - S1[s2 + N] : S2[sd + N] - S1[sd + N];
-            const double* Su = S + (size_t)s.eidx[lane] * 2 * N;
-            cv[t] = lane == ua[t] ? 0.0 : Su[s1 + N] - Su[s2 + N];
+            cv[t] = lane == ua[t] ? 0.0 : T2[su] - T1[su];
           }
           if (lane < np) s.vr[wid][t][lane] = rv;
         }

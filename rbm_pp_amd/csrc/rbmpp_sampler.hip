@@ -36,8 +36,10 @@ __global__ void k_slater_elements(DevModel m, const double* __restrict__ f, doub
     S[(size_t)ri * two + rj + N] = slt_ij * cc + slt_ji * ss;
     S[(size_t)(ri + N) * two + rj] = -slt_ij * ss - slt_ji * cc;
     S[(size_t)(ri + N) * two + rj + N] = d * cs;
-    // the mirrored entries are written by the (rj, ri) thread; skew symmetry
-    // is exact by construction (negation commutes with rounding)
+    // the mirrored entries are written by the (rj, ri) thread.  The same-spin
+    // blocks are exact negations of their mirrors (d changes sign exactly);
+    // the off-diagonal ones are when cs = 0 (one of cc, ss is zero, so each
+    // entry is one rounded product), which the block-form ratios rely on
     (void)bad;
   }
 }
@@ -644,12 +646,18 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
           const int s1 = aup ? i1 : i2, s2 = aup ? i2 : i1;
           const double* S1 = Sq + (size_t)s1 * 2 * N;
           const double* S2 = Sq + (size_t)s2 * 2 * N;
+          // c(u) = Slater[s_u][s1+N] - Slater[s_u][s2+N] read from rows s1+N
+          // and s2+N: with cos sin = 0 one of the two weights of every
+          // off-diagonal entry is zero, so Slater[r+N][c] is the exact
+          // negation of Slater[c][r+N] and the difference is bit-identical,
+          // from two rows instead of one row per lane
+          const double* T1 = Sq + (size_t)(s1 + N) * 2 * N;
+          const double* T2 = Sq + (size_t)(s2 + N) * 2 * N;
           double rv = 0.0;
           if (lane < np) {
-            const int sd = s.eidx[np + lane];
+            const int sd = s.eidx[np + lane], su = s.eidx[lane];
             rv = lane == db ? S2[s1 + N] - S1[s2 + N] : S2[sd + N] - S1[sd + N];
-            const double* Su = Sq + (size_t)s.eidx[lane] * 2 * N;
-            bcv = lane == bua ? 0.0 : Su[s1 + N] - Su[s2 + N];
+            bcv = lane == bua ? 0.0 : T2[su] - T1[su];
           }
           double* rt = wv;  // r(d) for every d, zero past np
           rt[lane] = rv;
