@@ -255,10 +255,11 @@ def main():
     # the N proposals reads them once (8 C N^2/2 bytes) and an accepted one
     # (fraction A) reads and writes them once more: 4 C N^3 (1 + 2A).
     # In block form (normal projection: every X = [[0, F], [-F^T, 0]]) the
-    # sampler keeps F^-1, (N/2)^2 entries per copy: half of that,
-    # 2 C N^3 (1 + 2A).
+    # sampler keeps F^-1, (N/2)^2 entries per copy, and applies an accepted
+    # move's update inside the next proposal's read pass: each proposal reads
+    # them once and an accepted one writes them once, 2 C N^3 (1 + A).
     block = args.projection == "normal"
-    b_unit = (2.0 if block else 4.0) * Cc * N ** 3 * (1.0 + 2.0 * acc)
+    b_unit = 2.0 * Cc * N ** 3 * (1.0 + acc) if block else 4.0 * Cc * N ** 3 * (1.0 + 2.0 * acc)
     achieved = b_unit * sweeps_per_launch / t_sweep / 1e9
     # the same sweep priced with the reference's layout and passes (full
     # N x N X^-1; calculate_new_InvM_two reads it again to form its update

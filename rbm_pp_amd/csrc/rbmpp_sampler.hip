@@ -503,6 +503,39 @@ __device__ __attribute__((noinline)) void wg_measure_sample(const DevModel& m, S
 // out of the plain sampler so it does not weigh on its register allocation).
 // BLK: the copies are in block form (DevModel::block) and each wave keeps its
 // copy's F^-1 instead of the folded X^-1.
+// Block form, one wave: the rank-2 update of an accepted exchange applied to
+// F^-1 in place, B'[d][u] = B[d][u] + B[d][ua] al_u + x_d be_u with
+// x_d = B[d].c (old B).  Lane u holds al_u, be_u, c_u; ua is wave-uniform.
+// The sweep applies the same update inside the next ratio pass; this pass
+// serves the points where F^-1 must be current without a proposal (before a
+// checked recompute, the fused measurement and the end of the launch).
+__device__ __attribute__((noinline)) void blk_update_pass(double* Mq, int np, int lane, double cv, int ua,
+                                                          double al, double be) {
+  constexpr int UN = 8;
+  ua = __builtin_amdgcn_readfirstlane(ua);
+  const __amdgpu_buffer_rsrc_t rs = block_rsrc(Mq, np * np * 8);
+  const int vo = lane < np ? 8 * lane : RBMPP_OOB;
+  const int rowb = 8 * np;
+  int d = 0;
+  for (; d + UN <= np; d += UN) {
+    double g[UN];
+#pragma unroll
+    for (int u = 0; u < UN; u++) g[u] = bload1(rs, vo, (d + u) * rowb);
+#pragma unroll
+    for (int u = 0; u < UN; u++) {
+      const double xd = readlane_d(wave_sum_to63(g[u] * cv), 63);
+      const double cd = readlane_d(g[u], ua);
+      bstore1(fma(cd, al, fma(xd, be, g[u])), rs, vo, (d + u) * rowb);
+    }
+  }
+  for (; d < np; d++) {
+    const double g = bload1(rs, vo, d * rowb);
+    const double xd = readlane_d(wave_sum_to63(g * cv), 63);
+    const double cd = readlane_d(g, ua);
+    bstore1(fma(cd, al, fma(xd, be, g)), rs, vo, d * rowb);
+  }
+}
+
 template <int NMAX, bool LDSMAT, bool MEAS, bool BLK>
 __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
   constexpr int KD = NMAX >= 128 ? NMAX / 128 : 1;
@@ -565,11 +598,24 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
   double* Mq = Mw + (size_t)q * fold_size(N);
   const double* Sq = m.se + (size_t)q * 4 * NN;
   const int H = N >> 1;
+  // block form: an accepted exchange leaves its update of F^-1 pending
+  // (coefficients of lane u: pal, pbe, pcv; wave-uniform pua) and the next
+  // ratio pass applies it to each row as it reads it, so that an accept
+  // costs one store pass instead of a read-modify-write pass more.  A
+  // recompute without drift check drops it (it rebuilds F^-1 from x).
+  bool pend = false;
+  int pua = 0;
+  double pal = 0.0, pbe = 0.0, pcv = 0.0;
 
   for (int i0 = -a.nwarm; i0 <= a.nsamp && status == WS_OK; i0++) {
     // ---- periodic recompute (make_sample.f90:74-128)
     const int limit = a.fixed_k > 0 ? a.fixed_k : nfast;
     if (nacc_tmp > limit) {
+      if constexpr (BLK) {
+        // the drift check compares with the fast-updated F^-1
+        if (pend && q < C) blk_update_pass(Mq, m.npair, lane, pcv, pua, pal, pbe);
+        pend = false;
+      }
       double dpf = 0.0, dinv = 0.0;
       wg_recompute(m, s, Mw, gscr, 1, &dpf, &dinv);
       status = s.status;
@@ -589,6 +635,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
         __syncthreads();
         wg_x_to_ele(m, s);
         wg_recompute(m, s, Mw, gscr, 0, nullptr, nullptr);
+        pend = false;
         if (s.status) { status = s.status; break; }
         const double r1 = (cmod(s.psi) + a.shift) / (cmod(psi_old) + a.shift);
         if (!(rng.u01() < r1 * r1)) {
@@ -665,17 +712,42 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
           const __amdgpu_buffer_rsrc_t rs = block_rsrc(Mq, np * np * 8);
           const int vo = lane < np ? 8 * lane : RBMPP_OOB;
           const int rowb = 8 * np;
+          // one pass over the rows of F^-1: the pending update of the last
+          // accept first (stored back), then y += r(d) B[d]; row db is kept
           double y = 0.0;
+          brow = 0.0;
           int d = 0;
           for (; d + UN <= np; d += UN) {
             double g[UN];
 #pragma unroll
             for (int u = 0; u < UN; u++) g[u] = bload1(rs, vo, (d + u) * rowb);
+            if (pend) {
 #pragma unroll
-            for (int u = 0; u < UN; u++) y = fma(rt[d + u], g[u], y);
+              for (int u = 0; u < UN; u++) {
+                const double xd = readlane_d(wave_sum_to63(g[u] * pcv), 63);
+                const double cd = readlane_d(g[u], pua);
+                g[u] = fma(cd, pal, fma(xd, pbe, g[u]));
+                bstore1(g[u], rs, vo, (d + u) * rowb);
+              }
+            }
+#pragma unroll
+            for (int u = 0; u < UN; u++) {
+              y = fma(rt[d + u], g[u], y);
+              if (d + u == db) brow = g[u];
+            }
           }
-          for (; d < np; d++) y = fma(rt[d], bload1(rs, vo, d * rowb), y);
-          brow = bload1(rs, vo, db * rowb);
+          for (; d < np; d++) {
+            double g = bload1(rs, vo, d * rowb);
+            if (pend) {
+              const double xd = readlane_d(wave_sum_to63(g * pcv), 63);
+              const double cd = readlane_d(g, pua);
+              g = fma(cd, pal, fma(xd, pbe, g));
+              bstore1(g, rs, vo, d * rowb);
+            }
+            y = fma(rt[d], g, y);
+            if (d == db) brow = g;
+          }
+          pend = false;
           by = y;
           bgam = readlane_d(y, bua);
           bbet = readlane_d(brow, bua);
@@ -845,32 +917,14 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
             // Sherman-Morrison-Woodbury for F' = F + e_ua r^T + c e_db^T:
             // B'[d][u] = B[d][u] + B[d][ua] al_u + x_d be_u with x = B c (old B),
             // al = -((1 + B[db].c) y - (y.c) B[db]) / R,
-            // be = -((1 + y_ua) B[db] - B[db][ua] y) / R
-            const int np = m.npair;
+            // be = -((1 + y_ua) B[db] - B[db][ua] y) / R,
+            // left pending for the next pass over F^-1
             const double ia = -1.0 / rq;
-            const double al = ia * ((1.0 + bdel) * by - beps * brow);
-            const double be = ia * ((1.0 + bgam) * brow - bbet * by);
-            const __amdgpu_buffer_rsrc_t rs = block_rsrc(Mq, np * np * 8);
-            const int vo = lane < np ? 8 * lane : RBMPP_OOB;
-            const int rowb = 8 * np;
-            int d = 0;
-            for (; d + UN <= np; d += UN) {
-              double g[UN];
-#pragma unroll
-              for (int u = 0; u < UN; u++) g[u] = bload1(rs, vo, (d + u) * rowb);
-#pragma unroll
-              for (int u = 0; u < UN; u++) {
-                const double xd = readlane_d(wave_sum_to63(g[u] * bcv), 63);
-                const double cd = readlane_d(g[u], bua);
-                bstore1(fma(cd, al, fma(xd, be, g[u])), rs, vo, (d + u) * rowb);
-              }
-            }
-            for (; d < np; d++) {
-              const double g = bload1(rs, vo, d * rowb);
-              const double xd = readlane_d(wave_sum_to63(g * bcv), 63);
-              const double cd = readlane_d(g, bua);
-              bstore1(fma(cd, al, fma(xd, be, g)), rs, vo, d * rowb);
-            }
+            pal = ia * ((1.0 + bdel) * by - beps * brow);
+            pbe = ia * ((1.0 + bgam) * brow - bbet * by);
+            pcv = bcv;
+            pua = bua;
+            pend = true;
           } else {
           const double a_ = -pa, b_ = pb, c_ = qa, d_ = -qb;
           const double e_ = -bMa - vec_ba;
@@ -1019,6 +1073,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
           // a copy passed close to a singular configuration: its
           // fast-updated X^-1 would lose accuracy
           wg_recompute(m, s, Mw, gscr, 0, nullptr, nullptr);
+          pend = false;
           status = s.status;
           nrec++;
           nacc_tmp = 0;
@@ -1034,8 +1089,18 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
       const size_t row = (size_t)w * a.nsamp + (i0 - 1);
       for (int i = tid; i < N; i += nth) a.xs[row * N + i] = (signed char)s.x[i];
       if (tid == 0) a.psis[row] = s.psi;
-      if constexpr (LDSMAT && MEAS) wg_measure_sample(m, s, Mw, row, a);
+      if constexpr (LDSMAT && MEAS) {
+        if constexpr (BLK) {
+          if (pend && q < C) blk_update_pass(Mq, m.npair, lane, pcv, pua, pal, pbe);
+          pend = false;
+          __syncthreads();  // every copy's F^-1 is current
+        }
+        wg_measure_sample(m, s, Mw, row, a);
+      }
     }
+  }
+  if constexpr (BLK) {
+    if (pend && q < C) blk_update_pass(Mq, m.npair, lane, pcv, pua, pal, pbe);
   }
   __syncthreads();
   // ---- write the walker state back
