@@ -714,16 +714,17 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
           const int rowb = 8 * np;
           // one pass over the rows of F^-1: the pending update of the last
           // accept first (stored back), then y += r(d) B[d]; row db is kept
+          constexpr int UB = 25;  // rows in flight (N = 100: two groups)
           double y = 0.0;
           brow = 0.0;
           int d = 0;
-          for (; d + UN <= np; d += UN) {
-            double g[UN];
+          for (; d + UB <= np; d += UB) {
+            double g[UB];
 #pragma unroll
-            for (int u = 0; u < UN; u++) g[u] = bload1(rs, vo, (d + u) * rowb);
+            for (int u = 0; u < UB; u++) g[u] = bload1(rs, vo, (d + u) * rowb);
             if (pend) {
 #pragma unroll
-              for (int u = 0; u < UN; u++) {
+              for (int u = 0; u < UB; u++) {
                 const double xd = readlane_d(wave_sum_to63(g[u] * pcv), 63);
                 const double cd = readlane_d(g[u], pua);
                 g[u] = fma(cd, pal, fma(xd, pbe, g[u]));
@@ -731,7 +732,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
               }
             }
 #pragma unroll
-            for (int u = 0; u < UN; u++) {
+            for (int u = 0; u < UB; u++) {
               y = fma(rt[d + u], g[u], y);
               if (d + u == db) brow = g[u];
             }
