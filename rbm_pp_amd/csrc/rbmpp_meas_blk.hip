@@ -25,7 +25,7 @@ constexpr int BNT = 1024;                // threads per workgroup
 constexpr int BNW = BNT / RBMPP_WAVE;    // waves: one per projection copy
 constexpr int NMX = RBMPP_NLDS;          // largest N
 constexpr int HMX = rbmpp_block::HMX;    // largest np = N/2
-constexpr int BT = 2;                    // bonds per wave and pass
+constexpr int BT = 4;                    // bonds per wave and pass
 
 struct BlkSmem {
   double binv[HMX * HMX];        // F^-1 of the current copy: B[d][u] at d * np + u

@@ -140,6 +140,19 @@ def test_sampler_trajectory_bit_exact(oracle_lib, L, ar, ac, W, ns, K, proj):
         # after many fast updates: normwise over the copies of the walker
         assert _relerr(wk["pf"][w], out["pf"][-1]) < 1e-8
         assert wk["naccept"][w] == out["naccept"]
+    # the X^-1 each walker holds after the launch (in block form an accepted
+    # last proposal's update is applied at the end of the launch) against a
+    # recompute of its final configuration.  The walker keeps its own
+    # electron labels, a permutation of rbmpp_eval's site order applied to
+    # rows and columns alike, so the sorted magnitudes of each copy's entries
+    # are compared.
+    wi = eng.get_walkers(with_invm=True)
+    rec = eng.eval(wi["x"], with_invm=True)["invm"]
+    for w in range(W):
+        for q in range(rec.shape[1]):
+            a = np.sort(np.abs(wi["invm"][w, q].ravel()))
+            b = np.sort(np.abs(rec[w, q].ravel()))
+            assert _relerr(a, b) < 1e-6, f"walker {w} copy {q} X^-1"
     # a second call continues every chain from its last sample
     eng.make_sample(False, shift=1e-3)
     x2, _ = eng.get_samples()
