@@ -45,11 +45,13 @@ __device__ __forceinline__ double philox_u01(uint32_t k0, uint32_t k1, uint32_t 
 #pragma unroll
   for (int r = 0; r < 10; r++) {
     if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
-    uint64_t p0 = (uint64_t)0xD2511F53u * c0;
-    uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
-    uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
-    uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
-    c1 = (uint32_t)p1; c3 = (uint32_t)p0; c0 = n0; c2 = n2;
+    // high and low words of the 32 x 32 -> 64-bit products as two 32-bit
+    // multiplies, which the scalar unit has (a 64-bit product is vector only)
+    const uint32_t h0 = __umulhi(0xD2511F53u, c0), l0 = 0xD2511F53u * c0;
+    const uint32_t h1 = __umulhi(0xCD9E8D57u, c2), l1 = 0xCD9E8D57u * c2;
+    const uint32_t n0 = h1 ^ c1 ^ k0;
+    const uint32_t n2 = h0 ^ c3 ^ k1;
+    c1 = l1; c3 = l0; c0 = n0; c2 = n2;
   }
   uint64_t z = ((uint64_t)c1 << 32) | c0;
   return (double)(z >> 11) * 0x1.0p-53;

@@ -536,6 +536,16 @@ __device__ __attribute__((noinline)) void blk_update_pass(double* Mq, int np, in
   }
 }
 
+// Every thread of the sweep draws the same Philox numbers and reads the same
+// shared configuration, so these values are equal in all lanes.  Saying so
+// keeps the counter, the draws and the branches on them on the scalar unit
+// (a branch on a shared-memory load is otherwise taken as divergent, and
+// everything after it, the Philox rounds included, moves to vector registers).
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint64_t uni64(uint64_t v) {
+  return ((uint64_t)(uint32_t)uni((int)(uint32_t)(v >> 32)) << 32) | (uint32_t)uni((int)(uint32_t)v);
+}
+
 template <int NMAX, bool LDSMAT, bool MEAS, bool BLK>
 __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
   constexpr int KD = NMAX >= 128 ? NMAX / 128 : 1;
@@ -579,15 +589,24 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
   int status = WS_OK;
   // ---- set-up (make_sample.f90:50-62)
   if (a.first) {
-    status = wg_initial_x(m, s, rng, Mw, gscr);
+    // the out-of-line set-up draws from a copy, so that the sweep's own
+    // generator never has its address taken and stays in scalar registers
+    Rng r0 = rng;
+    status = uni(wg_initial_x(m, s, r0, Mw, gscr));
+    rng.ctr = uni64(r0.ctr);
   } else {
     for (int i = tid; i < N; i += nth) s.x[i] = a.ws.x[(size_t)w * N + i];
     __syncthreads();
     wg_x_to_ele(m, s);
     wg_recompute(m, s, Mw, gscr, 0, nullptr, nullptr);
-    status = s.status;
-    if (status == WS_OK && rng.u01() < a.p_restart) status = wg_initial_x(m, s, rng, Mw, gscr);
+    status = uni(s.status);
+    if (status == WS_OK && uni(rng.u01() < a.p_restart)) {
+      Rng r0 = rng;
+      status = uni(wg_initial_x(m, s, r0, Mw, gscr));
+      rng.ctr = uni64(r0.ctr);
+    }
   }
+  rng.ctr = uni64(rng.ctr);
   int nacc_tmp = 0;
   int nfast = a.ws.nfastup[w];
   long long naccept = 0;
@@ -618,32 +637,33 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
       }
       double dpf = 0.0, dinv = 0.0;
       wg_recompute(m, s, Mw, gscr, 1, &dpf, &dinv);
-      status = s.status;
+      status = uni(s.status);
       nrec++;
       nacc_tmp = 0;
       if (a.fixed_k <= 0) {
-        if (dinv > 1.0e-5 || dpf > 1.0e-5) nfast = max(1, (int)llround((double)nfast / 1.2));
+        if (uni(dinv > 1.0e-5 || dpf > 1.0e-5)) nfast = max(1, (int)llround((double)nfast / 1.2));
         else nfast = min(100, (int)llround((double)nfast * 1.1));
       }
       if (status) break;
     }
     for (int iu = 0; iu < a.nupdate; iu++) {
+      rng.ctr = uni64(rng.ctr);
       // ---- global spin flip (make_sample.f90:135-158)
-      if (rng.u01() < a.p_global) {
+      if (uni(rng.u01() < a.p_global)) {
         const double2 psi_old = s.psi;
         for (int i = tid; i < N; i += nth) s.x[i] = -s.x[i];
         __syncthreads();
         wg_x_to_ele(m, s);
         wg_recompute(m, s, Mw, gscr, 0, nullptr, nullptr);
         pend = false;
-        if (s.status) { status = s.status; break; }
+        if (uni(s.status)) { status = uni(s.status); break; }
         const double r1 = (cmod(s.psi) + a.shift) / (cmod(psi_old) + a.shift);
-        if (!(rng.u01() < r1 * r1)) {
+        if (!uni(rng.u01() < r1 * r1)) {
           for (int i = tid; i < N; i += nth) s.x[i] = -s.x[i];
           __syncthreads();
           wg_x_to_ele(m, s);
           wg_recompute(m, s, Mw, gscr, 0, nullptr, nullptr);
-          if (s.status) { status = s.status; break; }
+          if (uni(s.status)) { status = uni(s.status); break; }
         }
       }
       // ---- exchange candidate (make_sample.f90:244-294)
@@ -658,7 +678,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
           if (ix >= m.L) ix -= m.L;
           if (iy >= m.L) iy -= m.L;
           const int c2 = ix * m.L + iy;
-          if (s.x[c1] != s.x[c2]) { i1 = c1; i2 = c2; break; }
+          if (uni(s.x[c1] != s.x[c2])) { i1 = c1; i2 = c2; break; }
         }
       }
       if (i1 < 0) { status = WS_CANDIDATE; break; }
@@ -911,7 +931,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
       double2 psin = make_double2(0.0, 0.0);
       for (int qq = 0; qq < C; qq++) psin = cadd(psin, s.psiq[qq]);
       const double r1 = (cmod(psin) + a.shift) / (cmod(s.psi) + a.shift);
-      if (rng.u01() < r1 * r1) {
+      if (uni(rng.u01() < r1 * r1)) {
         // ---- accept: rank-4 update of this copy's X^-1 (calculate_new_InvM_two_child)
         if (q < C) {
           if constexpr (BLK) {
@@ -1070,12 +1090,12 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
         __syncthreads();
         if (i0 > 0) naccept++;
         nacc_tmp++;
-        if (s.guard) {
+        if (uni(s.guard)) {
           // a copy passed close to a singular configuration: its
           // fast-updated X^-1 would lose accuracy
           wg_recompute(m, s, Mw, gscr, 0, nullptr, nullptr);
           pend = false;
-          status = s.status;
+          status = uni(s.status);
           nrec++;
           nacc_tmp = 0;
           if (tid == 0) s.guard = 0;
