@@ -9,8 +9,12 @@
 // (pfupdate_two.f90:113-224) in one read-modify-write pass, reusing P and Q.
 // X^-1 is held in folded skew storage (N^2/2 FP64, rbmpp_device.h), so each
 // pass moves half the bytes of the full matrix and the walkers resident on
-// the chip keep their X^-1 within the last-level cache.  All threads draw the
-// same Philox numbers, so the control flow is uniform across the workgroup.
+// the chip keep their X^-1 within the last-level cache.  In block form
+// (normal projection, BLK) each wave keeps F^-1 of its copy ((N/2)^2 FP64),
+// the ratio is det F'/det F, and an accepted move's rank-2 update is applied
+// by the next proposal's ratio pass as it reads each row (one pass over F^-1
+// per proposal).  All threads draw the same Philox numbers, so the control
+// flow is uniform across the workgroup and runs on the scalar unit.
 #include "rbmpp_kernels.h"
 #include "rbmpp_measure.h"
 #include "rbmpp_block.h"
