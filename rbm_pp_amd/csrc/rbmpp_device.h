@@ -153,6 +153,43 @@ __device__ __forceinline__ double wave_sum_to63(double v) {
   v = dpp_add<0x143, 0xc>(v);
   return v;
 }
+// Four wave sums at once (gfx950 lane swaps, every lane active).
+// v_permlane32_swap(x, y) gives [x lanes 0-31 | y lanes 0-31] and
+// [x lanes 32-63 | y lanes 32-63]; v_permlane16_swap(x, y) gives
+// [x row 0 | y row 0 | x row 2 | y row 2] and the odd rows likewise (rows of
+// 16 lanes).  Adding each pair folds a and b into one register (a on lanes
+// 0-31, b on 32-63), c and d into another, then the four into rows 0-3 as
+// a, c, b, d; the row stages of wave_sum_to63 finish each row in its last
+// lane: lane 15 holds the sum of a, lane 47 that of b, lane 31 that of c,
+// lane 63 that of d.
+template <bool SIXTEEN>
+__device__ __forceinline__ double swap_add(double x, double y) {
+  const unsigned long long bx = __builtin_bit_cast(unsigned long long, x);
+  const unsigned long long by = __builtin_bit_cast(unsigned long long, y);
+  const unsigned xl = (unsigned)bx, xh = (unsigned)(bx >> 32), yl = (unsigned)by, yh = (unsigned)(by >> 32);
+  unsigned f0, f1, h0, h1;
+  if constexpr (SIXTEEN) {
+    const auto lo = __builtin_amdgcn_permlane16_swap(xl, yl, false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap(xh, yh, false, false);
+    f0 = lo[0]; f1 = lo[1]; h0 = hi[0]; h1 = hi[1];
+  } else {
+    const auto lo = __builtin_amdgcn_permlane32_swap(xl, yl, false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap(xh, yh, false, false);
+    f0 = lo[0]; f1 = lo[1]; h0 = hi[0]; h1 = hi[1];
+  }
+  const double first = __builtin_bit_cast(double, ((unsigned long long)h0 << 32) | f0);
+  const double second = __builtin_bit_cast(double, ((unsigned long long)h1 << 32) | f1);
+  return first + second;
+}
+__device__ __forceinline__ double wave_sum4(double a, double b, double c, double d) {
+  double v = swap_add<true>(swap_add<false>(a, b), swap_add<false>(c, d));
+  v = dpp_add<0x111, 0xf>(v);
+  v = dpp_add<0x112, 0xf>(v);
+  v = dpp_add<0x114, 0xf>(v);
+  v = dpp_add<0x118, 0xf>(v);
+  return v;
+}
+
 // the value of lane `l` (wave-uniform) in every lane
 __device__ __forceinline__ double readlane_d(double v, int l) {
   const long long b = __builtin_bit_cast(long long, v);

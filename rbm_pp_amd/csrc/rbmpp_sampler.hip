@@ -747,8 +747,21 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
 #pragma unroll
             for (int u = 0; u < UB; u++) g[u] = bload1(rs, vo, (d + u) * rowb);
             if (pend) {
+              // x_d = B[d].c four rows at a time (wave_sum4), the rest singly
+              constexpr int U4 = UB / 4 * 4;
 #pragma unroll
-              for (int u = 0; u < UB; u++) {
+              for (int u = 0; u < U4; u += 4) {
+                const double v = wave_sum4(g[u] * pcv, g[u + 1] * pcv, g[u + 2] * pcv, g[u + 3] * pcv);
+                const double xd[4] = {readlane_d(v, 15), readlane_d(v, 47), readlane_d(v, 31), readlane_d(v, 63)};
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                  const double cd = readlane_d(g[u + j], pua);
+                  g[u + j] = fma(cd, pal, fma(xd[j], pbe, g[u + j]));
+                  bstore1(g[u + j], rs, vo, (d + u + j) * rowb);
+                }
+              }
+#pragma unroll
+              for (int u = U4; u < UB; u++) {
                 const double xd = readlane_d(wave_sum_to63(g[u] * pcv), 63);
                 const double cd = readlane_d(g[u], pua);
                 g[u] = fma(cd, pal, fma(xd, pbe, g[u]));
