@@ -208,15 +208,23 @@ __global__ __launch_bounds__(BNT) void k_meas_blk(LocalArgs a) {
             for (int t = 0; t < BT; t++) y[t] = fma(s.vr[wid][t][d], bv, y[t]);
           }
         }
+        // B[db].c and y.c of the four bonds by two four-way wave sums
+        // (lanes 15, 47, 31, 63 hold bonds 0-3)
+        static_assert(BT == 4, "the four-way wave sums take four bonds");
+        double brow[BT];
+#pragma unroll
+        for (int t = 0; t < BT; t++) brow[t] = lane < np ? s.binv[db[t] * np + lane] : 0.0;
+        const double dv = wave_sum4(brow[0] * cv[0], brow[1] * cv[1], brow[2] * cv[2], brow[3] * cv[3]);
+        const double ev = wave_sum4(y[0] * cv[0], y[1] * cv[1], y[2] * cv[2], y[3] * cv[3]);
+        constexpr int SUMLANE[BT] = {15, 47, 31, 63};
 #pragma unroll
         for (int t = 0; t < BT; t++) {
           if (BT * p + t >= nb) break;  // wave-uniform
-          const double brow = lane < np ? s.binv[db[t] * np + lane] : 0.0;
           const double gam = readlane_d(y[t], ua[t]);
-          const double bet = readlane_d(brow, ua[t]);
-          const double del = wave_sum_to63(brow * cv[t]);
-          const double eps = wave_sum_to63(y[t] * cv[t]);
-          if (lane == 63) s.rat[bb[t]] = (1.0 + gam) * (1.0 + del) - eps * bet;
+          const double bet = readlane_d(brow[t], ua[t]);
+          const double del = readlane_d(dv, SUMLANE[t]);
+          const double eps = readlane_d(ev, SUMLANE[t]);
+          if (lane == 0) s.rat[bb[t]] = (1.0 + gam) * (1.0 + del) - eps * bet;
         }
         wave_sync();  // vr is rewritten by the next pass
       }
