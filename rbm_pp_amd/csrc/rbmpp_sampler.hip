@@ -737,10 +737,11 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
           const int vo = lane < np ? 8 * lane : RBMPP_OOB;
           const int rowb = 8 * np;
           // one pass over the rows of F^-1: the pending update of the last
-          // accept first (stored back), then y += r(d) B[d]; row db is kept
+          // accept first (stored back), then y += r(d) B[d]; row db is read
+          // again after the pass (a reload is cheaper than selecting it from
+          // every row)
           constexpr int UB = 25;  // rows in flight (N = 100: two groups)
           double y = 0.0;
-          brow = 0.0;
           int d = 0;
           for (; d + UB <= np; d += UB) {
             double g[UB];
@@ -769,10 +770,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
               }
             }
 #pragma unroll
-            for (int u = 0; u < UB; u++) {
-              y = fma(rt[d + u], g[u], y);
-              if (d + u == db) brow = g[u];
-            }
+            for (int u = 0; u < UB; u++) y = fma(rt[d + u], g[u], y);
           }
           for (; d < np; d++) {
             double g = bload1(rs, vo, d * rowb);
@@ -783,8 +781,9 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
               bstore1(g, rs, vo, d * rowb);
             }
             y = fma(rt[d], g, y);
-            if (d == db) brow = g;
           }
+          // row db as updated above (each lane reads back its own store)
+          brow = bload1(rs, vo, db * rowb);
           pend = false;
           by = y;
           bgam = readlane_d(y, bua);
