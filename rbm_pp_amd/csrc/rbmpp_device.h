@@ -190,6 +190,19 @@ __device__ __forceinline__ double wave_sum4(double a, double b, double c, double
   return v;
 }
 
+// Two wave sums at once: the 32-lane swap puts a on lanes 0-31 and b on
+// 32-63, the row stages and the row_bcast:15 stage of wave_sum_to63 finish
+// them: lane 31 holds the sum of a, lane 63 that of b.
+__device__ __forceinline__ double wave_sum2(double a, double b) {
+  double v = swap_add<false>(a, b);
+  v = dpp_add<0x111, 0xf>(v);
+  v = dpp_add<0x112, 0xf>(v);
+  v = dpp_add<0x114, 0xf>(v);
+  v = dpp_add<0x118, 0xf>(v);
+  v = dpp_add<0x142, 0xa>(v);
+  return v;
+}
+
 // the value of lane `l` (wave-uniform) in every lane
 __device__ __forceinline__ double readlane_d(double v, int l) {
   const long long b = __builtin_bit_cast(long long, v);

@@ -788,8 +788,11 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
           by = y;
           bgam = readlane_d(y, bua);
           bbet = readlane_d(brow, bua);
-          bdel = readlane_d(wave_sum_to63(brow * bcv), 63);
-          beps = readlane_d(wave_sum_to63(y * bcv), 63);
+          {
+            const double v = wave_sum2(brow * bcv, y * bcv);
+            bdel = readlane_d(v, 31);
+            beps = readlane_d(v, 63);
+          }
           const double ratio = (1.0 + bgam) * (1.0 + bdel) - beps * bbet;
           rq = ratio;
           const double pfn = ratio * s.pf[q];
