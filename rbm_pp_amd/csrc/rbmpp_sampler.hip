@@ -103,7 +103,7 @@ template <int NMAX, bool LDSMAT>
 __device__ __attribute__((noinline)) void wg_recompute(const DevModel& m, SweepSmem<NMAX, LDSMAT>& s, double* Mw,
                              double* gscratch, int check, double* drift_pf, double* drift_inv) {
   const int tid = threadIdx.x, nth = blockDim.x;
-  const int N = m.N, C = m.C, P = m.P;
+  const int N = m.N, C = m.C;
   wg_theta_sign(m, s, check);
   // Pf and X^-1 of each copy: with 16 waves and N <= 112 the register-tile
   // Gauss-Jordan (rbmpp_skew.h), otherwise row-swapping Gauss-Jordan on a
