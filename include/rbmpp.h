@@ -193,6 +193,10 @@ int rbmpp_get_measured(const rbmpp_handle* h, double* psi, double* eloc, double*
  * (optional) InvM in the reference's layout invm[w][q][(j)*N+(i)] = InvM(i+1,j+1,q+1). */
 int rbmpp_get_walkers(const rbmpp_handle* h, int32_t* x, double* pf, double* psi, double* invm,
                       int64_t* naccept_per_walker);
+/* Ele_Idx of every walker after make_sample: ele_idx[w*N + i] = 1-based site
+ * of electron i+1 (up electrons first, then down; interface_for_PP.f90:5-40).
+ * The walker's InvM of rbmpp_get_walkers is indexed by these electrons. */
+int rbmpp_get_walker_electrons(const rbmpp_handle* h, int32_t* ele_idx);
 /* Full evaluation (calc_theta, calculate_MAll, sign_PP, calc_amplitude_RBM_PP)
  * of n configurations: psi[s][0..C] complex, pf[s][C], invm optional. */
 int rbmpp_eval(rbmpp_handle* h, int32_t n, const int32_t* x, double* psi, double* pf,

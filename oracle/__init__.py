@@ -228,9 +228,12 @@ class Oracle:
                     invm=invm.reshape(Cc, N, N).transpose(0, 2, 1),  # [q, i, j] = InvM(i,j,q)
                     psi_wo=pwo.view(np.complex128), psi=psi.view(np.complex128))
 
-    def recompute(self):
+    def recompute(self, state=None):
+        """Full recompute of Pf and X^-1 in the state's electron order;
+        returns the drift {max |dPf| / <|Pf|>, max |dInvM| / max |InvM|} of the
+        fast-updated values it replaces."""
         drift = np.zeros(2)
-        rc = lib().om_recompute(self.h, self.st, drift)
+        rc = lib().om_recompute(self.h, self.st if state is None else state, drift)
         if rc:
             raise RuntimeError(f"om_recompute: {rc}")
         return drift

@@ -80,6 +80,7 @@ def lib():
         L.rbmpp_get_measured.argtypes = [vp, dp, dp, dp]
         L.rbmpp_n_samples.restype = C.c_int64; L.rbmpp_n_samples.argtypes = [vp]
         L.rbmpp_get_walkers.argtypes = [vp, ip, dp, dp, dp, C.POINTER(C.c_int64)]
+        L.rbmpp_get_walker_electrons.argtypes = [vp, ip]
         L.rbmpp_eval.argtypes = [vp, C.c_int32, ip, dp, dp, dp]
         L.rbmpp_ratio_exchange.argtypes = [vp, C.c_int32, ip, ip, ip, dp, dp]
         L.rbmpp_local.argtypes = [vp, C.c_int32, ip, dp, dp, dp]
@@ -237,6 +238,12 @@ class Engine:
         if with_invm:
             out["invm"] = inv.reshape(W, Cc, N, N).transpose(0, 1, 3, 2)  # [w,q,i,j] = InvM(i,j)
         return out
+
+    def walker_electrons(self) -> np.ndarray:
+        """Ele_Idx of every walker, 0-based sites: [w, i] = site of electron i."""
+        e = np.zeros((self.W, self.N), np.int32)
+        self._check(lib().rbmpp_get_walker_electrons(self.h, _i(e)))
+        return e - 1
 
     def eval(self, xs, with_invm: bool = False):
         xs = np.ascontiguousarray(np.atleast_2d(xs), np.int32)

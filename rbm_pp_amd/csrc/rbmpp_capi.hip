@@ -390,15 +390,33 @@ int rbmpp_make_sample(rbmpp_handle* h, int32_t first_iteration, double shift, in
   return RBMPP_OK;
 }
 
-// (re)allocate the measurement buffers for n configurations
+// frees a buffer made by dalloc
+static void dfree(rbmpp_handle* h, void* p) {
+  if (!p) return;
+  auto it = std::find(h->allocs.begin(), h->allocs.end(), p);
+  if (it != h->allocs.end()) h->allocs.erase(it);
+  hipFree(p);
+}
+
+// (re)allocate the measurement buffers for n configurations: all or nothing,
+// so a failed request leaves the previous buffers in place
 static int ensure_measure(rbmpp_handle* h, size_t n) {
   if (n <= h->meas_cap && h->d_mpsi) return 0;
   int rc;
   const size_t rows = h->real_only ? n : 2 * n;
-  if ((rc = dalloc(h, &h->d_mpsi, n)) || (rc = dalloc(h, &h->d_eloc, n)) || (rc = dalloc(h, &h->d_rw, n)) ||
-      (rc = dalloc(h, &h->d_orows, rows * h->nv)) || (rc = dalloc(h, &h->d_kappa, rows)) ||
-      (rc = dalloc(h, &h->d_status, n)))
+  double2 *mpsi = nullptr, *eloc = nullptr;
+  double *rw = nullptr, *orows = nullptr, *kappa = nullptr;
+  int* status = nullptr;
+  if ((rc = dalloc(h, &mpsi, n)) || (rc = dalloc(h, &eloc, n)) || (rc = dalloc(h, &rw, n)) ||
+      (rc = dalloc(h, &orows, rows * h->nv)) || (rc = dalloc(h, &kappa, rows)) || (rc = dalloc(h, &status, n))) {
+    for (void* q : {(void*)mpsi, (void*)eloc, (void*)rw, (void*)orows, (void*)kappa, (void*)status}) dfree(h, q);
+    (void)hipGetLastError();
     return rc;
+  }
+  for (void* q : {(void*)h->d_mpsi, (void*)h->d_eloc, (void*)h->d_rw, (void*)h->d_orows, (void*)h->d_kappa,
+                  (void*)h->d_status})
+    dfree(h, q);
+  h->d_mpsi = mpsi; h->d_eloc = eloc; h->d_rw = rw; h->d_orows = orows; h->d_kappa = kappa; h->d_status = status;
   h->meas_cap = n;
   // the local kernel is grid-strided; N > RBMPP_NLDS needs one N x N
   // recompute scratch per block, sized once for the largest grid
@@ -765,6 +783,22 @@ int rbmpp_sr_step(rbmpp_handle* h, const rbmpp_sr_config* c, double* gamma, rbmp
 }
 
 // ---------------------------------------------------------- test entries
+// device temporaries of one call, freed by free_tmp; a failed allocation
+// returns RBMPP_E_NOMEM before anything is launched on the buffer
+extern "C++" template <typename T>
+static int tmp_alloc(rbmpp_handle* h, T** p, size_t n, std::vector<void*>& tmp) {
+  *p = nullptr;
+  hipError_t e = hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();  // clear the sticky error of the failed request
+    return fail(h, RBMPP_E_NOMEM, std::string("hipMalloc (") + std::to_string(n * sizeof(T)) +
+                                      " bytes): " + hipGetErrorString(e));
+  }
+  tmp.push_back((void*)*p);
+  return 0;
+}
+
 static int upload_x(rbmpp_handle* h, int n, const int32_t* x, signed char** d_out, std::vector<void*>& tmp) {
   std::vector<signed char> xb((size_t)n * h->N);
   for (size_t k = 0; k < xb.size(); k++) {
@@ -777,8 +811,8 @@ static int upload_x(rbmpp_handle* h, int n, const int32_t* x, signed char** d_ou
     if (up != h->npair) return fail(h, RBMPP_E_ARG, "configuration not in the Sz=0 sector");
   }
   signed char* d = nullptr;
-  HIPCHK(h, hipMalloc((void**)&d, xb.size()));
-  tmp.push_back(d);
+  int rc = tmp_alloc(h, &d, xb.size(), tmp);
+  if (rc) return rc;
   HIPCHK(h, hipMemcpyAsync(d, xb.data(), xb.size(), hipMemcpyHostToDevice, h->st));
   *d_out = d;
   return 0;
@@ -787,6 +821,13 @@ static int upload_x(rbmpp_handle* h, int n, const int32_t* x, signed char** d_ou
 static void free_tmp(std::vector<void*>& tmp) {
   for (void* p : tmp) hipFree(p);
   tmp.clear();
+}
+
+// synchronous device-to-host copy with the status mapped to RBMPP_E_HIP
+static int copy_out(rbmpp_handle* h, void* dst, const void* src, size_t bytes) {
+  HIPCHK(h, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  return 0;
 }
 
 int rbmpp_set_samples(rbmpp_handle* h, int32_t n, const int32_t* x) {
@@ -881,6 +922,18 @@ int rbmpp_get_walkers(const rbmpp_handle* h0, int32_t* x, double* pf, double* ps
   return RBMPP_OK;
 }
 
+int rbmpp_get_walker_electrons(const rbmpp_handle* h0, int32_t* ele_idx) {
+  rbmpp_handle* h = const_cast<rbmpp_handle*>(h0);
+  if (!h || !ele_idx) return RBMPP_E_ARG;
+  if (!h->walkers_ready) return fail(h, RBMPP_E_STATE, "get_walker_electrons: walkers not initialised");
+  const size_t n = (size_t)h->W * h->N;
+  std::vector<int> e(n);
+  int rc = copy_out(h, e.data(), h->ws.eidx, sizeof(int) * n);
+  if (rc) return rc;
+  for (size_t k = 0; k < n; k++) ele_idx[k] = e[k] + 1;  // 1-based sites, as Ele_Idx
+  return RBMPP_OK;
+}
+
 int rbmpp_eval(rbmpp_handle* h, int32_t n, const int32_t* x, double* psi, double* pf, double* invm) {
   if (!h || !x || n <= 0) return RBMPP_E_ARG;
   std::vector<void*> tmp;
@@ -889,14 +942,20 @@ int rbmpp_eval(rbmpp_handle* h, int32_t n, const int32_t* x, double* psi, double
   double *d_pf = nullptr, *d_inv = nullptr;
   double2* d_pq = nullptr;
   const size_t C = h->C, NN = (size_t)h->N * h->N;
-  if (!rc && hipMalloc((void**)&d_pf, sizeof(double) * n * C) == hipSuccess) tmp.push_back(d_pf);
-  if (!rc && hipMalloc((void**)&d_pq, sizeof(double2) * n * C) == hipSuccess) tmp.push_back(d_pq);
-  if (!rc && invm && hipMalloc((void**)&d_inv, sizeof(double) * n * C * NN) == hipSuccess) tmp.push_back(d_inv);
+  if (!rc) rc = tmp_alloc(h, &d_pf, (size_t)n * C, tmp);
+  if (!rc) rc = tmp_alloc(h, &d_pq, (size_t)n * C, tmp);
+  if (!rc && invm) rc = tmp_alloc(h, &d_inv, (size_t)n * C * NN, tmp);
   if (!rc) rc = run_local(h, n, d_x, invm ? 4 : 0, -1.0, nullptr, nullptr, d_inv, d_pf, d_pq, nullptr, nullptr);
+  std::vector<double2> p0, pq;
   if (!rc) {
-    std::vector<double2> p0(n), pq(n * C);
-    hipMemcpy(p0.data(), h->d_mpsi, sizeof(double2) * n, hipMemcpyDeviceToHost);
-    hipMemcpy(pq.data(), d_pq, sizeof(double2) * n * C, hipMemcpyDeviceToHost);
+    p0.resize(n);
+    pq.resize((size_t)n * C);
+    rc = copy_out(h, p0.data(), h->d_mpsi, sizeof(double2) * n);
+    if (!rc) rc = copy_out(h, pq.data(), d_pq, sizeof(double2) * n * C);
+    if (!rc && pf) rc = copy_out(h, pf, d_pf, sizeof(double) * n * C);
+    if (!rc && invm) rc = copy_out(h, invm, d_inv, sizeof(double) * n * C * NN);
+  }
+  if (!rc) {
     if (psi)
       for (int s = 0; s < n; s++) {
         psi[(size_t)s * 2 * (C + 1)] = p0[s].x;
@@ -906,8 +965,6 @@ int rbmpp_eval(rbmpp_handle* h, int32_t n, const int32_t* x, double* psi, double
           psi[(size_t)s * 2 * (C + 1) + 2 * (q + 1) + 1] = pq[s * C + q].y;
         }
       }
-    if (pf) hipMemcpy(pf, d_pf, sizeof(double) * n * C, hipMemcpyDeviceToHost);
-    if (invm) hipMemcpy(invm, d_inv, sizeof(double) * n * C * NN, hipMemcpyDeviceToHost);
   }
   free_tmp(tmp);
   return rc;
@@ -929,19 +986,17 @@ int rbmpp_ratio_exchange(rbmpp_handle* h, int32_t n, const int32_t* x, const int
   double* d_pf = nullptr;
   double2* d_ps = nullptr;
   const size_t C = h->C;
-  if (!rc) {
-    hipMalloc((void**)&d1, sizeof(int) * n); tmp.push_back(d1);
-    hipMalloc((void**)&d2, sizeof(int) * n); tmp.push_back(d2);
-    hipMalloc((void**)&d_pf, sizeof(double) * n * C); tmp.push_back(d_pf);
-    hipMalloc((void**)&d_ps, sizeof(double2) * n); tmp.push_back(d_ps);
-    hipMemcpyAsync(d1, i1, sizeof(int) * n, hipMemcpyHostToDevice, h->st);
-    hipMemcpyAsync(d2, i2, sizeof(int) * n, hipMemcpyHostToDevice, h->st);
-    rc = run_local(h, n, d_x, 8, -1.0, d1, d2, nullptr, nullptr, nullptr, d_ps, d_pf);
-  }
-  if (!rc) {
-    if (pf_new) hipMemcpy(pf_new, d_pf, sizeof(double) * n * C, hipMemcpyDeviceToHost);
-    if (psi_new) hipMemcpy(psi_new, d_ps, sizeof(double2) * n, hipMemcpyDeviceToHost);
-  }
+  if (!rc) rc = tmp_alloc(h, &d1, (size_t)n, tmp);
+  if (!rc) rc = tmp_alloc(h, &d2, (size_t)n, tmp);
+  if (!rc) rc = tmp_alloc(h, &d_pf, (size_t)n * C, tmp);
+  if (!rc) rc = tmp_alloc(h, &d_ps, (size_t)n, tmp);
+  if (!rc && hipMemcpyAsync(d1, i1, sizeof(int) * n, hipMemcpyHostToDevice, h->st) != hipSuccess)
+    rc = fail(h, RBMPP_E_HIP, "ratio_exchange: copy of i1");
+  if (!rc && hipMemcpyAsync(d2, i2, sizeof(int) * n, hipMemcpyHostToDevice, h->st) != hipSuccess)
+    rc = fail(h, RBMPP_E_HIP, "ratio_exchange: copy of i2");
+  if (!rc) rc = run_local(h, n, d_x, 8, -1.0, d1, d2, nullptr, nullptr, nullptr, d_ps, d_pf);
+  if (!rc && pf_new) rc = copy_out(h, pf_new, d_pf, sizeof(double) * n * C);
+  if (!rc && psi_new) rc = copy_out(h, psi_new, d_ps, sizeof(double2) * n);
   free_tmp(tmp);
   return rc;
 }
@@ -963,19 +1018,18 @@ int rbmpp_local(rbmpp_handle* h, int32_t n, const int32_t* x, double* eloc, doub
                    nullptr);
     h->real_only = saved;
   }
-  if (!rc) {
+  if (!rc && eloc) rc = copy_out(h, eloc, h->d_eloc, sizeof(double2) * n);
+  if (!rc && psi) rc = copy_out(h, psi, h->d_mpsi, sizeof(double2) * n);
+  if (!rc && oloc) {
     const size_t nv = h->nv;
-    if (eloc) hipMemcpy(eloc, h->d_eloc, sizeof(double2) * n, hipMemcpyDeviceToHost);
-    if (psi) hipMemcpy(psi, h->d_mpsi, sizeof(double2) * n, hipMemcpyDeviceToHost);
-    if (oloc) {
-      std::vector<double> rows(2 * nv * n);
-      hipMemcpy(rows.data(), h->d_orows, sizeof(double) * rows.size(), hipMemcpyDeviceToHost);
+    std::vector<double> rows(2 * nv * n);
+    rc = copy_out(h, rows.data(), h->d_orows, sizeof(double) * rows.size());
+    if (!rc)
       for (int s = 0; s < n; s++)
         for (size_t k = 0; k < nv; k++) {
           oloc[((size_t)s * nv + k) * 2] = rows[(2 * (size_t)s) * nv + k];
           oloc[((size_t)s * nv + k) * 2 + 1] = rows[(2 * (size_t)s + 1) * nv + k];
         }
-    }
   }
   h->measured = 0;
   free_tmp(tmp);

@@ -52,7 +52,7 @@ module rbmpp_mod
   public :: rbmpp_make_sample, rbmpp_measure, rbmpp_sr_step, rbmpp_set_s_precision
   public :: rbmpp_measure_energy, rbmpp_measure_correlation
   public :: rbmpp_set_samples, rbmpp_get_samples, rbmpp_n_samples
-  public :: rbmpp_get_walkers, rbmpp_eval, rbmpp_ratio_exchange, rbmpp_local
+  public :: rbmpp_get_walkers, rbmpp_get_walker_electrons, rbmpp_eval, rbmpp_ratio_exchange, rbmpp_local
   public :: rbmpp_get_smat, rbmpp_timings, rbmpp_get_measured
   public :: rbmpp_error_message
 
@@ -189,6 +189,13 @@ module rbmpp_mod
       integer(c_int32_t), intent(out) :: x(*)
       real(c_double), intent(out) :: pf(*), psi(*), invm(*)
       integer(c_int64_t), intent(out) :: naccept(*)
+    end function
+
+    integer(c_int) function rbmpp_get_walker_electrons(h, ele_idx) &
+        bind(C, name="rbmpp_get_walker_electrons")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), intent(out) :: ele_idx(*)
     end function
 
     integer(c_int) function rbmpp_eval(h, n, x, psi, pf, invm) bind(C, name="rbmpp_eval")

@@ -142,17 +142,23 @@ def test_sampler_trajectory_bit_exact(oracle_lib, L, ar, ac, W, ns, K, proj):
         assert wk["naccept"][w] == out["naccept"]
     # the X^-1 each walker holds after the launch (in block form an accepted
     # last proposal's update is applied at the end of the launch) against a
-    # recompute of its final configuration.  The walker keeps its own
-    # electron labels, a permutation of rbmpp_eval's site order applied to
-    # rows and columns alike, so the sorted magnitudes of each copy's entries
-    # are compared.
+    # recompute of its final configuration, entry by entry.  The walker keeps
+    # its own electron labels (Ele_Idx); rbmpp_eval numbers the electrons of
+    # each spin in site order, so electron i of the walker is electron
+    # rank[i] of the recompute and InvM_walker(i, j) = InvM_eval(rank[i], rank[j]).
     wi = eng.get_walkers(with_invm=True)
+    ele = eng.walker_electrons()
     rec = eng.eval(wi["x"], with_invm=True)["invm"]
+    H = spec.N // 2
     for w in range(W):
+        e = ele[w]
+        assert (wi["x"][w][e[:H]] == 1).all() and (wi["x"][w][e[H:]] == -1).all()
+        rank = np.empty(spec.N, np.int64)
+        rank[:H] = np.argsort(np.argsort(e[:H]))
+        rank[H:] = H + np.argsort(np.argsort(e[H:]))
         for q in range(rec.shape[1]):
-            a = np.sort(np.abs(wi["invm"][w, q].ravel()))
-            b = np.sort(np.abs(rec[w, q].ravel()))
-            assert _relerr(a, b) < 1e-6, f"walker {w} copy {q} X^-1"
+            want = rec[w, q][np.ix_(rank, rank)]
+            assert _relerr(wi["invm"][w, q], want) < 1e-8, f"walker {w} copy {q} X^-1"
     # a second call continues every chain from its last sample
     eng.make_sample(False, shift=1e-3)
     x2, _ = eng.get_samples()
@@ -196,6 +202,49 @@ def test_measure_s_and_sr_step(oracle_lib, L, ar, ac, proj):
     assert rep["lambda_"] == pytest.approx(rep_ref[0], rel=1e-8)
     assert _relerr(gam - gam0, gam_ref - gam0) < 1e-8
     np.testing.assert_allclose(eng.gamma(), gam, rtol=0, atol=0)
+
+
+@pytest.mark.parametrize("max_change,clipped", [(1e-4, True), (0.04, False)])
+def test_sr_step_clip_branch_10x10(oracle_lib, max_change, clipped):
+    """Both branches of the max-change clip (SR_pdposv.f90:329-354) on the
+    10x10 workload at delta_tau = 0.05 (delW_max = 1.0e-3 here, so
+    max_change = 1e-4 clips): the HIP step, delW_max and lambda against the
+    oracle.
+    The two solve the same shifted system in different orders, so the bound
+    is the system's condition number times a few units of rounding:
+    tol = max(1e-10, 50 eps cond(A)), with A the shifted matrix of the
+    oracle's own S (stated in the assertion message)."""
+    api = _api()
+    spec, p = _model(10, 4, 0)
+    orc = oracle_lib.Oracle(spec, p)
+    n = 96
+    xs = _configs(spec.N, n, seed=910)
+    eng = api.Engine(spec, params=p, n_walkers=1, n_samples=n)
+    eng.set_samples(xs)
+    shift = 2e-3
+    eng.calc_phys_quantity_opt(shift=shift)
+    ref = orc.measure(xs, shift=shift, want_s=True)
+    norm = ref["norm"]
+    gam0 = orc.gamma()
+    rc, gam_ref, rep_ref, x_ref = orc.sr_step(ref["Etot"] / norm, norm, ref["gvec"], ref["ovec"], ref["smat"],
+                                              gam0, s_factor0=1e-3, delta_tau=0.05, max_change=max_change)
+    assert rc == 0
+    assert (rep_ref[1] > max_change) == clipped
+    gam, rep = eng.sr_step(gam0, s_factor0=1e-3, delta_tau=0.05, max_change=max_change, refine_iters=1)
+    # condition of the scaled, shifted system (SR_pdposv.f90:18-170)
+    ob = ref["ovec"] / norm
+    S = ref["smat"] / norm - np.outer(ob, ob)
+    nvr = spec.nv_rbm
+    D = np.ones(spec.nv); D[nvr:] = rep_ref[2]
+    A = S * np.outer(D, D)
+    A[np.diag_indices(spec.nv)] += rep_ref[3] * rep_ref[4]
+    tol = max(1e-10, 50 * np.finfo(float).eps * np.linalg.cond(A))
+    assert rep["delw_max"] == pytest.approx(rep_ref[1], rel=tol), tol
+    assert rep["lambda_"] == pytest.approx(rep_ref[0], rel=tol), tol
+    assert _relerr(gam - gam0, gam_ref - gam0) < tol, tol
+    if clipped:
+        dg = (gam - gam0)[:nvr]
+        assert np.abs(dg).max() <= max_change * (1 + 1e-12)
 
 
 @pytest.mark.parametrize("L,ar,ac", [(4, 2, 1), (6, 2, 0), (10, 4, 0)])

@@ -337,20 +337,90 @@ def test_measure_and_sr_invariants(oracle_lib):
     # mean |f| renormalised to 0.25 (SR_pdposv.f90:414)
     assert np.mean(np.abs(gam_new[spec.nv_rbm:])) == pytest.approx(0.25, abs=1e-12)
     # the solution solves the shifted, rescaled system: rebuild it independently
+    x_np, lam = _sr_solution_numpy(spec, m, 1e-3)
+    assert rep[0] == pytest.approx(lam, rel=1e-9)
+    np.testing.assert_allclose(x, x_np, rtol=1e-8, atol=1e-12 * np.abs(x_np).max())
+
+
+def _sr_solution_numpy(spec, m, s_factor0):
+    """SR_pdposv.f90:4-323 rebuilt with numpy: centred S and g, PP block
+    scaling by sqrt(mean positive RBM diagonal / mean positive PP diagonal),
+    relative diagonal shift max(1e-4, s_factor0) of the mean positive
+    diagonal, dense solve, trust region lambda = sqrt(x^T A x) 5/L, scaling
+    undone.  Returns (x, lambda)."""
+    nv, nvr, norm = spec.nv, spec.nv_rbm, m["norm"]
+    ovec = m["ovec"] / norm
+    etot = m["Etot"] / norm
+    S = m["smat"] / norm - np.outer(ovec, ovec)
     g = m["gvec"] / norm - 2 * etot * ovec
-    nvr = spec.nv_rbm
     d = np.diag(S)
     scal = np.sqrt(d[:nvr][d[:nvr] > 0].mean() / d[nvr:][d[nvr:] > 0].mean())
     D = np.ones(nv); D[nvr:] = scal
     Ss = S * np.outer(D, D); gs = g * D
     Sav = np.diag(Ss)[np.diag(Ss) > 0].mean()
-    Ss[np.diag_indices(nv)] += Sav * 1e-3
+    Ss[np.diag_indices(nv)] += Sav * max(1e-4, s_factor0)
     xs_ = np.linalg.solve(Ss, gs)
     lam = np.sqrt(xs_ @ Ss @ xs_) * 5 / spec.L
-    assert rep[0] == pytest.approx(lam, rel=1e-9)
     if lam > 1:
         xs_ = xs_ / lam
-    np.testing.assert_allclose(x, xs_ * D, rtol=1e-8, atol=1e-12 * np.abs(xs_).max())
+    return xs_ * D, lam
+
+
+def _sr_update_numpy(gam0, x, delta_tau, max_change, nsl, slater_abs_av=0.25):
+    """SR_pdposv.f90:329-366 rebuilt from the solution x: the max-change clip
+    scales the whole step so that its largest component is max_change, then
+    the pair amplitudes are renormalised to mean |f| = slater_abs_av."""
+    step = delta_tau * np.asarray(x)
+    dmax = np.abs(step).max()
+    if dmax > max_change:
+        step = step * (max_change / dmax)
+    gam = np.asarray(gam0) - step
+    f = gam[len(gam) - nsl:]
+    gam[len(gam) - nsl:] = f * (slater_abs_av / np.abs(f).mean())
+    return gam, dmax, step
+
+
+@pytest.mark.parametrize("delta_tau,clipped", [(1.0, True), (0.5, True), (1e-4, False)])
+def test_sr_step_clip_and_gamma_update(oracle_lib, delta_tau, clipped):
+    """Both branches of the max-change clip of SR_pdposv.f90:329-340 and the
+    f renormalisation of :342-354, against an independent numpy rebuild from
+    the solution x (which test_measure_and_sr_invariants pins by a numpy
+    solve of the scaled, shifted system)."""
+    spec, p = _model(L=4, alphar=2, alphac=0)
+    orc = oracle_lib.Oracle(spec, p)
+    rng = np.random.default_rng(15)
+    xs = np.array([_rand_x(rng, spec.N) for _ in range(200)])
+    m = orc.measure(xs, shift=1e-3, want_s=True)
+    etot = m["Etot"] / m["norm"]
+    gam0 = orc.gamma()
+    max_change = 0.04
+    rc, gam, rep, x = orc.sr_step(etot, m["norm"], m["gvec"], m["ovec"], m["smat"], gam0,
+                                  s_factor0=1e-3, delta_tau=delta_tau, max_change=max_change)
+    assert rc == 0
+    x_np, lam = _sr_solution_numpy(spec, m, 1e-3)
+    np.testing.assert_allclose(x, x_np, rtol=1e-8, atol=1e-12 * np.abs(x_np).max())
+    want, dmax, step = _sr_update_numpy(gam0, x, delta_tau, max_change, spec.n_slater)
+    assert (dmax > max_change) == clipped
+    assert rep[1] == pytest.approx(dmax, rel=1e-15)  # delW_max
+    np.testing.assert_allclose(gam, want, rtol=1e-14, atol=1e-15)
+    if clipped:
+        # the clipped step's largest component is max_change exactly and the
+        # step stays parallel to x
+        assert np.abs(step).max() == pytest.approx(max_change, rel=1e-14)
+        k = int(np.argmax(np.abs(x)))
+        np.testing.assert_allclose(step, x * (step[k] / x[k]), rtol=1e-13, atol=1e-18)
+    else:
+        np.testing.assert_allclose(step, delta_tau * x, rtol=1e-15)
+    # the rbm part of gamma moves by the step alone; f is renormalised
+    nvr = spec.nv_rbm
+    np.testing.assert_allclose(gam[:nvr], gam0[:nvr] - step[:nvr], rtol=1e-14, atol=1e-16)
+    assert np.abs(gam[nvr:]).mean() == pytest.approx(0.25, rel=1e-14)
+    # a planted mistake fails: clipping only the largest component
+    bad = gam0 - delta_tau * x
+    kmax = int(np.argmax(np.abs(delta_tau * x)))
+    bad[kmax] = gam0[kmax] - np.sign(x[kmax]) * max_change
+    if clipped:
+        assert np.abs(bad[:nvr] - gam[:nvr]).max() > 1e-6
 
 
 def test_sampler_energy_4x4_matches_enumeration(oracle_lib):
