@@ -211,6 +211,19 @@ int rbmpp_ratio_exchange(rbmpp_handle* h, int32_t n, const int32_t* x, const int
  * complex, oloc[s][Nv] complex (d psi / d gamma_k); oloc may be NULL. */
 int rbmpp_local(rbmpp_handle* h, int32_t n, const int32_t* x, double* eloc, double* psi,
                 double* oloc);
+/* rbmpp_local with oloc = NULL (local energies and amplitudes of n
+ * configurations, not reweighted), allocating nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  Its
+ * device staging (the configurations, amplitudes, local energies, statuses and
+ * the estimator's per-workgroup scratch) lives in one fixed array of 64 MiB per
+ * GPU, held from the first rbmpp_create on that GPU until the process ends.
+ * It takes at least (64 MiB - 32 MiB - 2 KiB) / (5 N + 44) configurations per
+ * call (61,677 at N = 100, 25,341 at N = 256); a larger n returns RBMPP_E_ARG
+ * with the limit in the message.  Calls that overlap on one GPU (from any
+ * handle or thread) race on the array and must be serialised by the caller;
+ * the array keeps the last call's contents. */
+int rbmpp_local_noalloc(rbmpp_handle* h, int32_t n, const int32_t* x, double* eloc, double* psi);
 /* The centred S = <O O^T>/norm - <O><O>^T of the last rbmpp_measure (before
  * the SR step's scaling and shift), Nv x Nv column-major, in the configured
  * precision; no counterpart in the reference (its Smat is the raw sum). */

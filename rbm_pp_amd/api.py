@@ -84,6 +84,7 @@ def lib():
         L.rbmpp_eval.argtypes = [vp, C.c_int32, ip, dp, dp, dp]
         L.rbmpp_ratio_exchange.argtypes = [vp, C.c_int32, ip, ip, ip, dp, dp]
         L.rbmpp_local.argtypes = [vp, C.c_int32, ip, dp, dp, dp]
+        L.rbmpp_local_noalloc.argtypes = [vp, C.c_int32, ip, dp, dp]
         L.rbmpp_get_smat.argtypes = [vp, dp]
         L.rbmpp_timings.argtypes = [vp, dp]
         L.rbmpp_set_s_precision.argtypes = [vp, C.c_int32]
@@ -274,6 +275,15 @@ class Engine:
         if want_o:
             out["oloc"] = o.view(np.complex128).reshape(n, self.nv)
         return out
+
+    def local_noalloc(self, xs):
+        """rbmpp_local without O through the allocation-free entry point
+        (device staging in the library's fixed array)."""
+        xs = np.ascontiguousarray(np.atleast_2d(xs), np.int32)
+        n = xs.shape[0]
+        el = np.zeros(2 * n); psi = np.zeros(2 * n)
+        self._check(lib().rbmpp_local_noalloc(self.h, n, _i(xs), _d(el), _d(psi)))
+        return dict(eloc=el.view(np.complex128), psi=psi.view(np.complex128))
 
     def set_projection(self, projection: int) -> None:
         """Spin-projection weights: PROJ_NORMAL (normal_code, cos(pi/2) taken as

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -27,6 +28,7 @@ struct rbmpp_handle {
   rbmpp_config cfg;
   int L, N, P, C, ac, ar, nsl, nv, nv_rbm, mc, mr, npair, W, nspw;
   int real_only;
+  int device = 0;  // HIP device current at rbmpp_create
   hipStream_t st = nullptr;
   std::string err;
   std::vector<int> wmap, invmap, orb;
@@ -75,6 +77,18 @@ struct rbmpp_handle {
   hipEvent_t e0 = nullptr, e1 = nullptr;
 };
 
+// Device staging of rbmpp_local_noalloc: one fixed array per GPU, held until
+// the process ends.  In host code its name is only a stand-in; the device
+// address of each GPU's copy comes from hipGetSymbolAddress (noalloc_setup,
+// called by rbmpp_create with the handle's device current) and is the only
+// form in which host code uses the array.
+static constexpr size_t NOALLOC_BYTES = (size_t)64 << 20;
+static constexpr size_t NOALLOC_WG_BYTES = (size_t)32 << 20;  // per-workgroup estimator scratch
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+static constexpr int NOALLOC_MAX_DEVICES = 64;
+static unsigned char* g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+static std::mutex g_noalloc_mu;
+
 static int fail(rbmpp_handle* h, int code, const std::string& msg) {
   if (h) h->err = msg;
   return code;
@@ -96,6 +110,19 @@ static int dalloc(rbmpp_handle* h, T** p, size_t n) {
     return fail(h, RBMPP_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
   }
   h->allocs.push_back((void*)*p);
+  return 0;
+}
+
+// Records the device address of this GPU's staging array (once per device).
+static int noalloc_setup(rbmpp_handle* h) {
+  if (h->device < 0 || h->device >= NOALLOC_MAX_DEVICES) return fail(h, RBMPP_E_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (g_noalloc_ptr[h->device]) return 0;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return fail(h, RBMPP_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_noalloc_ptr[h->device] = (unsigned char*)ptr;
   return 0;
 }
 
@@ -145,6 +172,28 @@ static double toc(rbmpp_handle* h) {
 }
 
 static int ensure_measure(rbmpp_handle* h, size_t n);
+
+// int32 configurations -> the estimators' int8 layout
+__global__ void k_x32_to_x8(size_t n, const int32_t* __restrict__ x, signed char* __restrict__ y) {
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+    y[k] = (signed char)x[k];
+}
+
+// out = {first configuration with a nonzero status or -1, its status}
+__global__ void k_first_status(int n, const int* __restrict__ st, int* __restrict__ out) {
+  __shared__ int first;
+  if (threadIdx.x == 0) first = n;
+  __syncthreads();
+  for (int k = threadIdx.x; k < n; k += blockDim.x)
+    if (st[k] != 0) atomicMin(&first, k);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    out[0] = first < n ? first : -1;
+    out[1] = first < n ? st[first] : 0;
+  }
+}
+
+static size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 extern "C" {
 
@@ -215,6 +264,7 @@ int rbmpp_create(const rbmpp_config* cfg, const int32_t* w_map, const double* pa
   h->gamma.assign(h->nv, 0.0);
   *out = h;
 
+  HIPCHK(h, hipGetDevice(&h->device));
   HIPCHK(h, hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
   HIPCHK(h, hipEventCreate(&h->e0));
   HIPCHK(h, hipEventCreate(&h->e1));
@@ -255,6 +305,7 @@ int rbmpp_create(const rbmpp_config* cfg, const int32_t* w_map, const double* pa
   HIPCHK(h, hipMemcpyAsync(ws.nacc_tmp, zi.data(), sizeof(int) * W, hipMemcpyHostToDevice, h->st));
   HIPCHK(h, hipMemcpyAsync(ws.ctr, zc.data(), sizeof(unsigned long long) * W, hipMemcpyHostToDevice, h->st));
   HIPCHK(h, hipStreamSynchronize(h->st));
+  if ((rc = noalloc_setup(h))) return rc;
   return RBMPP_OK;
 }
 
@@ -1034,6 +1085,80 @@ int rbmpp_local(rbmpp_handle* h, int32_t n, const int32_t* x, double* eloc, doub
   h->measured = 0;
   free_tmp(tmp);
   return rc;
+}
+
+int rbmpp_local_noalloc(rbmpp_handle* h, int32_t n, const int32_t* x, double* eloc, double* psi) {
+  if (!h || !x || n <= 0) return RBMPP_E_ARG;
+  const int N = h->N;
+  for (size_t k = 0; k < (size_t)n * N; k++)
+    if (x[k] != 1 && x[k] != -1) return fail(h, RBMPP_E_ARG, "x must be +-1");
+  for (int s = 0; s < n; s++) {
+    int up = 0;
+    for (int i = 0; i < N; i++) up += x[(size_t)s * N + i] == 1;
+    if (up != h->npair) return fail(h, RBMPP_E_ARG, "configuration not in the Sz=0 sector");
+  }
+  unsigned char* base = (h->device >= 0 && h->device < NOALLOC_MAX_DEVICES) ? g_noalloc_ptr[h->device] : nullptr;
+  if (!base) return fail(h, RBMPP_E_STATE, "local_noalloc: staging array of this device not set up");
+  const DevModel hm = devmodel(h);
+  // per-workgroup scratch of the estimator that will run: F^-1 of every copy
+  // (block form) or the N x N recompute matrix (N > RBMPP_NLDS)
+  const size_t np = (size_t)h->npair;
+  const size_t wg = hm.block ? (size_t)h->C * np * np * sizeof(double)
+                             : (N > RBMPP_NLDS ? (size_t)N * N * sizeof(double) : 0);
+  const int gcap = wg ? (int)std::min<size_t>(1024, NOALLOC_WG_BYTES / wg) : 1024;
+  // per configuration: x as int32 and int8, psi and eloc (complex), rw, status
+  const size_t per_cfg = (size_t)N * 5 + 2 * sizeof(double2) + sizeof(double) + sizeof(int);
+  const size_t fixed = 256 + round256((size_t)gcap * wg) + 7 * 256;
+  const int64_t nmax = (int64_t)((NOALLOC_BYTES - fixed) / per_cfg);
+  if ((int64_t)n > nmax)
+    return fail(h, RBMPP_E_ARG, "local_noalloc: n = " + std::to_string(n) + " exceeds the staging limit of " +
+                                    std::to_string(nmax) + " configurations at N = " + std::to_string(N));
+  const int grid = std::min(n, gcap);
+  size_t off = 0;
+  int* d_hdr = (int*)(base + off); off += 256;
+  double* d_wg = (double*)(base + off); off += round256((size_t)grid * wg);
+  int32_t* d_x32 = (int32_t*)(base + off); off += round256((size_t)n * N * sizeof(int32_t));
+  signed char* d_x8 = (signed char*)(base + off); off += round256((size_t)n * N);
+  double2* d_psi = (double2*)(base + off); off += round256((size_t)n * sizeof(double2));
+  double2* d_eloc = (double2*)(base + off); off += round256((size_t)n * sizeof(double2));
+  double* d_rw = (double*)(base + off); off += round256((size_t)n * sizeof(double));
+  int* d_st = (int*)(base + off); off += round256((size_t)n * sizeof(int));
+  if (off > NOALLOC_BYTES) return fail(h, RBMPP_E_ARG, "local_noalloc: staging layout exceeds the array");
+  // nothing in the array needs clearing: every workgroup writes its scratch
+  // before reading it, every configuration's status and outputs are written
+  // by the estimator, the header by k_first_status
+  HIPCHK(h, hipMemcpyAsync(d_x32, x, (size_t)n * N * sizeof(int32_t), hipMemcpyHostToDevice, h->st));
+  hipLaunchKernelGGL(k_x32_to_x8, dim3(256), dim3(256), 0, h->st, (size_t)n * N, (const int32_t*)d_x32, d_x8);
+  HIPCHK(h, hipGetLastError());
+  LocalArgs a;
+  memset(&a, 0, sizeof a);
+  a.m = h->d_model;
+  a.nconf = n;
+  a.xs = d_x8;
+  a.mode = 1;       // local energy, amplitude (as rbmpp_local without O)
+  a.shift = -1.0;   // not reweighted
+  a.scratch = wg ? d_wg : nullptr;
+  a.psi = d_psi;
+  a.eloc = d_eloc;
+  a.rw = d_rw;
+  a.status = d_st;
+  HIPCHK(h, hipMemcpyAsync(h->d_model, &hm, sizeof(DevModel), hipMemcpyHostToDevice, h->st));
+  if (launch_local(hm, a, grid, h->st) != 0) return fail(h, RBMPP_E_ARG, "local_noalloc: unsupported size");
+  HIPCHK(h, hipGetLastError());
+  hipLaunchKernelGGL(k_first_status, dim3(1), dim3(256), 0, h->st, n, (const int*)d_st, d_hdr);
+  HIPCHK(h, hipGetLastError());
+  int hdr[2] = {0, 0};
+  HIPCHK(h, hipMemcpyAsync(hdr, d_hdr, sizeof hdr, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  if (hdr[0] >= 0) {
+    char buf[128];
+    snprintf(buf, sizeof buf, "configuration %d: status %d", hdr[0], hdr[1]);
+    return fail(h, hdr[1], buf);
+  }
+  if (eloc) HIPCHK(h, hipMemcpyAsync(eloc, d_eloc, (size_t)n * sizeof(double2), hipMemcpyDeviceToHost, h->st));
+  if (psi) HIPCHK(h, hipMemcpyAsync(psi, d_psi, (size_t)n * sizeof(double2), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  return RBMPP_OK;
 }
 
 int rbmpp_set_s_precision(rbmpp_handle* h, int32_t s_precision) {

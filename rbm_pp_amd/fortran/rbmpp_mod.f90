@@ -53,7 +53,7 @@ module rbmpp_mod
   public :: rbmpp_measure_energy, rbmpp_measure_correlation
   public :: rbmpp_set_samples, rbmpp_get_samples, rbmpp_n_samples
   public :: rbmpp_get_walkers, rbmpp_get_walker_electrons, rbmpp_eval, rbmpp_ratio_exchange, rbmpp_local
-  public :: rbmpp_get_smat, rbmpp_timings, rbmpp_get_measured
+  public :: rbmpp_get_smat, rbmpp_timings, rbmpp_get_measured, rbmpp_local_noalloc
   public :: rbmpp_error_message
 
   interface
@@ -196,6 +196,15 @@ module rbmpp_mod
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr), value :: h
       integer(c_int32_t), intent(out) :: ele_idx(*)
+    end function
+
+    ! rbmpp_local without O, staging in a fixed device array (no allocation)
+    integer(c_int) function rbmpp_local_noalloc(h, n, x, eloc, psi) bind(C, name="rbmpp_local_noalloc")
+      import :: c_int, c_int32_t, c_double, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: n
+      integer(c_int32_t), intent(in) :: x(*)
+      real(c_double), intent(out) :: eloc(*), psi(*)
     end function
 
     integer(c_int) function rbmpp_eval(h, n, x, psi, pf, invm) bind(C, name="rbmpp_eval")
