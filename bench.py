@@ -35,30 +35,132 @@ def log(msg: str) -> None:
 
 
 def _cpu_chain(args):
-    L, alphar, J2, nsweeps, seed = args
+    L, alphar, J2, nsweeps, nmeas, seed = args
     import oracle
     from rbm_pp_amd import lattice as lt
     spec = lt.j1j2_model(L, alphar=alphar, J2=J2)
-    p = lt.random_params(spec)
+    p = lt.random_params(spec, seed_w=1, seed_f=2, sigma_w=0.01)
     orc = oracle.Oracle(spec, p)
     rng = oracle.Rng("mt", seed)
     t0 = time.perf_counter()
     # the reference's adaptive recompute period (fixed_k = 0), as the GPU run
-    orc.make_sample(rng, True, nsample=nsweeps, nwarmup=0, fixed_k=0)
-    return nsweeps, time.perf_counter() - t0
+    s = orc.make_sample(rng, True, nsample=nsweeps, nwarmup=0, fixed_k=0)
+    t_s = time.perf_counter() - t0
+    # calc_phys_quantity_opt on the chain's first samples (Pf, X^-1 recomputed
+    # per sample, E_loc, O_k); the S product is timed separately with BLAS
+    t0 = time.perf_counter()
+    orc.measure(s["x"][:nmeas], shift=0.0, want_s=False)
+    t_m = time.perf_counter() - t0
+    return nsweeps, t_s, nmeas, t_m
 
 
-def cpu_baseline(L, alphar, J2, nsweeps, procs):
-    """The oracle's restatement of make_sample (one single-threaded chain per
-    process, the reference's one-rank-per-core layout) on the host cores."""
+def _host_info():
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def cpu_baseline(L, alphar, J2, nsweeps, nmeas, procs, ns_step):
+    """The oracle's restatement of one SR iteration on the host cores: one
+    single-threaded chain per process (the reference's one-rank-per-core
+    layout) timed over make_sample and calc_phys_quantity_opt, plus the S
+    product and the dense solve with the host BLAS / LAPACK (numpy) on a
+    bounded number of rows, scaled to the step's ns_step rows.  Returns
+    (end-to-end samples/s, details)."""
     ctx = mp.get_context("fork")
     t0 = time.perf_counter()
     with ctx.Pool(procs) as pool:
-        res = pool.map(_cpu_chain, [(L, alphar, J2, nsweeps, 1000 + r) for r in range(procs)])
+        res = pool.map(_cpu_chain, [(L, alphar, J2, nsweeps, nmeas, 1000 + r) for r in range(procs)])
     wall = time.perf_counter() - t0
-    total = sum(r[0] for r in res)
-    slowest = max(r[1] for r in res)
-    return total / slowest, wall
+    r_samp = procs * nsweeps / max(r[1] for r in res)  # samples/s of the slowest chain x chains
+    r_meas = procs * nmeas / max(r[3] for r in res)
+    from rbm_pp_amd import lattice as lt
+    nv = lt.j1j2_model(L, alphar=alphar, J2=J2).nv
+    rows = 20000
+    O = np.random.default_rng(0).standard_normal((rows, nv))
+    t0 = time.perf_counter()
+    S = O.T @ O
+    t_s_rows = time.perf_counter() - t0
+    A = S / rows + 1e-3 * np.eye(nv)
+    t0 = time.perf_counter()
+    np.linalg.solve(A, np.ones(nv))
+    t_solve = time.perf_counter() - t0
+    t_iter = ns_step / r_samp + ns_step / r_meas + t_s_rows * ns_step / rows + t_solve
+    det = dict(sampler_samples_per_s=r_samp, measure_samples_per_s=r_meas,
+               s_build_s=t_s_rows * ns_step / rows, solve_s=t_solve, wall_s=wall)
+    return ns_step / t_iter, det
+
+
+def synthetic_orbitals(L, nsl):
+    """Orbital table of a pair product with nsl orbitals on an L x L lattice
+    (the SR step's size only): the diagonal pairs (i, i) share nsl / N
+    orbitals, every unordered pair i < j one of the others."""
+    N = L * L
+    nd = nsl // N
+    orb = np.zeros((N, N), np.int64)
+    for i in range(N):
+        orb[i, i] = i % nd
+    iu = np.triu_indices(N, 1)
+    idx = nd + np.arange(len(iu[0])) % (nsl - nd)
+    orb[iu] = idx
+    orb[(iu[1], iu[0])] = idx
+    return orb
+
+
+def sr_shape_bench(api, lt, L, alphar, J2, ns, nsl=None, seed=5):
+    """The SR step (S build + system solve) on a synthetic measurement of ns
+    configurations in the three S precisions: times, FLOP rate of the S build
+    against the precision's matrix-core peak, S and step error against FP64."""
+    spec = lt.j1j2_model(L, alphar=alphar, J2=J2)
+    if nsl is not None:
+        spec.orb = synthetic_orbitals(L, nsl)
+        spec.n_slater = nsl
+    p = lt.random_params(spec, seed_w=1, seed_f=2, sigma_w=0.01)
+    eng = api.Engine(spec, n_walkers=1, n_samples=1, params=p)
+    nv = spec.nv
+    eng.set_measured(ns, seed=seed)
+    f_s = nv * (nv + 1) * float(ns)
+    peaks = {"fp64": MFMA_PEAK_TFS["fp64"], "fp32": MFMA_PEAK_TFS["fp32"], "bf16x3": MFMA_PEAK_TFS["bf16"] / 3.0}
+    res = {}
+    for name, prec in (("fp64", api.S_FP64), ("fp32", api.S_FP32), ("bf16x3", api.S_BF16X3)):
+        eng.set_s_precision(prec)
+        best = None
+        for _ in range(2):  # the first call also captures the solve's graph
+            g, rep = eng.sr_step(p.gamma(), s_factor0=1e-4, delta_tau=0.05, refine_iters=1)
+            t = eng.timings()
+            if best is None or t["sr_total"] < best[1]["sr_total"]:
+                best = (g - p.gamma(), t, rep)
+        S = eng.smat() if nv <= 6000 else None
+        res[name] = dict(step=best[0], t=best[1], rep=best[2], S=S)
+        log(f"SR step nv={nv} ns={ns} {name}: S build {best[1]['s_build'] * 1e3:.1f} ms, solve "
+            f"{best[1]['solve'] * 1e3:.1f} ms, residual {best[2]['residual']:.1e}")
+    base = res["fp64"]
+    out = {"nv": nv, "configurations": ns, "lattice": f"{L}x{L}", "alphar": alphar, "n_slater": spec.n_slater}
+    for name, r in res.items():
+        ach = f_s / r["t"]["s_build"] / 1e12
+        d = dict(s_build_s=r["t"]["s_build"], solve_s=r["t"]["solve"], sr_step_s=r["t"]["sr_total"],
+                 tflops=ach, peak_tflops=peaks[name], mfma_frac=ach / peaks[name],
+                 residual=r["rep"]["residual"],
+                 step_rel_err=float(np.linalg.norm(r["step"] - base["step"]) / np.linalg.norm(base["step"])),
+                 speedup_vs_fp64=base["t"]["sr_total"] / r["t"]["sr_total"])
+        if r["S"] is not None:
+            d["s_rel_err"] = float(np.linalg.norm(r["S"] - base["S"]) / np.linalg.norm(base["S"]))
+        out[name] = d
+    eng.close()
+    return out
+
+
+def run_sr_shapes(api, lt):
+    """The 16x16 workload's SR step (alphar = 16, Nv = 5136, 4x10^5
+    configurations) and a synthetic Nv = 19984 (4x10^5 configurations)."""
+    return [sr_shape_bench(api, lt, 16, 16, 0.55, 400000),
+            sr_shape_bench(api, lt, 16, 16, 0.55, 400000, nsl=62 * 256)]
 
 
 def main():
@@ -78,6 +180,7 @@ def main():
                          "(make_sample.f90:74-128, at most 100)")
     ap.add_argument("--cpu-sweeps", type=int, default=600)
     ap.add_argument("--cpu-procs", type=int, default=0)
+    ap.add_argument("--cpu-meas", type=int, default=60, help="samples measured per CPU chain")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64-compare", action="store_true")
     ap.add_argument("--s-precision", choices=["fp64", "fp32", "bf16x3"], default="fp32",
@@ -92,6 +195,10 @@ def main():
                          "(the reference's cos(pi/2) = 6.1e-17, skew-form sampler), transverse")
     ap.add_argument("--no-correlation", action="store_true",
                     help="skip the correlation-run measurement on the last step's samples")
+    ap.add_argument("--no-sr-shapes", action="store_true",
+                    help="skip the SR steps at the 16x16 sizes (Nv = 5136 and a synthetic ~2e4)")
+    ap.add_argument("--sr-only", action="store_true",
+                    help="run only the SR steps at the large sizes and print them")
     ap.add_argument("--sampler-only", action="store_true",
                     help="profiling aid: thermalise, run one make_sample launch and exit")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -114,18 +221,48 @@ def main():
         print(json.dumps({"copy_bytes": n, "copies": 3}))
         return
 
+    if args.sr_only:
+        from rbm_pp_amd import api
+        from rbm_pp_amd import lattice as lt
+        print(json.dumps({"sr_shapes": run_sr_shapes(api, lt)}))
+        return
+
     # CPU baseline first: worker processes are forked before the GPU is touched
     cpu = None
     if not args.no_cpu and not args.sampler_only:
+        # the GPU box allots 16 host cores to its one GPU
         procs = args.cpu_procs or min(16, os.cpu_count() or 1)
-        log(f"cpu baseline: {procs} chains x {args.cpu_sweeps} sweeps (oracle make_sample)")
-        rate, wall = cpu_baseline(args.L, args.alphar, args.J2, args.cpu_sweeps, procs)
+        model, ncpu = _host_info()
+        ns_step = args.walkers * math.ceil(args.samples / args.walkers)
+        log(f"cpu baseline: {procs} chains x {args.cpu_sweeps} sweeps + {args.cpu_meas} measured samples each "
+            f"(oracle), host {model}, {ncpu} logical CPUs")
+        rate, det = cpu_baseline(args.L, args.alphar, args.J2, args.cpu_sweeps, args.cpu_meas, procs, ns_step)
+        cal = None
+        try:
+            from oracle import calibrate
+            from rbm_pp_amd import lattice as lt0
+            spec0 = lt0.j1j2_model(args.L, alphar=args.alphar, J2=args.J2)
+            cal = calibrate.fast_update_times(spec0, lt0.random_params(spec0, seed_w=1, seed_f=2, sigma_w=0.01))
+        except OSError:
+            cal = None
         cpu = {"value": rate, "unit": "samples/s", "cores": procs, "kind": "port",
-               "sample": f"{procs} independent chains x {args.cpu_sweeps} stored sweeps each "
-                         f"(N={args.L * args.L} proposals/sweep, adaptive recompute period as "
-                         f"make_sample.f90:74-128), "
-                         f"{wall:.1f} s wall"}
-        log(f"cpu baseline: {rate:.1f} samples/s")
+               "sample": f"{procs} chains (one per core, OMP_NUM_THREADS=1 each) x {args.cpu_sweeps} stored sweeps "
+                         f"(N={args.L * args.L} proposals each, adaptive recompute period, make_sample.f90:74-128) "
+                         f"and calc_phys_quantity_opt on {args.cpu_meas} samples per chain; S = O^T O on 20000 "
+                         f"rows and the dense solve with numpy BLAS/LAPACK, scaled to {ns_step} samples per SR "
+                         f"iteration; {det['wall_s']:.1f} s wall",
+               "host_cpu": model, "host_logical_cpus": ncpu,
+               "sampler_samples_per_s": det["sampler_samples_per_s"],
+               "measure_samples_per_s": det["measure_samples_per_s"],
+               "s_build_s": det["s_build_s"], "solve_s": det["solve_s"]}
+        if cal:
+            # seconds per call of pfupdate_two.f90's two routines, the
+            # restatement over the compiled reference on identical inputs
+            cpu["calibration"] = {"pfm_two_port_over_reference": cal["port_ratio_s"] / cal["ref_ratio_s"],
+                                  "invm_two_port_over_reference": cal["port_update_s"] / cal["ref_update_s"],
+                                  "pfm_two_reference_s": cal["ref_ratio_s"], "invm_two_reference_s": cal["ref_update_s"]}
+        log(f"cpu baseline: {rate:.1f} samples/s end to end (sampler {det['sampler_samples_per_s']:.1f}/s, "
+            f"measurement {det['measure_samples_per_s']:.1f}/s)")
 
     from rbm_pp_amd import api
     S_MODES = {"fp64": api.S_FP64, "fp32": api.S_FP32, "bf16x3": api.S_BF16X3}
@@ -246,6 +383,10 @@ def main():
                 "energy_per_site": c["Etot"] / 4 / N}
         log(f"correlation measurement: {t_dev:.2f} s for {ns_step} samples ({ns_step / t_dev:.0f} samples/s)")
 
+    sr_shapes = None
+    if not args.no_sr_shapes:
+        sr_shapes = run_sr_shapes(api, lt)
+
     samples = ns_step * args.steps
     acc = sum(r["nacc"] for r in recs) / samples / N
     sweeps_per_launch = args.walkers * sweeps_per_launch_walker
@@ -261,10 +402,6 @@ def main():
     block = args.projection == "normal"
     b_unit = 2.0 * Cc * N ** 3 * (1.0 + acc) if block else 4.0 * Cc * N ** 3 * (1.0 + 2.0 * acc)
     achieved = b_unit * sweeps_per_launch / t_sweep / 1e9
-    # the same sweep priced with the reference's layout and passes (full
-    # N x N X^-1; calculate_new_InvM_two reads it again to form its update
-    # vectors, pfupdate_two.f90): 8 C N^3 (1 + 3A) -- an equivalent rate
-    b_ref = 8.0 * Cc * N ** 3 * (1.0 + 3.0 * acc)
     # measured HBM bytes per walker sweep of this kernel and workload
     # (rocprofv3 FETCH_SIZE and WRITE_SIZE, each scaled by its reading for a
     # copy of known size), when the committed counter summary covers it
@@ -306,14 +443,15 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_sweep", "launch_s": t_sweep,
-                     "bytes_per_walker_sweep": b_unit,
-                     "reference_layout_equiv_GBs": b_ref * sweeps_per_launch / t_sweep / 1e9},
+                     "bytes_per_walker_sweep": b_unit},
         "cpu_baseline": cpu,
     }
     if fp64:
         out["s_build"] = fp64
     if corr:
         out["correlation"] = corr
+    if sr_shapes:
+        out["sr_shapes"] = sr_shapes
     print(json.dumps(out))
 
 

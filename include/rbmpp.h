@@ -91,9 +91,13 @@ typedef struct {
   double delta_tau;          /* step (main.f90:79) */
   double max_change;         /* 0.04 (main.f90:50) */
   double slater_abs_av;      /* 0.25 (module.f90:66) */
-  int32_t refine_iters;      /* FP64 iterative-refinement sweeps after the Cholesky solve */
-  int32_t reserved;
+  int32_t refine_iters;      /* FP64 iterative-refinement sweeps after the Cholesky solve (0..8) */
+  int32_t flags;             /* RBMPP_SR_* bits; 0 by default */
 } rbmpp_sr_config;
+
+/* rbmpp_sr_config.flags: run the SR system's launches eagerly instead of
+   replaying the hipGraph they are captured into on the first call */
+#define RBMPP_SR_NO_GRAPH 1
 
 typedef struct {
   double lambda;             /* sqrt(x^T S x) * 5/L (SR_pdposv.f90:309) */
@@ -189,6 +193,17 @@ int64_t rbmpp_n_samples(const rbmpp_handle* h);
  * Any pointer may be NULL.  Valid after rbmpp_measure / rbmpp_make_sample
  * with fused measurement. */
 int rbmpp_get_measured(const rbmpp_handle* h, double* psi, double* eloc, double* rw);
+/* Loads a measurement in place of rbmpp_measure, for rbmpp_sr_step: the
+ * per-configuration values of rbmpp_get_measured plus the O rows
+ * (orows[s*Nv + k] = O_loc(k) of configuration s, already reweighted as
+ * calc_phys_quantity.f90:187-219 stores them; complex models: rows 2s (re) and
+ * 2s+1 (im)), then the sums of main.f90:351-367.  With orows == NULL the
+ * values are synthetic instead (psi = rw = 1, eloc and O uniform in (-1, 1)
+ * from `seed`, the f_ii columns zero), so that the SR step can be timed at
+ * shapes the sampler is not run at.  n may exceed the sample store.  No
+ * counterpart in the reference. */
+int rbmpp_set_measured(rbmpp_handle* h, int64_t n, const double* psi, const double* eloc, const double* rw,
+                       const double* orows, uint64_t seed);
 /* Walker state after make_sample: x[w*N+i], pf[w*C+q], psi[2w+{0,1}] and
  * (optional) InvM in the reference's layout invm[w][q][(j)*N+(i)] = InvM(i+1,j+1,q+1). */
 int rbmpp_get_walkers(const rbmpp_handle* h, int32_t* x, double* pf, double* psi, double* invm,

@@ -729,6 +729,22 @@ static void invm_two_update(const om_model* m, int q, int msa, int msb, const in
   IM(invm, m, q, msb, msa) = IM(invm, m, q, msb, msa) + f * InvDet;
 }
 
+/* calculate_new_PfM_two and calculate_new_InvM_two on caller arrays, with the
+ * reference's arguments (pfupdate_two.f90:5-25, 93-111): 0-based electrons
+ * msa, msb, the electron-to-site list after the move, InvM of every copy in
+ * the reference's column-major layout.  For timing the restatement against
+ * the compiled reference on identical inputs. */
+void om_pfm_two(const om_model* m, int msa, int msb, const int* tmp_idx, const double* invm,
+                const double* pf_old, double* pf_new) {
+  double va[m->N], vb[m->N];
+  for (int q = 0; q < m->C; q++) pf_new[q] = pfm_two_ratio(m, q, msa, msb, tmp_idx, invm, va, vb) * pf_old[q];
+}
+
+void om_invm_two(const om_model* m, int msa, int msb, const int* ele_idx, double* invm) {
+  double vs[m->N], vt[m->N], vp[m->N], vq[m->N];
+  for (int q = 0; q < m->C; q++) invm_two_update(m, q, msa, msb, ele_idx, invm, vs, vt, vp, vq);
+}
+
 /* Proposed exchange of the spins at sites i1, i2 (must differ): ratio per copy,
  * new Pfaffians and new amplitude (make_sample.f90:167-198).  Leaves the
  * proposal in the state's scratch (thc2, thr2, tmp_idx, pf2, sgn2, psi2). */

@@ -71,6 +71,11 @@ void om_sign_pp(const om_model* m, const int* ele_idx, int* sgn);
 int  om_ratio_exchange(const om_model* m, const om_state* s, int i1, int i2,
                        double* ratio, double* pf_new, double* psi_new);
 int  om_accept_exchange(const om_model* m, om_state* s, int i1, int i2, om_rng* r);
+/* pfupdate_two.f90 with the reference's arguments (0-based electrons and
+   sites, InvM column-major per copy as above); timing comparisons only */
+void om_pfm_two(const om_model* m, int msa, int msb, const int* tmp_idx, const double* invm,
+                const double* pf_old, double* pf_new);
+void om_invm_two(const om_model* m, int msa, int msb, const int* ele_idx, double* invm);
 double om_eloc(const om_model* m, const om_state* s, double* eloc /*2*/, om_rng* r);
 int  om_oloc(const om_model* m, const om_state* s, double* oloc /*2*Nv*/);
 

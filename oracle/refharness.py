@@ -1,6 +1,7 @@
 """ctypes front end of oracle/_ref/libref.so: the reference's own Fortran
 routines (compiled from the reference sources by oracle/Makefile) behind the
-bind(C) harness oracle/ref_harness.f90.  Test infrastructure only; absent on
+bind(C) harness oracle/ref_harness.f90.  Test infrastructure only (tests/,
+the golden-vector script and bench.py's cpu_baseline calibration); absent on
 machines without the reference sources, where tests fall back to the
 committed fixtures under tests/golden/."""
 from __future__ import annotations
@@ -10,12 +11,12 @@ import os
 
 import numpy as np
 
-ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB = os.path.join(ROOT, "oracle", "_ref", "libref.so")
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_ref", "libref.so")
 # the same routines with transverse_code/spin_projection.f90
-LIB_TC = os.path.join(ROOT, "oracle", "_ref", "libref_tc.so")
+LIB_TC = os.path.join(HERE, "_ref", "libref_tc.so")
 # the Calc_Correlation tree, with the correlation entry points
-LIB_CC = os.path.join(ROOT, "oracle", "_ref", "libref_cc.so")
+LIB_CC = os.path.join(HERE, "_ref", "libref_cc.so")
 
 
 def _path(projection: int, tree: str | None) -> str:

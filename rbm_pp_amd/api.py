@@ -24,6 +24,7 @@ LIB = os.path.join(HERE, "librbmpp.so")
 
 S_FP64, S_FP32, S_BF16X3 = 0, 1, 2
 PROJ_NORMAL, PROJ_TRANSVERSE, PROJ_NORMAL_EXACT = 0, 1, 2
+SR_NO_GRAPH = 1
 
 
 class RbmppError(RuntimeError):
@@ -44,7 +45,7 @@ class Config(C.Structure):
 
 class SrConfig(C.Structure):
     _fields_ = [("s_factor0", C.c_double), ("delta_tau", C.c_double), ("max_change", C.c_double),
-                ("slater_abs_av", C.c_double), ("refine_iters", C.c_int32), ("reserved", C.c_int32)]
+                ("slater_abs_av", C.c_double), ("refine_iters", C.c_int32), ("flags", C.c_int32)]
 
 
 class SrReport(C.Structure):
@@ -78,6 +79,7 @@ def lib():
         L.rbmpp_set_samples.argtypes = [vp, C.c_int32, ip]
         L.rbmpp_get_samples.argtypes = [vp, ip, dp]
         L.rbmpp_get_measured.argtypes = [vp, dp, dp, dp]
+        L.rbmpp_set_measured.argtypes = [vp, C.c_int64, dp, dp, dp, dp, C.c_uint64]
         L.rbmpp_n_samples.restype = C.c_int64; L.rbmpp_n_samples.argtypes = [vp]
         L.rbmpp_get_walkers.argtypes = [vp, ip, dp, dp, dp, C.POINTER(C.c_int64)]
         L.rbmpp_get_walker_electrons.argtypes = [vp, ip]
@@ -202,9 +204,12 @@ class Engine:
                     wf_sgn=complex(ws[0], ws[1]), chi_sp=sp.reshape(nchi, 2), chi_vbs=vb.reshape(4, nchi))
 
     def sr_step(self, gamma=None, s_factor0: float = 1e-3, delta_tau: float = 0.05,
-                max_change: float = 0.04, slater_abs_av: float = 0.25, refine_iters: int = 1):
+                max_change: float = 0.04, slater_abs_av: float = 0.25, refine_iters: int = 1,
+                graph: bool = True):
+        """graph=False runs the SR system's launches eagerly instead of
+        replaying the captured graph (RBMPP_SR_NO_GRAPH)."""
         g = np.ascontiguousarray(self.gamma() if gamma is None else gamma, np.float64).copy()
-        c = SrConfig(s_factor0, delta_tau, max_change, slater_abs_av, refine_iters, 0)
+        c = SrConfig(s_factor0, delta_tau, max_change, slater_abs_av, refine_iters, 0 if graph else SR_NO_GRAPH)
         rep = SrReport()
         self._check(lib().rbmpp_sr_step(self.h, C.byref(c), _d(g), C.byref(rep)))
         return g, {k: getattr(rep, k) for k, _ in SrReport._fields_}
@@ -227,6 +232,18 @@ class Engine:
         psi = np.zeros(2 * n); el = np.zeros(2 * n); rw = np.zeros(n)
         self._check(lib().rbmpp_get_measured(self.h, _d(psi), _d(el), _d(rw)))
         return psi.view(np.complex128), el.view(np.complex128), rw
+
+    def set_measured(self, n: int, psi=None, eloc=None, rw=None, orows=None, seed: int = 1) -> None:
+        """Load a measurement (psi, reweighted eloc, rw per configuration and
+        the O rows) in place of calc_phys_quantity_opt; with orows=None a
+        seeded synthetic one is generated on the device."""
+        if orows is None:
+            self._check(lib().rbmpp_set_measured(self.h, n, None, None, None, None, seed))
+        else:
+            c2 = lambda a: np.ascontiguousarray(np.stack([np.real(a), np.imag(a)], -1).reshape(-1))
+            o = np.ascontiguousarray(orows, np.float64)
+            self._check(lib().rbmpp_set_measured(self.h, n, _d(c2(psi)), _d(c2(eloc)),
+                                                 _d(np.ascontiguousarray(rw, np.float64)), _d(o), seed))
 
     def get_walkers(self, with_invm: bool = False):
         W, N, Cc = self.W, self.N, self.C

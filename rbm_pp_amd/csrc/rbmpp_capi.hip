@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <climits>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -49,6 +50,7 @@ struct rbmpp_handle {
   signed char* d_xs = nullptr;
   double2* d_psis = nullptr;
   int64_t n_stored = 0;
+  int64_t n_meas = 0;  // configurations in the measurement buffers (rows of O / 1 or 2)
   int walkers_ready = 0;
   int minv_block = 0;  // ws.minv holds F^-1 per copy (block form) instead of folded X^-1
   int fused_valid = 0;     // measurement buffers hold the fused results of the stored samples
@@ -68,6 +70,15 @@ struct rbmpp_handle {
   // SR
   double *d_S = nullptr, *d_A = nullptr, *d_L = nullptr, *d_vec = nullptr;
   int* d_info = nullptr;
+  double* d_sc = nullptr;  // SR scalars (k_sr_prepare)
+  int* d_fl = nullptr;     // SR flags
+  unsigned char* d_fii = nullptr;  // [nv] 1 for the f_ii parameters
+  hipGraph_t sr_graph = nullptr;   // captured SR pipeline
+  hipGraphExec_t sr_exec = nullptr;
+  int sr_graph_iters = -1;         // refine_iters it was captured with
+  unsigned char* d_packed = nullptr;  // centred O rows packed for the S build
+  size_t packed_cap = 0;
+  int n_cu = 256;                     // compute units of the device
   DevModel* d_model = nullptr;  // device copy of devmodel(h)
   int measured = 0;
   int s_valid = 0;
@@ -206,6 +217,8 @@ void rbmpp_destroy(rbmpp_handle* h) {
   for (void* p : h->allocs) hipFree(p);
   if (h->e0) hipEventDestroy(h->e0);
   if (h->e1) hipEventDestroy(h->e1);
+  if (h->sr_exec) hipGraphExecDestroy(h->sr_exec);
+  if (h->sr_graph) hipGraphDestroy(h->sr_graph);
   if (h->st) hipStreamDestroy(h->st);
   delete h;
 }
@@ -227,7 +240,10 @@ int rbmpp_create(const rbmpp_config* cfg, const int32_t* w_map, const double* pa
   if (L < 2 || N % 2) return bad("L must give an even number of sites");
   if (P < 1 || h->C > RBMPP_CMAX) return bad("n_sym must be in [1, 8]");
   if (h->ac < 0 || h->ar < 0 || h->mr > RBMPP_MRMAX || h->mc > RBMPP_MCMAX) return bad("alpha out of range");
-  if (h->nsl < 1 || h->nsl > 4 * N) return bad("n_slater must be in [1, 4N]");
+  // the estimators bin the pair-product derivative into at most 4N orbitals
+  // (launch_local rejects more); a larger table only serves loaded
+  // measurements (rbmpp_set_measured)
+  if (h->nsl < 1 || h->nsl > N * N) return bad("n_slater must be in [1, N^2]");
   if (N > 256) return bad("N > 256 not supported");
   if (h->W < 1 || h->nspw < 0) return bad("n_walkers / n_samples");
   if (cfg->spin_parity != 1 && cfg->spin_parity != -1) return bad("spin_parity must be +-1");
@@ -265,6 +281,7 @@ int rbmpp_create(const rbmpp_config* cfg, const int32_t* w_map, const double* pa
   *out = h;
 
   HIPCHK(h, hipGetDevice(&h->device));
+  HIPCHK(h, hipDeviceGetAttribute(&h->n_cu, hipDeviceAttributeMultiprocessorCount, h->device));
   HIPCHK(h, hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
   HIPCHK(h, hipEventCreate(&h->e0));
   HIPCHK(h, hipEventCreate(&h->e1));
@@ -437,6 +454,7 @@ int rbmpp_make_sample(rbmpp_handle* h, int32_t first_iteration, double shift, in
   h->fused_valid = fused ? 1 : 0;
   h->fused_shift = shift;
   h->n_stored = (int64_t)h->W * h->nspw;
+  if (fused) h->n_meas = h->n_stored;
   h->measured = 0;
   return RBMPP_OK;
 }
@@ -480,8 +498,11 @@ static int ensure_measure(rbmpp_handle* h, size_t n) {
 
 static int ensure_part(rbmpp_handle* h, size_t n) {
   if (n <= h->part_cap) return 0;
-  int rc = dalloc(h, &h->d_part, n);
+  double* p = nullptr;
+  int rc = dalloc(h, &p, n);
   if (rc) return rc;
+  dfree(h, h->d_part);
+  h->d_part = p;
   h->part_cap = n;
   return 0;
 }
@@ -587,6 +608,7 @@ int rbmpp_measure(rbmpp_handle* h, double shift, double* etot, double* etot2, do
   if (!(h->fused_valid && h->fused_shift == shift))
     rc = run_local(h, n, h->d_xs, 1 | 2, shift, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
   if (rc) return rc;
+  h->n_meas = n;
   rc = reduce_measure(h, n, true);
   if (rc) return rc;
   h->t[2] = toc(h);
@@ -611,6 +633,7 @@ int rbmpp_measure_energy(rbmpp_handle* h, double shift, double* etot, double* et
   if (!(h->fused_valid && h->fused_shift == shift))
     rc = run_local(h, n, h->d_xs, 1, shift, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
   if (rc) return rc;
+  h->n_meas = n;
   rc = reduce_measure(h, n, false);
   if (rc) return rc;
   h->t[2] = toc(h);
@@ -643,6 +666,7 @@ int rbmpp_measure_correlation(rbmpp_handle* h, double shift, double* etot, doubl
   rc = run_local(h, n, h->d_xs, 1, shift, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
                  h->d_cpart, grid);
   if (rc) return rc;
+  h->n_meas = n;
   rc = reduce_measure(h, n, false);
   if (rc) return rc;
   launch_sum_strided((int)nsum, grid, (int)len, h->d_cpart, h->d_csum, h->st);
@@ -670,28 +694,72 @@ int rbmpp_measure_correlation(rbmpp_handle* h, double shift, double* etot, doubl
   return RBMPP_OK;
 }
 
-// centred S = <O O^T>/norm - <O><O>^T in the configured precision
+// the f_ii mask of the SR checks (SR_pdposv.f90:78-100) on the device
+static int ensure_fii(rbmpp_handle* h) {
+  if (h->d_fii) return 0;
+  const int nv = h->nv, N = h->N, nsl = h->nsl;
+  int rc = dalloc(h, &h->d_fii, (size_t)nv);
+  if (rc) return rc;
+  std::vector<unsigned char> fii(nv, 0);
+  for (int i = 0; i < N; i++) fii[h->orb[(size_t)i * N + i] + nv - nsl] = 1;
+  HIPCHK(h, hipMemcpyAsync(h->d_fii, fii.data(), nv, hipMemcpyHostToDevice, h->st));
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  return 0;
+}
+
+// Seeded synthetic measurement (rbmpp_set_measured with orows == NULL):
+// psi = 1, rw = 1, eloc and the O entries uniform in (-1, 1) from a hash of
+// (seed, row, column); the f_ii columns are zero, as in a measured O.
+__device__ __forceinline__ double synth_u(uint64_t seed, uint64_t a) {
+  uint64_t z = seed + 0x9e3779b97f4a7c15ull * (a + 1);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  z ^= z >> 31;
+  return (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+
+__global__ void k_synth_measurement(int64_t n, int nv, int cplx_rows, uint64_t seed,
+                                    const unsigned char* __restrict__ fii, double2* psi, double2* eloc, double* rw,
+                                    double* orows) {
+  const size_t rows = (size_t)n * (cplx_rows ? 2 : 1);
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < rows * nv; e += stride) {
+    const int k = (int)(e % nv);
+    orows[e] = fii[k] ? 0.0 : synth_u(seed, e);
+  }
+  for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < (size_t)n; s += stride) {
+    psi[s] = make_double2(1.0, 0.0);
+    rw[s] = 1.0;
+    eloc[s] = make_double2(synth_u(seed ^ 0x5555555555555555ull, s), 0.0);
+  }
+}
+
+// centred S = <O O^T>/norm - <O><O>^T in the configured precision: the O
+// rows are centred once in FP64 and packed in the precision of the products
+// (rbmpp_syrk.hip), then multiplied on the matrix cores
 static int build_s(rbmpp_handle* h) {
-  const int nv = h->nv, n = (int)h->n_stored;
+  const int nv = h->nv, n = (int)h->n_meas;
   int rc;
   if (!h->d_S && ((rc = dalloc(h, &h->d_S, (size_t)nv * nv)) || (rc = dalloc(h, &h->d_A, (size_t)nv * nv)) ||
                   (rc = dalloc(h, &h->d_L, (size_t)nv * nv)) || (rc = dalloc(h, &h->d_vec, (size_t)4 * nv)) ||
-                  (rc = dalloc(h, &h->d_info, 1))))
+                  (rc = dalloc(h, &h->d_info, 1)) || (rc = dalloc(h, &h->d_sc, 8)) || (rc = dalloc(h, &h->d_fl, 8))))
     return rc;
   const int rows = h->real_only ? n : 2 * n;
-  const int nt = syrk_tiles(nv), np = nt * (nt + 1) / 2;
-  // row splits: about two workgroups per CU (256 CUs)
-  int nsplit = std::max(1, std::min(64, (512 + np - 1) / np));
-  nsplit = std::max(1, std::min(nsplit, rows / 64));
-  const size_t ts = syrk_tile_size();
-  if ((rc = ensure_part(h, (size_t)nsplit * np * ts * ts))) return rc;
+  const SyrkPlan pl = syrk_plan(h->cfg.s_precision, rows, nv, h->n_cu);
+  if ((rc = ensure_part(h, pl.part_doubles))) return rc;
+  if (pl.packed_bytes > h->packed_cap) {
+    unsigned char* pk = nullptr;
+    if ((rc = dalloc(h, &pk, pl.packed_bytes))) return rc;
+    dfree(h, h->d_packed);
+    h->d_packed = pk;
+    h->packed_cap = pl.packed_bytes;
+  }
   // obar to device
   HIPCHK(h, hipMemcpyAsync(h->d_vec, h->m_o.data(), sizeof(double) * nv, hipMemcpyHostToDevice, h->st));
   launch_row_kappa(n, h->real_only ? 0 : 1, h->d_mpsi, h->d_rw, h->d_kappa, h->st);
-  const int prec = h->cfg.s_precision;  // RBMPP_S_FP64 / _FP32 / _BF16X3
   tic(h);
-  launch_syrk(prec, rows, nv, h->d_orows, h->d_kappa, h->d_vec, nsplit, h->d_part, 1.0 / h->m_norm, h->d_S,
-              h->st);
+  launch_center_pack(pl, rows, nv, h->d_orows, h->d_kappa, h->d_vec, h->d_packed, h->st);
+  launch_syrk_packed(pl, nv, h->d_packed, h->d_part, 1.0 / h->m_norm, h->d_S, h->st);
   HIPCHK(h, hipGetLastError());
   h->t[3] = toc(h);
   h->s_valid = 1;
@@ -713,92 +781,73 @@ int rbmpp_get_smat(rbmpp_handle* h, double* s) {
 int rbmpp_sr_step(rbmpp_handle* h, const rbmpp_sr_config* c, double* gamma, rbmpp_sr_report* rep) {
   if (!h || !c || !gamma) return RBMPP_E_ARG;
   if (!h->measured) return fail(h, RBMPP_E_STATE, "sr_step before measure");
+  if (c->refine_iters < 0 || c->refine_iters > 8) return fail(h, RBMPP_E_ARG, "refine_iters must be in [0, 8]");
   const int nv = h->nv, nvr = h->nv_rbm, N = h->N, nsl = h->nsl;
   int rc = build_s(h);
   if (rc) return rc;
+  if ((rc = ensure_fii(h))) return rc;
   hipEvent_t ea, eb;
-  hipEventCreate(&ea);
-  hipEventCreate(&eb);
-  hipEventRecord(ea, h->st);
-  // SR_pdposv.f90:4-6 : g <- g - 2 E <O>
+  HIPCHK(h, hipEventCreate(&ea));
+  HIPCHK(h, hipEventCreate(&eb));
+  HIPCHK(h, hipEventRecord(ea, h->st));
+  // SR_pdposv.f90:4-6 : g <- g - 2 E <O>; s_factor = max(1e-4, s_factor0)
   std::vector<double> g(nv);
   for (int k = 0; k < nv; k++) g[k] = h->m_g[k] - 2.0 * h->m_etot * h->m_o[k];
-  // SR_pdposv.f90:18-40 : block scaling factor from mean positive diagonals
-  std::vector<double> d(nv);
-  launch_diag(nv, h->d_S, h->d_vec + nv, h->st);
-  HIPCHK(h, hipMemcpyAsync(d.data(), h->d_vec + nv, sizeof(double) * nv, hipMemcpyDeviceToHost, h->st));
-  HIPCHK(h, hipStreamSynchronize(h->st));
-  double avR = 0, avP = 0;
-  int i1 = 0, i2 = 0;
-  for (int k = 0; k < nv; k++)
-    if (d[k] > 0.0) {
-      if (k < nvr) { i1++; avR += d[k]; } else { i2++; avP += d[k]; }
-    }
-  if (i1 != nvr || i2 != nsl - nsl / N) {
-    int kbad = -1;
-    for (int k = 0; k < nv && kbad < 0; k++)
-      if (!(d[k] > 0.0) && (k < nvr || !std::isfinite(d[k]) || d[k] < 0.0)) kbad = k;
-    const std::string at = kbad >= 0 ? " (first: S[" + std::to_string(kbad) + "] = " + std::to_string(d[kbad]) + ")" : "";
-    if (i1 != nvr) return fail(h, RBMPP_E_SDIAG, "i1/=Nv_RBM (SR_pdposv.f90:32)" + at);
-    return fail(h, RBMPP_E_SDIAG, "i2 /= N_Slater - N_Slater/N (SR_pdposv.f90:33): " + std::to_string(i2) + " positive" + at);
-  }
-  avR = nvr > 0 ? avR / i1 : 1.0;
-  avP = avP / i2;
-  const double scal = std::sqrt(avR / avP);
-  std::vector<int> fii(nv, 0);
-  for (int i = 0; i < N; i++) fii[h->orb[(size_t)i * N + i] + nv - nsl] = 1;
-  double sav = 0;
-  int cnt = 0;
-  for (int k = 0; k < nv; k++) {
-    const double dk = d[k] * (k >= nvr ? scal * scal : 1.0);
-    if (dk < 0.0) return fail(h, RBMPP_E_SDIAG, "diagonal element of S matrix is not positive");
-    if (dk == 0.0 && !fii[k]) return fail(h, RBMPP_E_SDIAG, "zero S diagonal for other than f_ii");
-    if (dk > 0.0 && fii[k]) return fail(h, RBMPP_E_SDIAG, "positive S diagonal for f_ii");
-    if (dk > 0.0) { sav += dk; cnt++; }
-  }
-  sav /= cnt;
   const double sf = std::max(1e-4, c->s_factor0);
-  for (int k = nvr; k < nv; k++) g[k] *= scal;
-  // A = D S D + Sdiag_av * s_factor I ; L = chol(A)
-  launch_scale_shift(nv, nvr, scal, sav * sf, h->d_S, h->d_A, h->st);
-  HIPCHK(h, hipMemcpyAsync(h->d_L, h->d_A, sizeof(double) * nv * nv, hipMemcpyDeviceToDevice, h->st));
-  HIPCHK(h, hipMemsetAsync(h->d_info, 0, sizeof(int), h->st));
-  launch_cholesky(nv, h->d_L, h->d_info, h->st);
-  HIPCHK(h, hipGetLastError());
-  int info = 0;
-  HIPCHK(h, hipMemcpyAsync(&info, h->d_info, sizeof(int), hipMemcpyDeviceToHost, h->st));
-  HIPCHK(h, hipStreamSynchronize(h->st));
-  if (info != 0) return fail(h, RBMPP_E_NOT_SPD, "Cholesky: matrix not positive definite");
-  // x = A^-1 g with FP64 iterative refinement
-  double* dx = h->d_vec;           // solution
-  double* dr = h->d_vec + nv;      // residual / correction
-  double* dy = h->d_vec + 2 * nv;  // A x
-  HIPCHK(h, hipMemcpyAsync(dx, g.data(), sizeof(double) * nv, hipMemcpyHostToDevice, h->st));
-  launch_chol_solve(nv, h->d_L, dx, h->st);
-  std::vector<double> x(nv), ax(nv);
-  double resid = 0;
-  for (int it = 0; it <= c->refine_iters; it++) {
-    launch_gemv(nv, h->d_A, dx, dy, h->st);
-    HIPCHK(h, hipMemcpyAsync(x.data(), dx, sizeof(double) * nv, hipMemcpyDeviceToHost, h->st));
-    HIPCHK(h, hipMemcpyAsync(ax.data(), dy, sizeof(double) * nv, hipMemcpyDeviceToHost, h->st));
-    HIPCHK(h, hipStreamSynchronize(h->st));
-    double rn = 0, gn = 0;
-    std::vector<double> r(nv);
-    for (int k = 0; k < nv; k++) { r[k] = g[k] - ax[k]; rn += r[k] * r[k]; gn += g[k] * g[k]; }
-    resid = std::sqrt(rn / std::max(gn, 1e-300));
-    if (it == c->refine_iters) break;
-    HIPCHK(h, hipMemcpyAsync(dr, r.data(), sizeof(double) * nv, hipMemcpyHostToDevice, h->st));
-    launch_chol_solve(nv, h->d_L, dr, h->st);
-    std::vector<double> corr(nv);
-    HIPCHK(h, hipMemcpyAsync(corr.data(), dr, sizeof(double) * nv, hipMemcpyDeviceToHost, h->st));
-    HIPCHK(h, hipStreamSynchronize(h->st));
-    for (int k = 0; k < nv; k++) x[k] += corr[k];
-    HIPCHK(h, hipMemcpyAsync(dx, x.data(), sizeof(double) * nv, hipMemcpyHostToDevice, h->st));
+  SrPipeline p;
+  p.nv = nv; p.nvr = nvr; p.refine_iters = c->refine_iters;
+  p.S = h->d_S; p.fii = h->d_fii; p.A = h->d_A; p.L = h->d_L;
+  p.x = h->d_vec; p.r = h->d_vec + nv; p.ax = h->d_vec + 2 * nv;
+  double* d_g = h->d_vec + 3 * nv;
+  p.g = d_g;
+  p.sc = h->d_sc; p.fl = h->d_fl;
+  HIPCHK(h, hipMemcpyAsync(d_g, g.data(), sizeof(double) * nv, hipMemcpyHostToDevice, h->st));
+  HIPCHK(h, hipMemcpyAsync(h->d_sc + 3, &sf, sizeof(double), hipMemcpyHostToDevice, h->st));
+  // the launch sequence (scalars, system, Cholesky, solves, refinement,
+  // lambda) has no host synchronisation; it is captured once into a graph
+  // and replayed (RBMPP_SR_NO_GRAPH runs it eagerly)
+  if (c->flags & RBMPP_SR_NO_GRAPH) {
+    launch_sr_pipeline(p, h->st);
+    HIPCHK(h, hipGetLastError());
+  } else {
+    if (!h->sr_exec || h->sr_graph_iters != c->refine_iters) {
+      if (h->sr_exec) { hipGraphExecDestroy(h->sr_exec); h->sr_exec = nullptr; }
+      if (h->sr_graph) { hipGraphDestroy(h->sr_graph); h->sr_graph = nullptr; }
+      HIPCHK(h, hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
+      launch_sr_pipeline(p, h->st);
+      const hipError_t le = hipGetLastError();
+      HIPCHK(h, hipStreamEndCapture(h->st, &h->sr_graph));
+      HIPCHK(h, le);
+      HIPCHK(h, hipGraphInstantiate(&h->sr_exec, h->sr_graph, nullptr, nullptr, 0));
+      h->sr_graph_iters = c->refine_iters;
+    }
+    HIPCHK(h, hipGraphLaunch(h->sr_exec, h->st));
   }
+  std::vector<double> x(nv);
+  double sc[8];
+  int fl[8];
+  HIPCHK(h, hipMemcpyAsync(x.data(), p.x, sizeof(double) * nv, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(h, hipMemcpyAsync(sc, h->d_sc, sizeof sc, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(h, hipMemcpyAsync(fl, h->d_fl, sizeof fl, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(h, hipEventRecord(eb, h->st));
+  HIPCHK(h, hipEventSynchronize(eb));
+  float ms = 0;
+  hipEventElapsedTime(&ms, ea, eb);
+  h->t[4] = ms * 1e-3;
+  hipEventDestroy(ea);
+  hipEventDestroy(eb);
+  // SR_pdposv.f90:32-33, 78-100 : the structural checks of the diagonal
+  if (fl[0] != nvr) return fail(h, RBMPP_E_SDIAG, "i1/=Nv_RBM (SR_pdposv.f90:32): " + std::to_string(fl[0]) + " positive");
+  if (fl[1] != nsl - nsl / N)
+    return fail(h, RBMPP_E_SDIAG, "i2 /= N_Slater - N_Slater/N (SR_pdposv.f90:33): " + std::to_string(fl[1]) + " positive");
+  if (fl[3] == 1) return fail(h, RBMPP_E_SDIAG, "diagonal element of S matrix is not positive (S[" + std::to_string(fl[2]) + "])");
+  if (fl[3] == 2) return fail(h, RBMPP_E_SDIAG, "zero S diagonal for other than f_ii (S[" + std::to_string(fl[2]) + "])");
+  if (fl[3] == 3) return fail(h, RBMPP_E_SDIAG, "positive S diagonal for f_ii (S[" + std::to_string(fl[2]) + "])");
+  if (fl[4] != 0) return fail(h, RBMPP_E_NOT_SPD, "Cholesky: matrix not positive definite at pivot " + std::to_string(fl[4]));
+  const double scal = sc[0], sav = sc[2];
+  const double resid = std::sqrt(sc[5] / std::max(sc[6], 1e-300));
   // SR_pdposv.f90:251-311 : lambda = sqrt(x^T A x), trust region 5/L
-  double lam = 0;
-  for (int k = 0; k < nv; k++) lam += x[k] * ax[k];
-  lam = std::sqrt(lam) * 5.0 / (double)h->L;
+  const double lam = std::sqrt(sc[4]) * 5.0 / (double)h->L;
   if (lam > 1.0)
     for (int k = 0; k < nv; k++) x[k] /= lam;
   for (int k = nvr; k < nv; k++) x[k] *= scal;
@@ -817,13 +866,6 @@ int rbmpp_sr_step(rbmpp_handle* h, const rbmpp_sr_config* c, double* gamma, rbmp
   for (int k = nv - nsl; k < nv; k++) r1 += std::fabs(gamma[k]);
   r1 /= nsl;
   for (int k = nv - nsl; k < nv; k++) gamma[k] = gamma[k] * c->slater_abs_av / r1;
-  hipEventRecord(eb, h->st);
-  hipEventSynchronize(eb);
-  float ms = 0;
-  hipEventElapsedTime(&ms, ea, eb);
-  h->t[4] = ms * 1e-3;
-  hipEventDestroy(ea);
-  hipEventDestroy(eb);
   if (rep) {
     rep->lambda = lam; rep->delw_max = dmax; rep->scaling = scal; rep->sdiag_av = sav; rep->s_factor = sf;
     rep->residual = resid; rep->mean_abs_f = r1; rep->k_nonzero = knz; rep->reserved = 0;
@@ -915,11 +957,38 @@ int rbmpp_get_samples(const rbmpp_handle* h0, int32_t* x, double* psi) {
 int rbmpp_get_measured(const rbmpp_handle* h0, double* psi, double* eloc, double* rw) {
   rbmpp_handle* h = const_cast<rbmpp_handle*>(h0);
   if (!h) return RBMPP_E_ARG;
-  const size_t n = (size_t)h->n_stored;
-  if (n > h->meas_cap || !h->d_mpsi) return fail(h, RBMPP_E_STATE, "get_measured: no measurement");
+  const size_t n = (size_t)h->n_meas;
+  if (n == 0 || n > h->meas_cap || !h->d_mpsi) return fail(h, RBMPP_E_STATE, "get_measured: no measurement");
   if (psi) HIPCHK(h, hipMemcpy(psi, h->d_mpsi, sizeof(double2) * n, hipMemcpyDeviceToHost));
   if (eloc) HIPCHK(h, hipMemcpy(eloc, h->d_eloc, sizeof(double2) * n, hipMemcpyDeviceToHost));
   if (rw) HIPCHK(h, hipMemcpy(rw, h->d_rw, sizeof(double) * n, hipMemcpyDeviceToHost));
+  return RBMPP_OK;
+}
+
+int rbmpp_set_measured(rbmpp_handle* h, int64_t n, const double* psi, const double* eloc, const double* rw,
+                       const double* orows, uint64_t seed) {
+  if (!h || n <= 0 || n > (int64_t)(INT32_MAX / 2)) return RBMPP_E_ARG;
+  if (orows && (!psi || !eloc || !rw)) return fail(h, RBMPP_E_ARG, "set_measured: psi, eloc and rw are needed with orows");
+  int rc = ensure_measure(h, (size_t)n);
+  if (rc) return rc;
+  h->fused_valid = 0;
+  const size_t rows = (size_t)n * (h->real_only ? 1 : 2);
+  if (orows) {
+    HIPCHK(h, hipMemcpyAsync(h->d_mpsi, psi, sizeof(double2) * n, hipMemcpyHostToDevice, h->st));
+    HIPCHK(h, hipMemcpyAsync(h->d_eloc, eloc, sizeof(double2) * n, hipMemcpyHostToDevice, h->st));
+    HIPCHK(h, hipMemcpyAsync(h->d_rw, rw, sizeof(double) * n, hipMemcpyHostToDevice, h->st));
+    HIPCHK(h, hipMemcpyAsync(h->d_orows, orows, sizeof(double) * rows * h->nv, hipMemcpyHostToDevice, h->st));
+  } else {
+    if ((rc = ensure_fii(h))) return rc;
+    hipLaunchKernelGGL(k_synth_measurement, dim3(4096), dim3(256), 0, h->st, n, h->nv, h->real_only ? 0 : 1, seed,
+                       (const unsigned char*)h->d_fii, h->d_mpsi, h->d_eloc, h->d_rw, h->d_orows);
+    HIPCHK(h, hipGetLastError());
+  }
+  h->n_meas = n;
+  rc = reduce_measure(h, (int)n, true);
+  if (rc) return rc;
+  h->measured = 1;
+  h->s_valid = 0;
   return RBMPP_OK;
 }
 

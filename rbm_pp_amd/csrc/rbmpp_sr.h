@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "rbmpp_device.h"
 
 void launch_scalar_partials(int ns, const double2* psi, const double* rw, const double2* eloc, double* part,
@@ -11,13 +13,38 @@ void launch_gvec_partials(int ns, int nv, int cplx_rows, const double* orows, co
                           const double* rw, const double2* eloc, double* part, int nchunk, hipStream_t st);
 void launch_row_kappa(int ns, int cplx_rows, const double2* psi, const double* rw, double* kappa,
                       hipStream_t st);
-int syrk_tiles(int nv);
-int syrk_tile_size();
-void launch_syrk(int precision, int nrows, int nv, const double* orows, const double* kappa, const double* obar,
-                 int nsplit, double* part, double inv_norm, double* S, hipStream_t st);
-void launch_scale_shift(int nv, int nv_rbm, double scal, double shift, const double* S, double* A,
-                        hipStream_t st);
-void launch_diag(int nv, const double* S, double* d, hipStream_t st);
+// S build from the centred, packed O rows (rbmpp_syrk.hip)
+struct SyrkPlan {
+  int mode;          // RBMPP_S_FP64 / _FP32 / _BF16X3
+  int kb;            // rows per K-block (16 FP64, 32 FP32 / BF16)
+  int nvp;           // nv rounded up to the 128-column tile
+  int ntile, npair;  // column tiles, upper-triangle tile pairs
+  int nkb;           // K-blocks
+  int nsplit, kb_per_split;
+  size_t packed_bytes;   // nkb * nvp * 128
+  size_t part_doubles;   // nsplit * npair * 128 * 128
+};
+SyrkPlan syrk_plan(int precision, int nrows, int nv, int cus);
+void launch_center_pack(const SyrkPlan& pl, int nrows, int nv, const double* orows, const double* kappa,
+                        const double* obar, unsigned char* packed, hipStream_t st);
+void launch_syrk_packed(const SyrkPlan& pl, int nv, const unsigned char* packed, double* part, double inv_norm,
+                        double* S, hipStream_t st);
 void launch_gemv(int n, const double* A, const double* x, double* y, hipStream_t st);
 void launch_cholesky(int n, double* A, int* info, hipStream_t st);
+// L L^T x = b in place (blocked forward and back substitution)
 void launch_chol_solve(int n, const double* L, double* x, hipStream_t st);
+
+// The SR system on the device (scalars, D S D + shift I, Cholesky, solve,
+// refinement, lambda and residual), every launch on one stream with no host
+// synchronisation, so that the sequence can be captured in a graph.
+struct SrPipeline {
+  int nv, nvr, refine_iters;
+  const double* S;           // centred S, row-major nv x nv
+  const unsigned char* fii;  // [nv] 1 for the f_ii parameters
+  const double* g;           // [nv] g - 2 E <O> (unscaled)
+  double *A, *L;             // nv x nv: D S D + shift I and its factor
+  double *x, *r, *ax;        // [nv]: solution, residual / correction, A x
+  double* sc;                // [8] scalars (see k_sr_prepare)
+  int* fl;                   // [8] flags and Cholesky info
+};
+void launch_sr_pipeline(const SrPipeline& p, hipStream_t st);

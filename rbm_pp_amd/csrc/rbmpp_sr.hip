@@ -1,10 +1,8 @@
 // Reductions over the measured samples and the linear algebra of the SR step
-// on gfx950: centred S build (SYRK over the O rows on the matrix cores, FP64
-// centring, FP64 / FP32 / split-BF16x3 products), PP-block scaling and diagonal shift, blocked FP64 Cholesky
+// on gfx950: PP-block scaling and diagonal shift, blocked FP64 Cholesky
 // (replaces ScaLAPACK pdposv, SR_pdposv.f90:232), triangular solves,
-// residual / lambda products.
+// residual / lambda products.  The S build itself is in rbmpp_syrk.hip.
 #include "rbmpp_sr.h"
-#include <type_traits>
 
 // ---------------------------------------------------------------- sums
 // main.f90:351-361 operands: per-block partials of
@@ -78,237 +76,6 @@ __global__ void k_row_kappa(int ns, int cplx_rows, const double2* __restrict__ p
   }
 }
 
-// ---------------------------------------------------------------- S build
-// S = <O O^T>/norm - <O><O>^T (SR_pdposv.f90:11-15) as a SYRK on the matrix
-// cores over the centred rows c = O_row - kappa_row * obar.  The centring is
-// done in FP64 while a 32-row stage is staged into LDS; the products then run
-// in the configured precision:
-//   MODE 0: FP64 MFMA (v_mfma_f64_16x16x4_f64), FP64 accumulation;
-//   MODE 1: FP32 MFMA (v_mfma_f32_16x16x4_f32), FP32 accumulation;
-//   MODE 2: split BF16 x 3 (v_mfma_f32_16x16x32_bf16): c = hi + lo with
-//           hi = bf16(c), lo = bf16(c - hi); c c' ~ hi hi' + hi lo' + lo hi',
-//           FP32 accumulation.
-// FP32 accumulators are added into FP64 ones every SYRK_FLUSH stages; row
-// splits (grid.y) are summed in FP64 by k_syrk_finish.  grid.x enumerates the
-// 128 x 128 upper-triangle tile pairs; the 4 waves own 64 x 64 quadrants of
-// 16 x 16 MFMA blocks.
-constexpr int ST = 128, SK = 32, SYRK_FLUSH = 32;
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef double f64x4 __attribute__((ext_vector_type(4)));
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-
-template <int MODE>
-struct SyrkLds {
-  // MODE 0/1: panels [k][column] (+pad); MODE 2: hi/lo panels [column][k]
-  static constexpr int KP = SK + 8;
-  typedef typename std::conditional<MODE == 0, double, float>::type T;
-  union {
-    struct { T a[SK][ST + 4]; T b[SK][ST + 4]; } kc;
-    struct { __bf16 ahi[ST][KP], alo[ST][KP], bhi[ST][KP], blo[ST][KP]; } bf;
-  };
-};
-
-template <int MODE>
-__global__ __launch_bounds__(256) void k_syrk_mfma(int nrows, int nv, const double* __restrict__ orows,
-                                                   const double* __restrict__ kappa,
-                                                   const double* __restrict__ obar, int ntile, double* part) {
-  __shared__ SyrkLds<MODE> L;
-  int p = blockIdx.x, ti = 0;
-  while (p >= ntile - ti) { p -= ntile - ti; ti++; }
-  const int tj = ti + p;
-  const bool diag = ti == tj;
-  const int chunk = ((nrows + gridDim.y - 1) / gridDim.y + SK - 1) / SK * SK;
-  const int r0 = blockIdx.y * chunk, r1 = min(nrows, r0 + chunk);
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wi = wave >> 1, wj = wave & 1;
-  // staging: column c of both panels, rows hk*16 .. hk*16+15 of the stage
-  const int c = tid & (ST - 1), hk = tid >> 7;
-  const int ci = ti * ST + c, cj = tj * ST + c;
-  const double oa = ci < nv ? obar[ci] : 0.0, ob = cj < nv ? obar[cj] : 0.0;
-
-  f64x4 acc64[4][4];
-  f32x4 acc32[4][4];
-#pragma unroll
-  for (int x = 0; x < 4; x++)
-#pragma unroll
-    for (int y = 0; y < 4; y++) {
-      acc64[x][y] = f64x4{0.0, 0.0, 0.0, 0.0};
-      acc32[x][y] = f32x4{0.f, 0.f, 0.f, 0.f};
-    }
-  int nstage = 0;
-  // the stage's rows are loaded one stage ahead, so the global loads of
-  // stage s+1 are in flight while the matrix cores work on stage s
-  double va[16], vb[16];
-  auto load_stage = [&](int k0) {
-#pragma unroll
-    for (int u = 0; u < 16; u++) {
-      const int row = k0 + hk * 16 + u;
-      double x = 0.0, y = 0.0;
-      if (row < r1) {
-        const double kap = kappa[row];
-        const double* orow = orows + (size_t)row * nv;
-        if (ci < nv) x = orow[ci] - kap * oa;
-        if (!diag && cj < nv) y = orow[cj] - kap * ob;
-      }
-      va[u] = x;
-      vb[u] = y;
-    }
-  };
-  if (r0 < r1) load_stage(r0);
-  for (int k0 = r0; k0 < r1; k0 += SK) {
-    // ---- stage rows k0..k0+SK-1 (centred in FP64, converted once)
-    if constexpr (MODE == 2) {
-#pragma unroll
-      for (int g = 0; g < 2; g++) {
-        bf16x8 h, l, hb, lb;
-#pragma unroll
-        for (int t = 0; t < 8; t++) {
-          const double x = va[8 * g + t], y = vb[8 * g + t];
-          const __bf16 xh = (__bf16)(float)x, yh = (__bf16)(float)y;
-          h[t] = xh;
-          l[t] = (__bf16)(float)(x - (double)(float)xh);
-          hb[t] = yh;
-          lb[t] = (__bf16)(float)(y - (double)(float)yh);
-        }
-        *(bf16x8*)&L.bf.ahi[c][hk * 16 + 8 * g] = h;
-        *(bf16x8*)&L.bf.alo[c][hk * 16 + 8 * g] = l;
-        if (!diag) {
-          *(bf16x8*)&L.bf.bhi[c][hk * 16 + 8 * g] = hb;
-          *(bf16x8*)&L.bf.blo[c][hk * 16 + 8 * g] = lb;
-        }
-      }
-    } else {
-      typedef typename SyrkLds<MODE>::T T;
-#pragma unroll
-      for (int u = 0; u < 16; u++) {
-        L.kc.a[hk * 16 + u][c] = (T)va[u];
-        if (!diag) L.kc.b[hk * 16 + u][c] = (T)vb[u];
-      }
-    }
-    __syncthreads();
-    if (k0 + SK < r1) load_stage(k0 + SK);
-    // ---- products of this wave's 4 x 4 blocks
-    const int ib = wi * 64 + (lane & 15), jb = wj * 64 + (lane & 15);
-    if constexpr (MODE == 2) {
-      const int kk = 8 * (lane >> 4);
-      bf16x8 ah[4], al[4], bh[4], bl[4];
-#pragma unroll
-      for (int x = 0; x < 4; x++) {
-        ah[x] = *(const bf16x8*)&L.bf.ahi[ib + 16 * x][kk];
-        al[x] = *(const bf16x8*)&L.bf.alo[ib + 16 * x][kk];
-        bh[x] = diag ? *(const bf16x8*)&L.bf.ahi[jb + 16 * x][kk] : *(const bf16x8*)&L.bf.bhi[jb + 16 * x][kk];
-        bl[x] = diag ? *(const bf16x8*)&L.bf.alo[jb + 16 * x][kk] : *(const bf16x8*)&L.bf.blo[jb + 16 * x][kk];
-      }
-#pragma unroll
-      for (int x = 0; x < 4; x++)
-#pragma unroll
-        for (int y = 0; y < 4; y++) {
-          acc32[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[x], bh[y], acc32[x][y], 0, 0, 0);
-          acc32[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bl[y], acc32[x][y], 0, 0, 0);
-          acc32[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bh[y], acc32[x][y], 0, 0, 0);
-        }
-    } else {
-#pragma unroll
-      for (int s = 0; s < SK / 4; s++) {
-        const int kr = 4 * s + (lane >> 4);
-        typename SyrkLds<MODE>::T av[4], bv[4];
-#pragma unroll
-        for (int x = 0; x < 4; x++) {
-          av[x] = L.kc.a[kr][ib + 16 * x];
-          bv[x] = diag ? L.kc.a[kr][jb + 16 * x] : L.kc.b[kr][jb + 16 * x];
-        }
-#pragma unroll
-        for (int x = 0; x < 4; x++)
-#pragma unroll
-          for (int y = 0; y < 4; y++) {
-            if constexpr (MODE == 0)
-              acc64[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[x], bv[y], acc64[x][y], 0, 0, 0);
-            else
-              acc32[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[x], bv[y], acc32[x][y], 0, 0, 0);
-          }
-      }
-    }
-    if constexpr (MODE != 0) {
-      if (++nstage == SYRK_FLUSH) {
-        nstage = 0;
-#pragma unroll
-        for (int x = 0; x < 4; x++)
-#pragma unroll
-          for (int y = 0; y < 4; y++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-              acc64[x][y][r] += (double)acc32[x][y][r];
-              acc32[x][y][r] = 0.f;
-            }
-      }
-    }
-    __syncthreads();
-  }
-  if constexpr (MODE != 0) {
-#pragma unroll
-    for (int x = 0; x < 4; x++)
-#pragma unroll
-      for (int y = 0; y < 4; y++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) acc64[x][y][r] += (double)acc32[x][y][r];
-  }
-  // ---- FP64 partial tile of this split.  Output element (row, col) of a
-  // 16 x 16 block: FP32 MFMA lane l, register r -> (4 (l/16) + r, l % 16);
-  // FP64 MFMA -> (l/16 + 4 r, l % 16).
-  const size_t tile = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
-  double* out = part + tile * ST * ST;
-#pragma unroll
-  for (int x = 0; x < 4; x++)
-#pragma unroll
-    for (int y = 0; y < 4; y++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int i = wi * 64 + 16 * x + (MODE == 0 ? (lane >> 4) + 4 * r : 4 * (lane >> 4) + r);
-        const int j = wj * 64 + 16 * y + (lane & 15);
-        out[i * ST + j] = acc64[x][y][r];
-      }
-}
-
-// Sum the split partials of each tile pair, scale by 1/norm and write both
-// triangles of S (row-major nv x nv).
-__global__ void k_syrk_finish(int nv, int ntile, int npairs, int nsplit, const double* __restrict__ part,
-                              double inv_norm, double* S) {
-  int p = blockIdx.x, ti = 0;
-  while (p >= ntile - ti) { p -= ntile - ti; ti++; }
-  const int tj = ti + p;
-  for (int e = threadIdx.x; e < ST * ST; e += blockDim.x) {
-    const int a = e / ST, b = e - a * ST;
-    const int i = ti * ST + a, j = tj * ST + b;
-    if (i >= nv || j >= nv) continue;
-    double t = 0;
-    for (int z = 0; z < nsplit; z++) t += part[((size_t)z * npairs + blockIdx.x) * ST * ST + e];
-    t *= inv_norm;
-    S[(size_t)i * nv + j] = t;
-    S[(size_t)j * nv + i] = t;
-  }
-}
-
-// ------------------------------------------------------- SR system set-up
-// Ssh = D S D + shift I with D = diag(1 (RBM), scal (PP)) (SR_pdposv.f90:56-73,
-// 167-170); returns into A (row-major).
-__global__ void k_scale_shift(int nv, int nv_rbm, double scal, double shift, const double* __restrict__ S,
-                              double* A) {
-  const size_t n2 = (size_t)nv * nv;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n2; e += (size_t)gridDim.x * blockDim.x) {
-    const int i = (int)(e / nv), j = (int)(e - (size_t)i * nv);
-    double v = S[e];
-    if (i >= nv_rbm) v *= scal;
-    if (j >= nv_rbm) v *= scal;
-    if (i == j) v += shift;
-    A[e] = v;
-  }
-}
-
-__global__ void k_diag(int nv, const double* __restrict__ S, double* d) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nv) d[i] = S[(size_t)i * nv + i];
-}
-
 // y = A x (row-major, one wave per row)
 __global__ void k_gemv(int n, const double* __restrict__ A, const double* __restrict__ x, double* y) {
   const int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -322,7 +89,7 @@ __global__ void k_gemv(int n, const double* __restrict__ A, const double* __rest
 
 // ------------------------------------------------------- blocked Cholesky
 // Right-looking, lower triangle of a row-major SPD matrix, block NB=32.
-#define CH_NB 32
+#define CH_NB 64
 
 // factor the diagonal block [k0, k0+nb)
 __global__ __launch_bounds__(256) void k_potrf_diag(int n, int k0, double* A, int* info) {
@@ -335,7 +102,7 @@ __global__ __launch_bounds__(256) void k_potrf_diag(int n, int k0, double* A, in
   __syncthreads();
   for (int j = 0; j < nb; j++) {
     if (B[j][j] <= 0.0 || !(B[j][j] == B[j][j])) {
-      if (threadIdx.x == 0) *info = k0 + j + 1;
+      if (threadIdx.x == 0 && *info == 0) *info = k0 + j + 1;  // first failing pivot
       return;
     }
     __syncthreads();
@@ -417,38 +184,180 @@ __global__ __launch_bounds__(256) void k_syrk_trailing(int n, int k0, int t0, in
     }
 }
 
-// L y = b then L^T x = y, single workgroup, in place in x
-__global__ __launch_bounds__(1024) void k_chol_solve(int n, const double* __restrict__ L, double* x) {
-  __shared__ double red[16];
-  __shared__ double xi;
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  for (int i = 0; i < n; i++) {
-    double t = 0;
-    for (int k = threadIdx.x; k < i; k += blockDim.x) t = fma(L[(size_t)i * n + k], x[k], t);
-    t = wave_sum(t);
-    if (lane == 0) red[wid] = t;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double s = 0;
-      for (int w = 0; w < nw; w++) s += red[w];
-      x[i] = (x[i] - s) / L[(size_t)i * n + i];
+// ------------------------------------------------- device SR pipeline
+// Scalars of the SR system, from the diagonal of S (SR_pdposv.f90:18-40,
+// 78-140): block scale scal = sqrt(mean positive RBM diagonal / mean positive
+// PP diagonal), Sdiag_av = mean positive diagonal of D S D, shift =
+// Sdiag_av * s_factor; and the structural checks of :32-33, 78-100 as flags.
+// One workgroup, partial sums per thread then a fixed-order tree (the result
+// does not depend on timing).
+//   sc: [0] scal [1] shift [2] Sdiag_av [3] s_factor (input) [4] lambda^2
+//       [5] |r|^2 [6] |g|^2
+//   fl: [0] i1 [1] i2 [2] first bad index [3] kind (1 negative, 2 zero not
+//       f_ii, 3 positive f_ii) [4] Cholesky info
+__global__ __launch_bounds__(1024) void k_sr_prepare(int nv, int nvr, const double* __restrict__ S,
+                                                     const unsigned char* __restrict__ fii, double* sc, int* fl) {
+  __shared__ double rs[4][1024];
+  __shared__ int ri[2][1024];
+  __shared__ int bad;
+  const int tid = threadIdx.x;
+  if (tid == 0) bad = nv;
+  double sR = 0, sP = 0, cR = 0, cP = 0;
+  int i1 = 0, i2 = 0;
+  __syncthreads();
+  for (int k = tid; k < nv; k += 1024) {
+    const double d = S[(size_t)k * nv + k];
+    const bool f = fii[k] != 0;
+    if (d > 0.0) {
+      if (k < nvr) { sR += d; cR += 1; i1++; } else { sP += d; cP += 1; i2++; }
+    }
+    if (d < 0.0 || !(d == d) || (d == 0.0 && !f) || (d > 0.0 && f)) atomicMin(&bad, k);
+  }
+  rs[0][tid] = sR; rs[1][tid] = sP; rs[2][tid] = cR; rs[3][tid] = cP;
+  ri[0][tid] = i1; ri[1][tid] = i2;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if (tid < o) {
+      for (int q = 0; q < 4; q++) rs[q][tid] += rs[q][tid + o];
+      ri[0][tid] += ri[0][tid + o];
+      ri[1][tid] += ri[1][tid + o];
     }
     __syncthreads();
   }
-  for (int i = n - 1; i >= 0; i--) {
-    double t = 0;
-    for (int k = i + 1 + threadIdx.x; k < n; k += blockDim.x) t = fma(L[(size_t)k * n + i], x[k], t);
-    t = wave_sum(t);
-    if (lane == 0) red[wid] = t;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double s = 0;
-      for (int w = 0; w < nw; w++) s += red[w];
-      x[i] = (x[i] - s) / L[(size_t)i * n + i];
+  if (tid == 0) {
+    const double avR = nvr > 0 ? rs[0][0] / rs[2][0] : 1.0;
+    const double avP = rs[1][0] / rs[3][0];
+    const double scal = sqrt(avR / avP);
+    const double sav = (rs[0][0] + scal * scal * rs[1][0]) / (rs[2][0] + rs[3][0]);
+    sc[0] = scal;
+    sc[2] = sav;
+    sc[1] = sav * sc[3];
+    fl[0] = ri[0][0];
+    fl[1] = ri[1][0];
+    fl[2] = bad < nv ? bad : -1;
+    int kind = 0;
+    if (bad < nv) {
+      const double d = S[(size_t)bad * nv + bad];
+      kind = (d < 0.0 || !(d == d)) ? 1 : (d == 0.0 ? 2 : 3);
     }
+    fl[3] = kind;
+    fl[4] = 0;  // Cholesky info, set by k_potrf_diag
+  }
+}
+
+// A = L = D S D + shift I, D = diag(1 (RBM), scal (PP)) (SR_pdposv.f90:56-73,
+// 167-170); x = D g
+__global__ void k_sr_system(int nv, int nvr, const double* __restrict__ sc, const double* __restrict__ S,
+                            double* __restrict__ A, double* __restrict__ L, const double* __restrict__ g,
+                            double* __restrict__ x) {
+  const double scal = sc[0], shift = sc[1];
+  const size_t n2 = (size_t)nv * nv;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n2; e += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e / nv), j = (int)(e - (size_t)i * nv);
+    double v = S[e];
+    if (i >= nvr) v *= scal;
+    if (j >= nvr) v *= scal;
+    if (i == j) v += shift;
+    A[e] = v;
+    L[e] = v;
+    if (e < (size_t)nv) x[e] = e >= (size_t)nvr ? g[e] * scal : g[e];
+  }
+}
+
+// Forward substitution on the diagonal block [k0, k0+nb) of the lower factor
+// (one wave, lane i owns row k0+i; the rows below were already updated)
+__global__ __launch_bounds__(64) void k_trsv_lower_diag(int n, int k0, const double* __restrict__ L, double* x) {
+  const int nb = min(CH_NB, n - k0), i = threadIdx.x;
+  double xi = i < nb ? x[k0 + i] : 0.0;
+  for (int j = 0; j < nb; j++) {
+    double xj = __shfl(xi, j, 64);
+    xj /= L[(size_t)(k0 + j) * n + k0 + j];
+    if (i == j) xi = xj;
+    if (i > j && i < nb) xi = fma(-L[(size_t)(k0 + i) * n + k0 + j], xj, xi);
+  }
+  if (i < nb) x[k0 + i] = xi;
+}
+
+// x_i -= sum_{j in block} L(i, j) x_j for the rows below the block
+__global__ __launch_bounds__(256) void k_trsv_lower_update(int n, int k0, const double* __restrict__ L, double* x) {
+  __shared__ double xb[CH_NB];
+  const int nb = min(CH_NB, n - k0);
+  if (threadIdx.x < nb) xb[threadIdx.x] = x[k0 + threadIdx.x];
+  __syncthreads();
+  const int i = k0 + nb + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* row = L + (size_t)i * n + k0;
+  double t = 0.0;
+  for (int j = 0; j < nb; j++) t = fma(row[j], xb[j], t);
+  x[i] -= t;
+}
+
+// Back substitution L^T x = y on the diagonal block (lane i owns row k0+i)
+__global__ __launch_bounds__(64) void k_trsv_upper_diag(int n, int k0, const double* __restrict__ L, double* x) {
+  const int nb = min(CH_NB, n - k0), i = threadIdx.x;
+  double xi = i < nb ? x[k0 + i] : 0.0;
+  for (int j = nb - 1; j >= 0; j--) {
+    double xj = __shfl(xi, j, 64);
+    xj /= L[(size_t)(k0 + j) * n + k0 + j];
+    if (i == j) xi = xj;
+    if (i < j) xi = fma(-L[(size_t)(k0 + j) * n + k0 + i], xj, xi);  // L^T(i, j) = L(j, i)
+  }
+  if (i < nb) x[k0 + i] = xi;
+}
+
+// x_i -= sum_{j in block} L(j, i) x_j for the rows above the block
+__global__ __launch_bounds__(256) void k_trsv_upper_update(int n, int k0, const double* __restrict__ L, double* x) {
+  __shared__ double xb[CH_NB];
+  const int nb = min(CH_NB, n - k0);
+  if (threadIdx.x < nb) xb[threadIdx.x] = x[k0 + threadIdx.x];
+  __syncthreads();
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= k0) return;
+  double t = 0.0;
+  for (int j = 0; j < nb; j++) t = fma(L[(size_t)(k0 + j) * n + i], xb[j], t);
+  x[i] -= t;
+}
+
+// r = D g - A x (one wave per row); with acc: x += 0 (plain residual)
+__global__ void k_residual(int n, int nvr, const double* __restrict__ sc, const double* __restrict__ A,
+                           const double* __restrict__ x, const double* __restrict__ g, double* __restrict__ r) {
+  const int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  double t = 0;
+  for (int j = lane; j < n; j += 64) t = fma(A[(size_t)row * n + j], x[j], t);
+  t = wave_sum(t);
+  if (lane == 0) r[row] = (row >= nvr ? g[row] * sc[0] : g[row]) - t;
+}
+
+__global__ void k_axpy1(int n, const double* __restrict__ r, double* __restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] += r[i];
+}
+
+// lambda^2 = x.(A x) and |r|^2, |Dg|^2 with r = D g - A x; ax holds A x.
+// One workgroup, fixed-order tree.
+__global__ __launch_bounds__(1024) void k_sr_final(int n, int nvr, double* __restrict__ sc,
+                                                   const double* __restrict__ x, const double* __restrict__ ax,
+                                                   const double* __restrict__ g) {
+  __shared__ double rs[3][1024];
+  const int tid = threadIdx.x;
+  const double scal = sc[0];
+  double l2 = 0, r2 = 0, g2 = 0;
+  for (int k = tid; k < n; k += 1024) {
+    const double gk = k >= nvr ? g[k] * scal : g[k];
+    l2 = fma(x[k], ax[k], l2);
+    r2 = fma(gk - ax[k], gk - ax[k], r2);
+    g2 = fma(gk, gk, g2);
+  }
+  rs[0][tid] = l2; rs[1][tid] = r2; rs[2][tid] = g2;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if (tid < o)
+      for (int q = 0; q < 3; q++) rs[q][tid] += rs[q][tid + o];
     __syncthreads();
   }
-  (void)xi;
+  if (tid == 0) { sc[4] = rs[0][0]; sc[5] = rs[1][0]; sc[6] = rs[2][0]; }
 }
 
 // ------------------------------------------------------------ launchers
@@ -467,27 +376,6 @@ void launch_row_kappa(int ns, int cplx_rows, const double2* psi, const double* r
                       hipStream_t st) {
   hipLaunchKernelGGL(k_row_kappa, dim3((ns + 255) / 256), dim3(256), 0, st, ns, cplx_rows, psi, rw, kappa);
 }
-int syrk_tiles(int nv) { return (nv + ST - 1) / ST; }
-int syrk_tile_size() { return ST; }
-void launch_syrk(int precision, int nrows, int nv, const double* orows, const double* kappa, const double* obar,
-                 int nsplit, double* part, double inv_norm, double* S, hipStream_t st) {
-  const int nt = syrk_tiles(nv), np = nt * (nt + 1) / 2;
-  const dim3 grid(np, nsplit);
-  if (precision == 2)
-    hipLaunchKernelGGL(k_syrk_mfma<2>, grid, dim3(256), 0, st, nrows, nv, orows, kappa, obar, nt, part);
-  else if (precision == 1)
-    hipLaunchKernelGGL(k_syrk_mfma<1>, grid, dim3(256), 0, st, nrows, nv, orows, kappa, obar, nt, part);
-  else
-    hipLaunchKernelGGL(k_syrk_mfma<0>, grid, dim3(256), 0, st, nrows, nv, orows, kappa, obar, nt, part);
-  hipLaunchKernelGGL(k_syrk_finish, dim3(np), dim3(256), 0, st, nv, nt, np, nsplit, part, inv_norm, S);
-}
-void launch_scale_shift(int nv, int nv_rbm, double scal, double shift, const double* S, double* A,
-                        hipStream_t st) {
-  hipLaunchKernelGGL(k_scale_shift, dim3(1024), dim3(256), 0, st, nv, nv_rbm, scal, shift, S, A);
-}
-void launch_diag(int nv, const double* S, double* d, hipStream_t st) {
-  hipLaunchKernelGGL(k_diag, dim3((nv + 255) / 256), dim3(256), 0, st, nv, S, d);
-}
 void launch_gemv(int n, const double* A, const double* x, double* y, hipStream_t st) {
   hipLaunchKernelGGL(k_gemv, dim3((n + 3) / 4), dim3(256), 0, st, n, A, x, y);
 }
@@ -503,5 +391,34 @@ void launch_cholesky(int n, double* A, int* info, hipStream_t st) {
   }
 }
 void launch_chol_solve(int n, const double* L, double* x, hipStream_t st) {
-  hipLaunchKernelGGL(k_chol_solve, dim3(1), dim3(1024), 0, st, n, L, x);
+  // blocked: per block of CH_NB rows a one-wave substitution on the diagonal
+  // block, then a multi-workgroup update of the remaining rows
+  for (int k0 = 0; k0 < n; k0 += CH_NB) {
+    hipLaunchKernelGGL(k_trsv_lower_diag, dim3(1), dim3(64), 0, st, n, k0, L, x);
+    const int rest = n - k0 - CH_NB;
+    if (rest > 0) hipLaunchKernelGGL(k_trsv_lower_update, dim3((rest + 255) / 256), dim3(256), 0, st, n, k0, L, x);
+  }
+  const int nblk = (n + CH_NB - 1) / CH_NB;
+  for (int b = nblk - 1; b >= 0; b--) {
+    const int k0 = b * CH_NB;
+    hipLaunchKernelGGL(k_trsv_upper_diag, dim3(1), dim3(64), 0, st, n, k0, L, x);
+    if (k0 > 0) hipLaunchKernelGGL(k_trsv_upper_update, dim3((k0 + 255) / 256), dim3(256), 0, st, n, k0, L, x);
+  }
+}
+
+void launch_sr_pipeline(const SrPipeline& p, hipStream_t st) {
+  const int n = p.nv;
+  hipLaunchKernelGGL(k_sr_prepare, dim3(1), dim3(1024), 0, st, n, p.nvr, p.S, p.fii, p.sc, p.fl);
+  hipLaunchKernelGGL(k_sr_system, dim3(1024), dim3(256), 0, st, n, p.nvr, (const double*)p.sc, p.S, p.A, p.L, p.g,
+                     p.x);
+  launch_cholesky(n, p.L, p.fl + 4, st);
+  launch_chol_solve(n, p.L, p.x, st);
+  for (int it = 0; it < p.refine_iters; it++) {
+    hipLaunchKernelGGL(k_residual, dim3((n + 3) / 4), dim3(256), 0, st, n, p.nvr, (const double*)p.sc, p.A, p.x,
+                       p.g, p.r);
+    launch_chol_solve(n, p.L, p.r, st);
+    hipLaunchKernelGGL(k_axpy1, dim3((n + 255) / 256), dim3(256), 0, st, n, p.r, p.x);
+  }
+  hipLaunchKernelGGL(k_gemv, dim3((n + 3) / 4), dim3(256), 0, st, n, p.A, p.x, p.ax);
+  hipLaunchKernelGGL(k_sr_final, dim3(1), dim3(1024), 0, st, n, p.nvr, p.sc, p.x, p.ax, p.g);
 }

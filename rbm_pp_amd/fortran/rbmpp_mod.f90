@@ -26,6 +26,7 @@ module rbmpp_mod
   integer(c_int), parameter, public :: RBMPP_S_FP64 = 0, RBMPP_S_FP32 = 1, RBMPP_S_BF16X3 = 2
   integer(c_int), parameter, public :: RBMPP_PROJ_NORMAL = 0, RBMPP_PROJ_TRANSVERSE = 1, &
                                         RBMPP_PROJ_NORMAL_EXACT = 2
+  integer(c_int), parameter, public :: RBMPP_SR_NO_GRAPH = 1
 
   ! struct rbmpp_config (field order and kinds as in rbmpp.h)
   type, bind(C), public :: rbmpp_config
@@ -39,7 +40,7 @@ module rbmpp_mod
 
   type, bind(C), public :: rbmpp_sr_config
     real(c_double)     :: s_factor0, delta_tau, max_change, slater_abs_av
-    integer(c_int32_t) :: refine_iters, reserved
+    integer(c_int32_t) :: refine_iters, flags
   end type
 
   type, bind(C), public :: rbmpp_sr_report
@@ -53,7 +54,7 @@ module rbmpp_mod
   public :: rbmpp_measure_energy, rbmpp_measure_correlation
   public :: rbmpp_set_samples, rbmpp_get_samples, rbmpp_n_samples
   public :: rbmpp_get_walkers, rbmpp_get_walker_electrons, rbmpp_eval, rbmpp_ratio_exchange, rbmpp_local
-  public :: rbmpp_get_smat, rbmpp_timings, rbmpp_get_measured, rbmpp_local_noalloc
+  public :: rbmpp_get_smat, rbmpp_timings, rbmpp_get_measured, rbmpp_set_measured, rbmpp_local_noalloc
   public :: rbmpp_error_message
 
   interface
@@ -196,6 +197,14 @@ module rbmpp_mod
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr), value :: h
       integer(c_int32_t), intent(out) :: ele_idx(*)
+    end function
+
+    ! a measurement loaded for rbmpp_sr_step (orows = c_null_ptr: synthetic)
+    integer(c_int) function rbmpp_set_measured(h, n, psi, eloc, rw, orows, seed) &
+        bind(C, name="rbmpp_set_measured")
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: h, psi, eloc, rw, orows
+      integer(c_int64_t), value :: n, seed
     end function
 
     ! rbmpp_local without O, staging in a fixed device array (no allocation)

@@ -19,7 +19,7 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.dirname(HERE))
 
 import oracle as O  # noqa: E402
-import refharness as RH  # noqa: E402
+from oracle import refharness as RH  # noqa: E402
 from rbm_pp_amd import lattice as lt  # noqa: E402
 
 CASES = {

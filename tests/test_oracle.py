@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import ed
-import refharness as RH
+from oracle import refharness as RH
 from rbm_pp_amd import lattice as lt
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
