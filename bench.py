@@ -163,6 +163,115 @@ def run_sr_shapes(api, lt):
             sr_shape_bench(api, lt, 16, 16, 0.55, 400000, nsl=62 * 256)]
 
 
+def energy_run(args):
+    """One full optimisation of the bench workload through the Fortran driver
+    (main.f90:300-574 on the C ABI: args.energy_iters SR iterations with S in
+    the chosen precision, parameters averaged over the last args.energy_nav,
+    then args.energy_nav measurement iterations at fixed parameters), from the
+    same seeds for every precision.  Prints one JSON line with the final energy
+    per site, its error bar and the per-iteration history."""
+    import subprocess
+    import tempfile
+    from rbm_pp_amd import lattice as lt
+    driver = os.path.join(ROOT, "rbm_pp_amd", "fortran", "rbmpp_vmc")
+    spec = lt.j1j2_model(args.L, alphar=args.alphar, J2=args.J2)
+    nspw = math.ceil(args.samples / args.walkers)
+    spec.nsample, spec.nstep, spec.nav = args.walkers * nspw, args.energy_iters, args.energy_nav
+    spec.delta_tau, spec.s_factor0, spec.s_factor_scale = 0.05, 1e-4, 1.0
+    p = lt.random_params(spec, seed_w=1, seed_f=2, sigma_w=0.01)
+    rundir = args.energy_dir or tempfile.mkdtemp(prefix="rbmpp_energy_")
+    os.makedirs(rundir, exist_ok=True)
+    lt.write_run_dir(rundir, spec, p)
+    prec = {"fp64": 0, "fp32": 1, "bf16x3": 2}[args.energy_run]
+    N = spec.N
+    log(f"energy run: {args.L}x{args.L}, S {args.energy_run}, {args.energy_iters} SR iterations, "
+        f"Nav {args.energy_nav}, {args.walkers} walkers x {nspw} samples, run dir {rundir}")
+    t0 = time.perf_counter()
+    proc = subprocess.Popen([driver, str(args.walkers), str(prec), str(args.recompute_k)], cwd=rundir,
+                            stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    it = 0
+    for line in proc.stdout:
+        if line.startswith("Iteration"):
+            it = line.split(":")[-1].strip()
+        elif "Total Energy" in line or "total energy" in line:
+            log(f"iteration {it}: E/N = {float(line.split(':')[-1]) / 4 / N:.6f}")
+    rc = proc.wait()
+    if rc != 0:
+        log(f"driver exit {rc}")
+        sys.exit(rc)
+    hist = np.loadtxt(os.path.join(rundir, "Energy_vs_Iteration.txt"))
+    e_av, e_err = [float(t) for t in open(os.path.join(rundir, "Average_Energy.txt")).readline().split()[:2]]
+    print(json.dumps({"energy_run": args.energy_run, "workload": f"j1j2_{args.L}x{args.L}_J2_{args.J2}_alphar{args.alphar}",
+                      "iterations": args.energy_iters, "nav": args.energy_nav,
+                      "samples_per_iteration": args.walkers * nspw,
+                      "energy_per_site": e_av / 4 / N, "error_per_site": e_err / 4 / N,
+                      "history_per_site": [float(v) / 4 / N for v in hist[:, 1]],
+                      "wall_s": time.perf_counter() - t0}))
+
+
+def batch_run(args):
+    """The uneven multi-lattice workload: L = 6, 8, 10, 12, 14 at J2/J1 = 0.4,
+    0.5, 0.55 (15 models), RBM alphar = 2, 2x2-sublattice pair product, C4v +
+    parity, args.batch_walkers walkers apportioned by N^2.  Each iteration:
+    one sampler launch over all walkers (rbmpp_batch_make_sample), then per
+    model the measurement and an SR step with split-BF16 x 3 S.  The same
+    sweeps launched model by model are timed once for comparison.  Prints one
+    JSON line."""
+    from rbm_pp_amd import api
+    from rbm_pp_amd import lattice as lt
+    models = [(L, J2) for L in (6, 8, 10, 12, 14) for J2 in (0.4, 0.5, 0.55)]
+    n2 = np.array([float(L) ** 4 for L, _ in models])
+    walkers = np.maximum(1, np.round(args.batch_walkers * n2 / n2.sum())).astype(int)
+    engines = []
+    for (L, J2), W in zip(models, walkers):
+        spec = lt.j1j2_model(L, alphar=2, J2=J2)
+        p = lt.random_params(spec, seed_w=1, seed_f=2, sigma_w=0.01)
+        engines.append(api.Engine(spec, params=p, n_walkers=int(W), n_samples=args.batch_samples, seed=L,
+                                  n_warmup=args.batch_warmup, n_warmup_first=args.batch_warmup,
+                                  s_precision=api.S_BF16X3, cond_guard=args.cond_guard))
+    batch = api.Batch(engines)
+    nmod = len(engines)
+    log(f"batch: {nmod} models, walkers {walkers.tolist()} (total {int(walkers.sum())}), "
+        f"{args.batch_samples} samples + {args.batch_warmup} warm-up sweeps per walker and iteration")
+    shifts = np.zeros(nmod)
+    batch.make_sample([1] * nmod, shifts)  # first call: initial configurations
+    t_launch, t_meas, t_sr, t_wall, E = [], [], [], [], None
+    for it in range(args.batch_run):
+        t0 = time.perf_counter()
+        batch.make_sample([0] * nmod, shifts)
+        t_launch.append(batch.sample_seconds())
+        tm = ts = 0.0
+        E = []
+        for k, e in enumerate(engines):
+            a = time.perf_counter()
+            r = e.calc_phys_quantity_opt(shift=shifts[k])
+            b = time.perf_counter()
+            e.sr_step(e.gamma(), s_factor0=1e-4, delta_tau=0.05, refine_iters=1)
+            tm += b - a
+            ts += time.perf_counter() - b
+            E.append(r["Etot"] / 4 / e.N)
+            shifts[k] = r["wf_av"] * 1e-3 / e.N / e.N
+        t_meas.append(tm)
+        t_sr.append(ts)
+        t_wall.append(time.perf_counter() - t0)
+        log(f"iteration {it}: {t_wall[-1]:.2f} s (sampler launch {t_launch[-1]:.2f} s, measurements {tm:.2f} s, "
+            f"SR steps {ts:.2f} s)")
+    # the same sweeps model by model
+    t_sep = 0.0
+    for k, e in enumerate(engines):
+        e.make_sample(False, shift=shifts[k])
+        t_sep += e.timings()["sample"]
+    log(f"model-by-model launches: {t_sep:.2f} s against {t_launch[-1]:.2f} s in one launch")
+    samples = float(walkers.sum()) * args.batch_samples
+    print(json.dumps({"batch_run": "uneven multi-lattice", "models": [f"{L}x{L} J2={J2}" for L, J2 in models],
+                      "walkers": walkers.tolist(), "samples_per_iteration": samples,
+                      "iterations": args.batch_run, "seconds_per_iteration": float(np.mean(t_wall)),
+                      "samples_per_s": samples / float(np.mean(t_wall)),
+                      "sampler_launch_s": float(np.mean(t_launch)), "model_by_model_launches_s": t_sep,
+                      "measurements_s": float(np.mean(t_meas)), "sr_steps_s": float(np.mean(t_sr)),
+                      "energy_per_site_last": E}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -199,6 +308,17 @@ def main():
                     help="skip the SR steps at the 16x16 sizes (Nv = 5136 and a synthetic ~2e4)")
     ap.add_argument("--sr-only", action="store_true",
                     help="run only the SR steps at the large sizes and print them")
+    ap.add_argument("--energy-run", choices=["fp64", "fp32", "bf16x3"],
+                    help="one full optimisation through the Fortran driver with this S precision; prints "
+                         "the final energy per site and its error bar")
+    ap.add_argument("--energy-iters", type=int, default=250)
+    ap.add_argument("--energy-nav", type=int, default=20)
+    ap.add_argument("--energy-dir", default=None, help="run directory of --energy-run")
+    ap.add_argument("--batch-run", type=int, default=0, metavar="ITERS",
+                    help="the uneven multi-lattice workload (15 models in one sampler launch) for ITERS iterations")
+    ap.add_argument("--batch-walkers", type=int, default=16384)
+    ap.add_argument("--batch-samples", type=int, default=4)
+    ap.add_argument("--batch-warmup", type=int, default=4)
     ap.add_argument("--sampler-only", action="store_true",
                     help="profiling aid: thermalise, run one make_sample launch and exit")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -219,6 +339,13 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": n, "copies": 3}))
+        return
+
+    if args.energy_run:
+        energy_run(args)
+        return
+    if args.batch_run:
+        batch_run(args)
         return
 
     if args.sr_only:

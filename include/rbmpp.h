@@ -181,6 +181,48 @@ int rbmpp_measure_energy(rbmpp_handle* h, double shift, double* etot, double* et
 int rbmpp_measure_correlation(rbmpp_handle* h, double shift, double* etot, double* etot2, double* norm,
                               double* wf_av, double* wf_sgn, double* chi_sp, double* chi_vbs);
 
+/* ---- tuning ----
+ * Times the launch configurations the kernels accept on this handle's own
+ * data and keeps the fastest for later calls; no counterpart in the
+ * reference.  RBMPP_TUNE_S_BUILD (after rbmpp_measure or rbmpp_set_measured,
+ * in the current S precision): output tile (128, or 256 for split BF16 x 3)
+ * and split-K count of the S build.  RBMPP_TUNE_MEASURE (stored samples):
+ * workgroups of the per-sample estimator, timed on up to 32768 stored samples
+ * (it overwrites the measurement buffers: measure again afterwards).  The
+ * sampler has no launch choice to tune: one walker per workgroup of one
+ * wave per projection copy, one workgroup per CU (its LDS). */
+#define RBMPP_TUNE_S_BUILD 1
+#define RBMPP_TUNE_MEASURE 2
+typedef struct {
+  int32_t s_tile, s_nsplit;                  /* chosen */
+  int32_t s_default_tile, s_default_nsplit;  /* built-in defaults, for comparison */
+  double s_build_s;                          /* S build with the choice, seconds */
+  int32_t measure_grid, measure_configurations;
+  double measure_s;                          /* estimator with the choice, seconds */
+} rbmpp_tune_report;
+int rbmpp_tune(rbmpp_handle* h, int32_t what, rbmpp_tune_report* rep);
+
+/* ---- several models in one sampler launch ----
+ * A batch of handles (e.g. lattices of different sizes and couplings, each
+ * with its own parameters, walkers and SR state) whose make_sample runs as
+ * one persistent launch over all their walkers: each workgroup takes the
+ * model its walker belongs to.  Every handle then measures and takes its SR
+ * step on its own (rbmpp_measure, rbmpp_sr_step), and may still sample alone.
+ * The batch covers the normal projection's block form (16 copies, N <= 104)
+ * and the skew form above N = 104; no fused measurement.  Walker w of handle
+ * i draws from the same Philox stream as in rbmpp_make_sample, so a batch
+ * call reproduces the handles' separate calls exactly.  No counterpart in
+ * the reference (one lattice per run, main.f90:75-141). */
+typedef struct rbmpp_batch rbmpp_batch;
+int rbmpp_batch_create(rbmpp_handle* const* handles, int32_t n, rbmpp_batch** out);
+/* first_iteration[i], shift[i], naccept[i] per handle, as rbmpp_make_sample */
+int rbmpp_batch_make_sample(rbmpp_batch* b, const int32_t* first_iteration, const double* shift,
+                            int64_t* naccept);
+/* device time of the last batch launch, seconds */
+double rbmpp_batch_sample_seconds(const rbmpp_batch* b);
+const char* rbmpp_batch_last_error(const rbmpp_batch* b);
+void rbmpp_batch_destroy(rbmpp_batch* b);
+
 /* ---- parity / test entry points on caller-given configurations ---- */
 /* Sample store: x[(s)*N + i] = +-1, s < n (walker-major: s = w*n_samples + k). */
 int rbmpp_set_samples(rbmpp_handle* h, int32_t n, const int32_t* x);

@@ -224,6 +224,42 @@ contains
     i2 = j - 1
   end subroutine
 
+  ! parameter.f90 (initialize_parameters, write_optimized_parameters) on the
+  ! files of the current directory; the caller changes into the run directory
+  subroutine ref_initialize_parameters() bind(C, name="ref_initialize_parameters")
+    call initialize_parameters(0)
+  end subroutine
+
+  subroutine ref_write_optimized_parameters() bind(C, name="ref_write_optimized_parameters")
+    call write_optimized_parameters(0)
+  end subroutine
+
+  ! the module's parameters and orbital table in the layouts of ref_set_params
+  ! (orb(i*N + j + 1) = Orbital_Idx(i+1, j+1) - 1)
+  subroutine ref_get_params(wc, bc, wr, br, f, orb) bind(C, name="ref_get_params")
+    real(c_double), intent(out) :: wc(*), bc(*), wr(*), br(*), f(*)
+    integer(c_int), intent(out) :: orb(*)
+    integer :: ff, iw, i, j
+    do ff = 1, alphac
+      bc(2*ff-1) = dble(bcirr(ff)); bc(2*ff) = dimag(bcirr(ff))
+      do iw = 1, N
+        wc(2*((ff-1)*N+iw)-1) = dble(Wcirr(iw,ff)); wc(2*((ff-1)*N+iw)) = dimag(Wcirr(iw,ff))
+      end do
+    end do
+    do ff = 1, alphar
+      br(ff) = brirr(ff)
+      do iw = 1, N
+        wr((ff-1)*N + iw) = Wrirr(iw,ff)
+      end do
+    end do
+    f(1:N_Slater) = Slater(1:N_Slater)
+    do i = 1, N
+      do j = 1, N
+        orb((i-1)*N + j) = Orbital_Idx(i,j) - 1
+      end do
+    end do
+  end subroutine
+
   subroutine ref_sgrnd(seed) bind(C, name="ref_sgrnd")
     integer(c_int), value :: seed
     call sgrnd(seed)

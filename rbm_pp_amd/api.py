@@ -27,6 +27,15 @@ PROJ_NORMAL, PROJ_TRANSVERSE, PROJ_NORMAL_EXACT = 0, 1, 2
 SR_NO_GRAPH = 1
 
 
+TUNE_S_BUILD, TUNE_MEASURE = 1, 2
+
+
+class TuneReport(C.Structure):
+    _fields_ = [("s_tile", C.c_int32), ("s_nsplit", C.c_int32), ("s_default_tile", C.c_int32),
+                ("s_default_nsplit", C.c_int32), ("s_build_s", C.c_double), ("measure_grid", C.c_int32),
+                ("measure_configurations", C.c_int32), ("measure_s", C.c_double)]
+
+
 class RbmppError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"rbmpp error {code}: {msg}")
@@ -91,6 +100,14 @@ def lib():
         L.rbmpp_timings.argtypes = [vp, dp]
         L.rbmpp_set_s_precision.argtypes = [vp, C.c_int32]
         L.rbmpp_set_projection.argtypes = [vp, C.c_int32]
+        L.rbmpp_tune.argtypes = [vp, C.c_int32, C.POINTER(TuneReport)]
+        L.rbmpp_batch_create.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(vp)]
+        L.rbmpp_batch_make_sample.argtypes = [vp, ip, dp, C.POINTER(C.c_int64)]
+        L.rbmpp_batch_sample_seconds.restype = C.c_double
+        L.rbmpp_batch_sample_seconds.argtypes = [vp]
+        L.rbmpp_batch_last_error.restype = C.c_char_p
+        L.rbmpp_batch_last_error.argtypes = [vp]
+        L.rbmpp_batch_destroy.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -302,6 +319,13 @@ class Engine:
         self._check(lib().rbmpp_local_noalloc(self.h, n, _i(xs), _d(el), _d(psi)))
         return dict(eloc=el.view(np.complex128), psi=psi.view(np.complex128))
 
+    def tune(self, what: int = TUNE_S_BUILD) -> dict:
+        """rbmpp_tune: time the S build's tile / split-K choices and (or) the
+        estimator grid on this engine's data, keep the fastest."""
+        r = TuneReport()
+        self._check(lib().rbmpp_tune(self.h, what, C.byref(r)))
+        return {k: getattr(r, k) for k, _ in TuneReport._fields_}
+
     def set_projection(self, projection: int) -> None:
         """Spin-projection weights: PROJ_NORMAL (normal_code, cos(pi/2) taken as
         0: block form), PROJ_NORMAL_EXACT (normal_code with the reference's
@@ -322,3 +346,44 @@ class Engine:
         self._check(lib().rbmpp_timings(self.h, _d(t)))
         return dict(slater=t[0], sample=t[1], measure=t[2], s_build=t[3], solve=t[4], sr_total=t[5],
                     recompute_count=t[6])
+
+
+class Batch:
+    """Several engines (lattices, couplings, parameters) sampled by one
+    persistent launch over all their walkers (rbmpp_batch_make_sample); each
+    engine measures and takes its SR step on its own."""
+
+    def __init__(self, engines):
+        self.engines = list(engines)
+        arr = (C.c_void_p * len(self.engines))(*[e.h for e in self.engines])
+        h = C.c_void_p()
+        rc = lib().rbmpp_batch_create(arr, len(self.engines), C.byref(h))
+        self.h = h
+        if rc != 0:
+            msg = lib().rbmpp_batch_last_error(h).decode() if h else ""
+            self.close()
+            raise RbmppError(rc, msg)
+
+    def make_sample(self, first_iteration, shift) -> np.ndarray:
+        n = len(self.engines)
+        first = np.ascontiguousarray(np.broadcast_to(np.asarray(first_iteration, np.int32), (n,)))
+        sh = np.ascontiguousarray(np.broadcast_to(np.asarray(shift, np.float64), (n,)))
+        nacc = np.zeros(n, np.int64)
+        rc = lib().rbmpp_batch_make_sample(self.h, _i(first), _d(sh), nacc.ctypes.data_as(C.POINTER(C.c_int64)))
+        if rc != 0:
+            raise RbmppError(rc, lib().rbmpp_batch_last_error(self.h).decode())
+        return nacc
+
+    def sample_seconds(self) -> float:
+        return lib().rbmpp_batch_sample_seconds(self.h)
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            lib().rbmpp_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -71,6 +71,7 @@ struct rbmpp_handle {
   double *d_S = nullptr, *d_A = nullptr, *d_L = nullptr, *d_vec = nullptr;
   int* d_info = nullptr;
   double* d_sc = nullptr;  // SR scalars (k_sr_prepare)
+  double* d_dinv = nullptr;  // inverses of the Cholesky factor's diagonal blocks
   int* d_fl = nullptr;     // SR flags
   unsigned char* d_fii = nullptr;  // [nv] 1 for the f_ii parameters
   hipGraph_t sr_graph = nullptr;   // captured SR pipeline
@@ -79,6 +80,9 @@ struct rbmpp_handle {
   unsigned char* d_packed = nullptr;  // centred O rows packed for the S build
   size_t packed_cap = 0;
   int n_cu = 256;                     // compute units of the device
+  // choices of rbmpp_tune (0: built-in defaults)
+  int tune_tile[3] = {0, 0, 0}, tune_nsplit[3] = {0, 0, 0};  // per S precision
+  int tune_meas_grid = 0;
   DevModel* d_model = nullptr;  // device copy of devmodel(h)
   int measured = 0;
   int s_valid = 0;
@@ -391,10 +395,10 @@ int rbmpp_get_gamma(const rbmpp_handle* h, double* gamma) {
   return RBMPP_OK;
 }
 
-int rbmpp_make_sample(rbmpp_handle* h, int32_t first_iteration, double shift, int64_t* naccept) {
-  if (!h) return RBMPP_E_ARG;
+// sweep arguments of one make_sample call (make_sample.f90:4-240 for all walkers)
+static int sweep_args(rbmpp_handle* h, int32_t first_iteration, double shift, bool allow_fused, SweepArgs& a,
+                      bool& fused) {
   if (!first_iteration && !h->walkers_ready) return fail(h, RBMPP_E_STATE, "make_sample: walkers not initialised");
-  SweepArgs a;
   a.m = h->d_model;
   a.ws = h->ws;
   a.nwalk = h->W;
@@ -417,21 +421,18 @@ int rbmpp_make_sample(rbmpp_handle* h, int32_t first_iteration, double shift, in
   a.measure = 0;
   a.m_psi = nullptr; a.m_eloc = nullptr; a.m_rw = nullptr; a.m_orows = nullptr;
   h->fused_valid = 0;
-  const bool fused = h->cfg.fused_measure && h->N <= RBMPP_NLDS && h->nspw > 0;
+  fused = allow_fused && h->cfg.fused_measure && h->N <= RBMPP_NLDS && h->nspw > 0;
   if (fused) {
     int rc = ensure_measure(h, (size_t)h->W * h->nspw);
     if (rc) return rc;
     a.measure = 1;
     a.m_psi = h->d_mpsi; a.m_eloc = h->d_eloc; a.m_rw = h->d_rw; a.m_orows = h->d_orows;
   }
-  tic(h);
-  {
-    const DevModel hm = devmodel(h);
-    if (launch_sweep(hm, a, h->st) != 0) return fail(h, RBMPP_E_ARG, "sweep: unsupported size");
-    h->minv_block = hm.block;  // layout of ws.minv written by this launch
-  }
-  HIPCHK(h, hipGetLastError());
-  h->t[1] = toc(h);
+  return 0;
+}
+
+// after a sweep launch: walker statuses, acceptance and recompute counts
+static int sweep_finish(rbmpp_handle* h, bool fused, double shift, int64_t* naccept) {
   std::vector<int> stat(h->W);
   std::vector<long long> nacc(h->W);
   std::vector<int> nrec(h->W);
@@ -457,6 +458,23 @@ int rbmpp_make_sample(rbmpp_handle* h, int32_t first_iteration, double shift, in
   if (fused) h->n_meas = h->n_stored;
   h->measured = 0;
   return RBMPP_OK;
+}
+
+int rbmpp_make_sample(rbmpp_handle* h, int32_t first_iteration, double shift, int64_t* naccept) {
+  if (!h) return RBMPP_E_ARG;
+  SweepArgs a;
+  bool fused = false;
+  int rc = sweep_args(h, first_iteration, shift, true, a, fused);
+  if (rc) return rc;
+  tic(h);
+  {
+    const DevModel hm = devmodel(h);
+    if (launch_sweep(hm, a, h->st) != 0) return fail(h, RBMPP_E_ARG, "sweep: unsupported size");
+    h->minv_block = hm.block;  // layout of ws.minv written by this launch
+  }
+  HIPCHK(h, hipGetLastError());
+  h->t[1] = toc(h);
+  return sweep_finish(h, fused, shift, naccept);
 }
 
 // frees a buffer made by dalloc
@@ -538,6 +556,7 @@ static int run_local(rbmpp_handle* h, int n, const signed char* d_x, int mode, d
   a.status = h->d_status;
   a.corr_part = d_corr;
   int cap = grid_cap > 0 ? grid_cap : h->local_grid;
+  if (grid_cap <= 0 && h->tune_meas_grid > 0) cap = std::min(cap, h->tune_meas_grid);
   const DevModel hm0 = devmodel(h);
   if (!d_corr && hm0.block) {
     // the block-form estimator keeps F^-1 of every copy per workgroup
@@ -742,10 +761,12 @@ static int build_s(rbmpp_handle* h) {
   int rc;
   if (!h->d_S && ((rc = dalloc(h, &h->d_S, (size_t)nv * nv)) || (rc = dalloc(h, &h->d_A, (size_t)nv * nv)) ||
                   (rc = dalloc(h, &h->d_L, (size_t)nv * nv)) || (rc = dalloc(h, &h->d_vec, (size_t)4 * nv)) ||
-                  (rc = dalloc(h, &h->d_info, 1)) || (rc = dalloc(h, &h->d_sc, 8)) || (rc = dalloc(h, &h->d_fl, 8))))
+                  (rc = dalloc(h, &h->d_info, 1)) || (rc = dalloc(h, &h->d_sc, 8)) || (rc = dalloc(h, &h->d_fl, 8)) ||
+                  (rc = dalloc(h, &h->d_dinv, chol_dinv_size(nv)))))
     return rc;
   const int rows = h->real_only ? n : 2 * n;
-  const SyrkPlan pl = syrk_plan(h->cfg.s_precision, rows, nv, h->n_cu);
+  const int prec = h->cfg.s_precision;
+  const SyrkPlan pl = syrk_plan(prec, rows, nv, h->n_cu, h->tune_tile[prec], h->tune_nsplit[prec]);
   if ((rc = ensure_part(h, pl.part_doubles))) return rc;
   if (pl.packed_bytes > h->packed_cap) {
     unsigned char* pk = nullptr;
@@ -800,7 +821,7 @@ int rbmpp_sr_step(rbmpp_handle* h, const rbmpp_sr_config* c, double* gamma, rbmp
   p.x = h->d_vec; p.r = h->d_vec + nv; p.ax = h->d_vec + 2 * nv;
   double* d_g = h->d_vec + 3 * nv;
   p.g = d_g;
-  p.sc = h->d_sc; p.fl = h->d_fl;
+  p.sc = h->d_sc; p.fl = h->d_fl; p.dinv = h->d_dinv;
   HIPCHK(h, hipMemcpyAsync(d_g, g.data(), sizeof(double) * nv, hipMemcpyHostToDevice, h->st));
   HIPCHK(h, hipMemcpyAsync(h->d_sc + 3, &sf, sizeof(double), hipMemcpyHostToDevice, h->st));
   // the launch sequence (scalars, system, Cholesky, solves, refinement,
@@ -1236,6 +1257,185 @@ int rbmpp_set_s_precision(rbmpp_handle* h, int32_t s_precision) {
   h->s_valid = 0;
   return RBMPP_OK;
 }
+
+// ------------------------------------------------------------ tuning
+int rbmpp_tune(rbmpp_handle* h, int32_t what, rbmpp_tune_report* rep) {
+  if (!h) return RBMPP_E_ARG;
+  if (rep) memset(rep, 0, sizeof *rep);
+  int rc = 0;
+  if (what & RBMPP_TUNE_S_BUILD) {
+    if (!h->measured) return fail(h, RBMPP_E_STATE, "tune: the S build is tuned on a measurement");
+    const int prec = h->cfg.s_precision, rows = h->real_only ? (int)h->n_meas : 2 * (int)h->n_meas;
+    const SyrkPlan base = syrk_plan(prec, rows, h->nv, h->n_cu);
+    std::vector<int> tiles = {128};
+    if (prec == RBMPP_S_BF16X3) tiles.push_back(256);
+    double best = 1e300;
+    int bt = 0, bs = 0;
+    for (int tile : tiles) {
+      const SyrkPlan p0 = syrk_plan(prec, rows, h->nv, h->n_cu, tile);
+      // split counts around the default (multiples of 8 for the XCD order)
+      std::vector<int> splits = {1, p0.nsplit};
+      for (int f : {8, 16, 32}) splits.push_back(f);
+      for (int ns : splits) {
+        const SyrkPlan p = syrk_plan(prec, rows, h->nv, h->n_cu, tile, ns);
+        if (p.nsplit != ns && ns != p0.nsplit) continue;  // clamped to the rows: same as another candidate
+        h->tune_tile[prec] = tile;
+        h->tune_nsplit[prec] = p.nsplit;
+        double t = 1e300;
+        for (int r = 0; r < 2; r++) {
+          if ((rc = build_s(h))) return rc;
+          t = std::min(t, h->t[3]);
+        }
+        if (t < best) { best = t; bt = tile; bs = p.nsplit; }
+      }
+    }
+    h->tune_tile[prec] = bt;
+    h->tune_nsplit[prec] = bs;
+    h->s_valid = 0;
+    if (rep) {
+      rep->s_tile = bt; rep->s_nsplit = bs; rep->s_build_s = best;
+      rep->s_default_tile = base.tile; rep->s_default_nsplit = base.nsplit;
+    }
+  }
+  if (what & RBMPP_TUNE_MEASURE) {
+    if (h->n_stored <= 0) return fail(h, RBMPP_E_STATE, "tune: the estimator is tuned on stored samples");
+    const int n = (int)std::min<int64_t>(h->n_stored, 32768);
+    const bool was_measured = h->measured;
+    double best = 1e300;
+    int bg = 0;
+    for (int grid : {256, 512, 1024, 2048, 4096}) {
+      h->tune_meas_grid = grid;
+      double t = 1e300;
+      for (int r = 0; r < 2; r++) {
+        tic(h);
+        if ((rc = run_local(h, n, h->d_xs, 1 | 2, 0.0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                            nullptr)))
+          return rc;
+        t = std::min(t, toc(h));
+      }
+      if (t < best) { best = t; bg = grid; }
+    }
+    h->tune_meas_grid = bg;
+    // the measurement buffers now hold the tuning runs
+    if (was_measured) h->measured = 0;
+    if (rep) { rep->measure_grid = bg; rep->measure_s = best; rep->measure_configurations = n; }
+  }
+  return RBMPP_OK;
+}
+
+// ------------------------------------------------------------ batches
+struct rbmpp_batch {
+  std::vector<rbmpp_handle*> hs;
+  std::vector<int> kind;
+  SweepBatchEntry* d_ent = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int device = 0;
+  double t_sample = 0;
+  std::string err;
+};
+
+static int bfail(rbmpp_batch* b, int code, const std::string& msg) {
+  if (b) b->err = msg;
+  return code;
+}
+
+#define BHIPCHK(b, x)                                                          \
+  do {                                                                        \
+    hipError_t e_ = (x);                                                      \
+    if (e_ != hipSuccess)                                                     \
+      return bfail(b, RBMPP_E_HIP, std::string(#x ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+void rbmpp_batch_destroy(rbmpp_batch* b) {
+  if (!b) return;
+  if (b->st) hipStreamSynchronize(b->st);
+  if (b->d_ent) hipFree(b->d_ent);
+  if (b->e0) hipEventDestroy(b->e0);
+  if (b->e1) hipEventDestroy(b->e1);
+  if (b->st) hipStreamDestroy(b->st);
+  delete b;
+}
+
+const char* rbmpp_batch_last_error(const rbmpp_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+int rbmpp_batch_create(rbmpp_handle* const* handles, int32_t n, rbmpp_batch** out) {
+  if (!handles || n < 1 || !out) return RBMPP_E_ARG;
+  *out = nullptr;
+  rbmpp_batch* b = new rbmpp_batch();
+  *out = b;
+  for (int i = 0; i < n; i++) {
+    rbmpp_handle* h = handles[i];
+    if (!h) return bfail(b, RBMPP_E_ARG, "null handle in batch");
+    for (int j = 0; j < i; j++)
+      if (handles[j] == h) return bfail(b, RBMPP_E_ARG, "a handle appears twice in the batch");
+    if (i > 0 && h->device != handles[0]->device) return bfail(b, RBMPP_E_ARG, "batch handles on different devices");
+    const int kind = sweep_kind(devmodel(h));
+    if (kind < 0)
+      return bfail(b, RBMPP_E_ARG, "handle " + std::to_string(i) +
+                                       ": the batch covers the block form (normal projection, 16 copies) up to "
+                                       "N = 104 and the skew form above");
+    b->hs.push_back(h);
+    b->kind.push_back(kind);
+  }
+  b->device = handles[0]->device;
+  BHIPCHK(b, hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking));
+  BHIPCHK(b, hipEventCreate(&b->e0));
+  BHIPCHK(b, hipEventCreate(&b->e1));
+  if (hipMalloc((void**)&b->d_ent, sizeof(SweepBatchEntry) * n) != hipSuccess) {
+    b->d_ent = nullptr;
+    (void)hipGetLastError();
+    return bfail(b, RBMPP_E_NOMEM, "batch: hipMalloc of the entry table");
+  }
+  return RBMPP_OK;
+}
+
+int rbmpp_batch_make_sample(rbmpp_batch* b, const int32_t* first_iteration, const double* shift, int64_t* naccept) {
+  if (!b || !first_iteration || !shift) return RBMPP_E_ARG;
+  const int n = (int)b->hs.size();
+  // the lattices with the most work per walker (N^3) first, so that their
+  // walkers start in the first wave of workgroups; results do not depend on
+  // the order (every walker draws from its own stream)
+  std::vector<int> order(n);
+  for (int i = 0; i < n; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->hs[x]->N > b->hs[y]->N; });
+  std::vector<SweepBatchEntry> ent(n);
+  int blocks = 0;
+  for (int e = 0; e < n; e++) {
+    const int i = order[e];
+    rbmpp_handle* h = b->hs[i];
+    bool fused = false;
+    const int rc = sweep_args(h, first_iteration[i], shift[i], false, ent[e].a, fused);
+    if (rc) return bfail(b, rc, "handle " + std::to_string(i) + ": " + h->err);
+    ent[e].first_block = blocks;
+    ent[e].kind = b->kind[i];
+    blocks += h->W;
+    // the model descriptor and walker state of each handle are current on
+    // its own stream
+    BHIPCHK(b, hipStreamSynchronize(h->st));
+  }
+  BHIPCHK(b, hipMemcpyAsync(b->d_ent, ent.data(), sizeof(SweepBatchEntry) * n, hipMemcpyHostToDevice, b->st));
+  BHIPCHK(b, hipEventRecord(b->e0, b->st));
+  launch_sweep_batch(b->d_ent, n, blocks, b->st);
+  BHIPCHK(b, hipGetLastError());
+  BHIPCHK(b, hipEventRecord(b->e1, b->st));
+  BHIPCHK(b, hipEventSynchronize(b->e1));
+  float ms = 0;
+  hipEventElapsedTime(&ms, b->e0, b->e1);
+  b->t_sample = ms * 1e-3;
+  for (int i = 0; i < n; i++) {
+    rbmpp_handle* h = b->hs[i];
+    h->minv_block = devmodel(h).block;
+    h->t[1] = b->t_sample;
+    int64_t na = 0;
+    const int rc = sweep_finish(h, false, shift[i], &na);
+    if (rc) return bfail(b, rc, "handle " + std::to_string(i) + ": " + h->err);
+    if (naccept) naccept[i] = na;
+  }
+  return RBMPP_OK;
+}
+
+double rbmpp_batch_sample_seconds(const rbmpp_batch* b) { return b ? b->t_sample : 0.0; }
 
 int rbmpp_timings(const rbmpp_handle* h, double* t) {
   if (!h || !t) return RBMPP_E_ARG;

@@ -93,6 +93,19 @@ __host__ __device__ inline int corr_len(int N) { return corr_sums(N) + 2 * (N / 
 
 void launch_slater(const DevModel& m, const double* f, double* se, int* bad, hipStream_t s);
 int launch_sweep(const DevModel& hm, const SweepArgs& a, hipStream_t s);
+
+// one entry of a multi-model sampler launch: the model's sweep arguments, the
+// first workgroup of its walkers and the kernel specialisation it needs
+enum { SWEEP_BLK64 = 0, SWEEP_BLK128 = 1, SWEEP_SKEW256 = 2 };
+struct SweepBatchEntry {
+  SweepArgs a;
+  int first_block;
+  int kind;
+};
+// the specialisation a model needs in the multi-model launch, or -1 (sizes
+// or projections the batch kernel does not cover)
+int sweep_kind(const DevModel& hm);
+void launch_sweep_batch(const SweepBatchEntry* d_ent, int nent, int nblocks, hipStream_t s);
 int launch_local(const DevModel& hm, const LocalArgs& a, int grid, hipStream_t s);
 // the N <= RBMPP_NLDS estimator (register Gauss-Jordan, 1024 threads)
 void launch_meas(const LocalArgs& a, int grid, hipStream_t s);

@@ -15,6 +15,8 @@
 // by the next proposal's ratio pass as it reads each row (one pass over F^-1
 // per proposal).  All threads draw the same Philox numbers, so the control
 // flow is uniform across the workgroup and runs on the scalar unit.
+#include <algorithm>
+
 #include "rbmpp_kernels.h"
 #include "rbmpp_measure.h"
 #include "rbmpp_block.h"
@@ -550,14 +552,14 @@ __device__ __forceinline__ uint64_t uni64(uint64_t v) {
   return ((uint64_t)(uint32_t)uni((int)(uint32_t)(v >> 32)) << 32) | (uint32_t)uni((int)(uint32_t)v);
 }
 
+// make_sample for walker w of a.ws (one workgroup of one wave per copy).
+// Called by k_sweep (one model) and k_sweep_batch (models of several
+// lattices in one launch); s is the workgroup's shared memory.
 template <int NMAX, bool LDSMAT, bool MEAS, bool BLK>
-__global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
+__device__ __forceinline__ void sweep_walker(const SweepArgs& a, const int w, SweepSmem<NMAX, LDSMAT>& s) {
   constexpr int KD = NMAX >= 128 ? NMAX / 128 : 1;
   constexpr int UN = 8;  // rows in flight per wave
-  __shared__ SweepSmem<NMAX, LDSMAT> s;
   const DevModel& m = *a.m;
-  const int w = blockIdx.x;
-  if (w >= a.nwalk) return;
   const int tid = threadIdx.x, nth = blockDim.x;
   const int lane = tid & (RBMPP_WAVE - 1), wid = tid >> 6;
   const int N = m.N, C = m.C, P = m.P;
@@ -1162,6 +1164,52 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
     a.ws.naccept[w] = naccept;
     a.ws.nrecompute[w] = nrec;
   }
+}
+
+template <int NMAX, bool LDSMAT, bool MEAS, bool BLK>
+__global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
+  __shared__ SweepSmem<NMAX, LDSMAT> s;
+  const int w = blockIdx.x;
+  if (w >= a.nwalk) return;
+  sweep_walker<NMAX, LDSMAT, MEAS, BLK>(a, w, s);
+}
+
+// Walkers of several models (lattices of different sizes, couplings and
+// parameters) in one launch: workgroup b belongs to the entry whose block
+// range holds it and runs that model's specialisation of the sweep over the
+// same shared-memory block.
+static constexpr size_t BATCH_SMEM =
+    std::max(std::max(sizeof(SweepSmem<64, true>), sizeof(SweepSmem<128, true>)), sizeof(SweepSmem<256, false>));
+
+__global__ __launch_bounds__(1024) void k_sweep_batch(const SweepBatchEntry* __restrict__ ent, int nent) {
+  __shared__ __align__(16) unsigned char smem[BATCH_SMEM];
+  const int b = blockIdx.x;
+  int k = 0;
+  while (k + 1 < nent && b >= ent[k + 1].first_block) k++;
+  k = __builtin_amdgcn_readfirstlane(k);
+  const SweepArgs a = ent[k].a;
+  const int w = b - ent[k].first_block;
+  if (w >= a.nwalk) return;
+  switch (ent[k].kind) {
+    case SWEEP_BLK64: sweep_walker<64, true, false, true>(a, w, *reinterpret_cast<SweepSmem<64, true>*>(smem)); break;
+    case SWEEP_BLK128: sweep_walker<128, true, false, true>(a, w, *reinterpret_cast<SweepSmem<128, true>*>(smem)); break;
+    default: sweep_walker<256, false, false, false>(a, w, *reinterpret_cast<SweepSmem<256, false>*>(smem)); break;
+  }
+}
+
+int sweep_kind(const DevModel& hm) {
+  const int N = hm.N;
+  if (hm.C > RBMPP_CMAX || hm.mr > RBMPP_MRMAX || hm.mc > RBMPP_MCMAX || (N & 1)) return -1;
+  const bool blk = hm.block && hm.C == RBMPP_CMAX && N <= RBMPP_NLDS;
+  if (N <= RBMPP_NLDS && !blk) return -1;  // skew form at N <= 104: not in the batch kernel
+  if (N <= 64) return SWEEP_BLK64;
+  if (N <= RBMPP_NLDS) return SWEEP_BLK128;
+  if (N <= 256) return SWEEP_SKEW256;
+  return -1;
+}
+
+void launch_sweep_batch(const SweepBatchEntry* d_ent, int nent, int nblocks, hipStream_t st) {
+  hipLaunchKernelGGL(k_sweep_batch, dim3(nblocks), dim3(1024), 0, st, d_ent, nent);
 }
 
 template <int NMAX, bool LDSMAT>

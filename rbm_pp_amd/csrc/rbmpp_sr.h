@@ -17,22 +17,30 @@ void launch_row_kappa(int ns, int cplx_rows, const double2* psi, const double* r
 struct SyrkPlan {
   int mode;          // RBMPP_S_FP64 / _FP32 / _BF16X3
   int kb;            // rows per K-block (16 FP64, 32 FP32 / BF16)
-  int nvp;           // nv rounded up to the 128-column tile
+  int tile;          // output tile edge: 128, or 256 (split BF16 x 3 at large nv)
+  int nvp;           // nv rounded up to the tile
   int ntile, npair;  // column tiles, upper-triangle tile pairs
   int nkb;           // K-blocks
   int nsplit, kb_per_split;
   size_t packed_bytes;   // nkb * nvp * 128
-  size_t part_doubles;   // nsplit * npair * 128 * 128
+  size_t part_doubles;   // nsplit * npair * tile * tile
 };
-SyrkPlan syrk_plan(int precision, int nrows, int nv, int cus);
+// tile <= 0: chosen from precision and nv; nsplit <= 0: about two resident
+// workgroups per CU
+SyrkPlan syrk_plan(int precision, int nrows, int nv, int cus, int tile = 0, int nsplit = 0);
 void launch_center_pack(const SyrkPlan& pl, int nrows, int nv, const double* orows, const double* kappa,
                         const double* obar, unsigned char* packed, hipStream_t st);
 void launch_syrk_packed(const SyrkPlan& pl, int nv, const unsigned char* packed, double* part, double inv_norm,
                         double* S, hipStream_t st);
 void launch_gemv(int n, const double* A, const double* x, double* y, hipStream_t st);
-void launch_cholesky(int n, double* A, int* info, hipStream_t st);
-// L L^T x = b in place (blocked forward and back substitution)
-void launch_chol_solve(int n, const double* L, double* x, hipStream_t st);
+// blocked Cholesky of the lower triangle in place (64-wide blocks); dinv gets
+// the inverses of the 64 x 64 diagonal blocks of the factor
+void launch_cholesky(int n, double* A, double* dinv, int* info, hipStream_t st);
+// L L^T x = b in place in x (blocked substitutions through the diagonal-block
+// inverses; tmp is an n-vector of scratch)
+void launch_chol_solve(int n, const double* L, const double* dinv, double* x, double* tmp, hipStream_t st);
+// doubles of dinv for an n x n factor
+inline size_t chol_dinv_size(int n) { return (size_t)((n + 63) / 64) * 64 * 64; }
 
 // The SR system on the device (scalars, D S D + shift I, Cholesky, solve,
 // refinement, lambda and residual), every launch on one stream with no host
@@ -43,7 +51,8 @@ struct SrPipeline {
   const unsigned char* fii;  // [nv] 1 for the f_ii parameters
   const double* g;           // [nv] g - 2 E <O> (unscaled)
   double *A, *L;             // nv x nv: D S D + shift I and its factor
-  double *x, *r, *ax;        // [nv]: solution, residual / correction, A x
+  double *x, *r, *ax;        // [nv]: solution, residual / correction, A x (also solve scratch)
+  double* dinv;              // inverses of the factor's diagonal blocks (chol_dinv_size)
   double* sc;                // [8] scalars (see k_sr_prepare)
   int* fl;                   // [8] flags and Cholesky info
 };

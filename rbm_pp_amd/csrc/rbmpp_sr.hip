@@ -4,6 +4,8 @@
 // residual / lambda products.  The S build itself is in rbmpp_syrk.hip.
 #include "rbmpp_sr.h"
 
+#include <algorithm>
+
 // ---------------------------------------------------------------- sums
 // main.f90:351-361 operands: per-block partials of
 // {Etot, Etot2, norm, wf_av} (calc_phys_quantity.f90:105-109).
@@ -91,68 +93,119 @@ __global__ void k_gemv(int n, const double* __restrict__ A, const double* __rest
 // Right-looking, lower triangle of a row-major SPD matrix, block NB=32.
 #define CH_NB 64
 
-// factor the diagonal block [k0, k0+nb)
-__global__ __launch_bounds__(256) void k_potrf_diag(int n, int k0, double* A, int* info) {
+// Factor the diagonal block [k0, k0+nb)^2 and invert its factor: one wave,
+// the block in LDS.  Factor: lane i owns row i (right-looking, column j
+// scaled by sqrt of the pivot, then the rank-1 update of the rows below).
+// Inverse X = L^-1: lane c owns column c (forward substitution on e_c).
+// L goes back into A's lower triangle, X (row-major, zero above the diagonal)
+// into dinv.  A pivot <= 0 or NaN sets info (first failing pivot, 1-based).
+__global__ __launch_bounds__(64) void k_potrf_inv(int n, int k0, double* A, double* dinv, int* info) {
   __shared__ double B[CH_NB][CH_NB + 1];
-  const int nb = min(CH_NB, n - k0);
-  for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
-    const int i = e / nb, j = e - i * nb;
-    B[i][j] = A[(size_t)(k0 + i) * n + k0 + j];
+  __shared__ double X[CH_NB][CH_NB + 1];
+  const int nb = min(CH_NB, n - k0), i = threadIdx.x;
+  for (int e = i; e < CH_NB * CH_NB; e += 64) {
+    const int r = e / CH_NB, c = e - r * CH_NB;
+    B[r][c] = (r < nb && c < nb) ? A[(size_t)(k0 + r) * n + k0 + c] : 0.0;
+    X[r][c] = 0.0;
   }
   __syncthreads();
   for (int j = 0; j < nb; j++) {
-    if (B[j][j] <= 0.0 || !(B[j][j] == B[j][j])) {
-      if (threadIdx.x == 0 && *info == 0) *info = k0 + j + 1;  // first failing pivot
+    const double p = B[j][j];
+    if (!(p > 0.0)) {
+      if (i == 0 && *info == 0) *info = k0 + j + 1;
       return;
     }
+    const double d = sqrt(p);
     __syncthreads();
-    const double d = sqrt(B[j][j]);
-    __syncthreads();
-    if (threadIdx.x == 0) B[j][j] = d;
-    for (int i = j + 1 + threadIdx.x; i < nb; i += blockDim.x) B[i][j] /= d;
-    __syncthreads();
-    for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
-      const int i = e / nb, l = e - i * nb;
-      if (i > j && l > j && l <= i) B[i][l] -= B[i][j] * B[l][j];
+    double lij = 0.0;
+    if (i == j) B[j][j] = d;
+    if (i > j && i < nb) {
+      lij = B[i][j] / d;
+      B[i][j] = lij;
     }
     __syncthreads();
+    if (i > j && i < nb)
+      for (int k = j + 1; k <= i; k++) B[i][k] = fma(-lij, B[k][j], B[i][k]);
+    __syncthreads();
   }
-  for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
-    const int i = e / nb, j = e - i * nb;
-    if (j <= i) A[(size_t)(k0 + i) * n + k0 + j] = B[i][j];
-  }
-}
-
-// panel: rows i >= k0+nb, columns [k0, k0+nb): L_ik = A_ik L_kk^-T
-__global__ __launch_bounds__(256) void k_trsm_panel(int n, int k0, double* A) {
-  __shared__ double Lk[CH_NB][CH_NB + 1];
-  const int nb = min(CH_NB, n - k0);
-  for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
-    const int i = e / nb, j = e - i * nb;
-    Lk[i][j] = A[(size_t)(k0 + i) * n + k0 + j];
+  // X = L^-1, column c by lane c
+  if (i < nb) {
+    X[i][i] = 1.0 / B[i][i];
+    for (int r = i + 1; r < nb; r++) {
+      double t = 0.0;
+      for (int k = i; k < r; k++) t = fma(B[r][k], X[k][i], t);
+      X[r][i] = -t / B[r][r];
+    }
   }
   __syncthreads();
-  const int i = k0 + nb + blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double r[CH_NB];
-#pragma unroll
-  for (int j = 0; j < CH_NB; j++) r[j] = (j < nb) ? A[(size_t)i * n + k0 + j] : 0.0;
-#pragma unroll
-  for (int j = 0; j < CH_NB; j++) {
-    if (j < nb) {
-      double v = r[j];
-#pragma unroll
-      for (int k = 0; k < CH_NB; k++)
-        if (k < j) v -= r[k] * Lk[j][k];
-      r[j] = v / Lk[j][j];
-    }
+  double* D = dinv + (size_t)(k0 / CH_NB) * CH_NB * CH_NB;
+  for (int e = i; e < CH_NB * CH_NB; e += 64) {
+    const int r = e / CH_NB, c = e - r * CH_NB;
+    D[e] = c <= r ? X[r][c] : 0.0;
+    if (r < nb && c <= r) A[(size_t)(k0 + r) * n + k0 + c] = B[r][c];
   }
-#pragma unroll
-  for (int j = 0; j < CH_NB; j++)
-    if (j < nb) A[(size_t)i * n + k0 + j] = r[j];
 }
 
-// trailing update of the lower triangle: A_ij -= sum_k L_ik L_jk, i,j >= t0
+typedef double f64x4_t __attribute__((ext_vector_type(4)));
+
+// Panel below the diagonal block, L21 = A21 L11^-T = A21 X^T: 64 rows per
+// workgroup on the FP64 matrix cores (16 x 16 x 4 blocks, 2 x 2 per wave;
+// lane l supplies row l % 16 at k = 4 s + l / 16 of both operands, output
+// register r is element (l / 16 + 4 r, l % 16) of a block).
+__global__ __launch_bounds__(256) void k_trsm_panel(int n, int k0, double* A, const double* __restrict__ dinv) {
+  constexpr int TS = 64;
+  __shared__ double Ai[TS][CH_NB + 1];
+  __shared__ double Xs[TS][CH_NB + 1];
+  const int nb = min(CH_NB, n - k0);
+  const int r0 = k0 + nb + blockIdx.x * TS;
+  const double* D = dinv + (size_t)(k0 / CH_NB) * CH_NB * CH_NB;
+  for (int e = threadIdx.x; e < TS * CH_NB; e += 256) {
+    const int r = e / CH_NB, k = e - r * CH_NB;
+    const int gi = r0 + r;
+    Ai[r][k] = (gi < n && k < nb) ? A[(size_t)gi * n + k0 + k] : 0.0;
+    Xs[r][k] = D[e];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wi = wave >> 1, wj = wave & 1;
+  const int m = lane & 15, ks = lane >> 4;
+  f64x4_t acc[2][2];
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++) acc[x][y] = f64x4_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+  for (int s4 = 0; s4 < CH_NB / 4; s4++) {
+    const int k = 4 * s4 + ks;
+    double a[2], b[2];
+#pragma unroll
+    for (int x = 0; x < 2; x++) {
+      a[x] = Ai[wi * 32 + 16 * x + m][k];
+      b[x] = Xs[wj * 32 + 16 * x + m][k];  // B(k, j) = X(j, k)
+    }
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+      for (int y = 0; y < 2; y++) acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[x], b[y], acc[x][y], 0, 0, 0);
+  }
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int gi = r0 + wi * 32 + 16 * x + ks + 4 * r;
+        const int j = wj * 32 + 16 * y + m;
+        if (gi < n && j < nb) A[(size_t)gi * n + k0 + j] = acc[x][y][r];
+      }
+}
+
+// trailing update of the lower triangle, A_ij -= sum_k L_ik L_jk for
+// i >= j >= t0 and k over the panel [k0, k0+nb): 64 x 64 tiles on the FP64
+// matrix cores (v_mfma_f64_16x16x4), both row panels staged in LDS; each wave
+// a 32 x 32 quadrant of 2 x 2 blocks.  MFMA lane l supplies row l % 16 at
+// k = 4 s + l / 16 of both operands; its output register r is element
+// (l / 16 + 4 r, l % 16) of the block.
 __global__ __launch_bounds__(256) void k_syrk_trailing(int n, int k0, int t0, int ntile, double* A) {
   constexpr int TS = 64;
   __shared__ double Li[TS][CH_NB + 1];
@@ -168,20 +221,38 @@ __global__ __launch_bounds__(256) void k_syrk_trailing(int n, int k0, int t0, in
     Lj[r][k] = (gj < n && k < nb) ? A[(size_t)gj * n + k0 + k] : 0.0;
   }
   __syncthreads();
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wi = wave >> 1, wj = wave & 1;
+  const int m = lane & 15, ks = lane >> 4;
+  f64x4_t acc[2][2];
 #pragma unroll
-  for (int a = 0; a < 4; a++)
+  for (int x = 0; x < 2; x++)
 #pragma unroll
-    for (int b = 0; b < 4; b++) {
-      const int r = ty * 4 + a, c = tx * 4 + b;
-      const int gi = t0 + ti * TS + r, gj = t0 + tj * TS + c;
-      if (gi < n && gj < n && gj <= gi) {
-        double t = 0;
-#pragma unroll 8
-        for (int k = 0; k < CH_NB; k++) t = fma(Li[r][k], Lj[c][k], t);
-        A[(size_t)gi * n + gj] -= t;
-      }
+    for (int y = 0; y < 2; y++) acc[x][y] = f64x4_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+  for (int s4 = 0; s4 < CH_NB / 4; s4++) {
+    const int k = 4 * s4 + ks;
+    double a[2], b[2];
+#pragma unroll
+    for (int x = 0; x < 2; x++) {
+      a[x] = Li[wi * 32 + 16 * x + m][k];
+      b[x] = Lj[wj * 32 + 16 * x + m][k];
     }
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+      for (int y = 0; y < 2; y++) acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[x], b[y], acc[x][y], 0, 0, 0);
+  }
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int gi = t0 + ti * TS + wi * 32 + 16 * x + ks + 4 * r;
+        const int gj = t0 + tj * TS + wj * 32 + 16 * y + m;
+        if (gi < n && gj <= gi) A[(size_t)gi * n + gj] -= acc[x][y][r];
+      }
 }
 
 // ------------------------------------------------- device SR pipeline
@@ -264,58 +335,53 @@ __global__ void k_sr_system(int nv, int nvr, const double* __restrict__ sc, cons
   }
 }
 
-// Forward substitution on the diagonal block [k0, k0+nb) of the lower factor
-// (one wave, lane i owns row k0+i; the rows below were already updated)
-__global__ __launch_bounds__(64) void k_trsv_lower_diag(int n, int k0, const double* __restrict__ L, double* x) {
-  const int nb = min(CH_NB, n - k0), i = threadIdx.x;
-  double xi = i < nb ? x[k0 + i] : 0.0;
-  for (int j = 0; j < nb; j++) {
-    double xj = __shfl(xi, j, 64);
-    xj /= L[(size_t)(k0 + j) * n + k0 + j];
-    if (i == j) xi = xj;
-    if (i > j && i < nb) xi = fma(-L[(size_t)(k0 + i) * n + k0 + j], xj, xi);
-  }
-  if (i < nb) x[k0 + i] = xi;
-}
-
-// x_i -= sum_{j in block} L(i, j) x_j for the rows below the block
-__global__ __launch_bounds__(256) void k_trsv_lower_update(int n, int k0, const double* __restrict__ L, double* x) {
-  __shared__ double xb[CH_NB];
-  const int nb = min(CH_NB, n - k0);
-  if (threadIdx.x < nb) xb[threadIdx.x] = x[k0 + threadIdx.x];
+// Forward substitution, block [k0, k0+nb): y_k = X_k b_k with X_k the inverse
+// of the diagonal block (every workgroup forms it from b, which no workgroup
+// of this launch writes), written to out by workgroup 0; then the rows below,
+// b_i -= sum_j L(i, k0 + j) y_j, each by the workgroup that owns it.
+__global__ __launch_bounds__(256) void k_trsv_fwd(int n, int k0, const double* __restrict__ L,
+                                                  const double* __restrict__ dinv, double* b, double* out) {
+  __shared__ double bk[CH_NB], yk[CH_NB];
+  const int nb = min(CH_NB, n - k0), t = threadIdx.x;
+  const double* D = dinv + (size_t)(k0 / CH_NB) * CH_NB * CH_NB;
+  if (t < CH_NB) bk[t] = t < nb ? b[k0 + t] : 0.0;
   __syncthreads();
-  const int i = k0 + nb + blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < CH_NB) {
+    double y = 0.0;
+    for (int j = 0; j <= t; j++) y = fma(D[t * CH_NB + j], bk[j], y);
+    yk[t] = y;
+    if (blockIdx.x == 0 && t < nb) out[k0 + t] = y;
+  }
+  __syncthreads();
+  const int i = k0 + nb + blockIdx.x * blockDim.x + t;
   if (i >= n) return;
   const double* row = L + (size_t)i * n + k0;
-  double t = 0.0;
-  for (int j = 0; j < nb; j++) t = fma(row[j], xb[j], t);
-  x[i] -= t;
+  double acc = 0.0;
+  for (int j = 0; j < nb; j++) acc = fma(row[j], yk[j], acc);
+  b[i] -= acc;
 }
 
-// Back substitution L^T x = y on the diagonal block (lane i owns row k0+i)
-__global__ __launch_bounds__(64) void k_trsv_upper_diag(int n, int k0, const double* __restrict__ L, double* x) {
-  const int nb = min(CH_NB, n - k0), i = threadIdx.x;
-  double xi = i < nb ? x[k0 + i] : 0.0;
-  for (int j = nb - 1; j >= 0; j--) {
-    double xj = __shfl(xi, j, 64);
-    xj /= L[(size_t)(k0 + j) * n + k0 + j];
-    if (i == j) xi = xj;
-    if (i < j) xi = fma(-L[(size_t)(k0 + j) * n + k0 + i], xj, xi);  // L^T(i, j) = L(j, i)
-  }
-  if (i < nb) x[k0 + i] = xi;
-}
-
-// x_i -= sum_{j in block} L(j, i) x_j for the rows above the block
-__global__ __launch_bounds__(256) void k_trsv_upper_update(int n, int k0, const double* __restrict__ L, double* x) {
-  __shared__ double xb[CH_NB];
-  const int nb = min(CH_NB, n - k0);
-  if (threadIdx.x < nb) xb[threadIdx.x] = x[k0 + threadIdx.x];
+// Back substitution L^T x = y, block [k0, k0+nb): x_k = X_k^T y_k, written to
+// out by workgroup 0; the rows above, y_i -= sum_j L(k0 + j, i) x_j.
+__global__ __launch_bounds__(256) void k_trsv_bwd(int n, int k0, const double* __restrict__ L,
+                                                  const double* __restrict__ dinv, double* y, double* out) {
+  __shared__ double yb[CH_NB], xk[CH_NB];
+  const int nb = min(CH_NB, n - k0), t = threadIdx.x;
+  const double* D = dinv + (size_t)(k0 / CH_NB) * CH_NB * CH_NB;
+  if (t < CH_NB) yb[t] = t < nb ? y[k0 + t] : 0.0;
   __syncthreads();
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < CH_NB) {
+    double x = 0.0;
+    for (int j = t; j < CH_NB; j++) x = fma(D[j * CH_NB + t], yb[j], x);
+    xk[t] = x;
+    if (blockIdx.x == 0 && t < nb) out[k0 + t] = x;
+  }
+  __syncthreads();
+  const int i = blockIdx.x * blockDim.x + t;
   if (i >= k0) return;
-  double t = 0.0;
-  for (int j = 0; j < nb; j++) t = fma(L[(size_t)(k0 + j) * n + i], xb[j], t);
-  x[i] -= t;
+  double acc = 0.0;
+  for (int j = 0; j < nb; j++) acc = fma(L[(size_t)(k0 + j) * n + i], xk[j], acc);
+  y[i] -= acc;
 }
 
 // r = D g - A x (one wave per row); with acc: x += 0 (plain residual)
@@ -379,30 +445,29 @@ void launch_row_kappa(int ns, int cplx_rows, const double2* psi, const double* r
 void launch_gemv(int n, const double* A, const double* x, double* y, hipStream_t st) {
   hipLaunchKernelGGL(k_gemv, dim3((n + 3) / 4), dim3(256), 0, st, n, A, x, y);
 }
-void launch_cholesky(int n, double* A, int* info, hipStream_t st) {
+void launch_cholesky(int n, double* A, double* dinv, int* info, hipStream_t st) {
   for (int k0 = 0; k0 < n; k0 += CH_NB) {
-    hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(256), 0, st, n, k0, A, info);
+    hipLaunchKernelGGL(k_potrf_inv, dim3(1), dim3(64), 0, st, n, k0, A, dinv, info);
     const int t0 = k0 + CH_NB;
     if (t0 >= n) break;
     const int rows = n - t0;
-    hipLaunchKernelGGL(k_trsm_panel, dim3((rows + 255) / 256), dim3(256), 0, st, n, k0, A);
+    hipLaunchKernelGGL(k_trsm_panel, dim3((rows + 63) / 64), dim3(256), 0, st, n, k0, A, (const double*)dinv);
     const int nt = (rows + 63) / 64;
     hipLaunchKernelGGL(k_syrk_trailing, dim3(nt * (nt + 1) / 2), dim3(256), 0, st, n, k0, t0, nt, A);
   }
 }
-void launch_chol_solve(int n, const double* L, double* x, hipStream_t st) {
-  // blocked: per block of CH_NB rows a one-wave substitution on the diagonal
-  // block, then a multi-workgroup update of the remaining rows
+
+void launch_chol_solve(int n, const double* L, const double* dinv, double* x, double* tmp, hipStream_t st) {
+  // forward: b = x (updated below each block) -> y in tmp
   for (int k0 = 0; k0 < n; k0 += CH_NB) {
-    hipLaunchKernelGGL(k_trsv_lower_diag, dim3(1), dim3(64), 0, st, n, k0, L, x);
-    const int rest = n - k0 - CH_NB;
-    if (rest > 0) hipLaunchKernelGGL(k_trsv_lower_update, dim3((rest + 255) / 256), dim3(256), 0, st, n, k0, L, x);
+    const int rest = std::max(n - k0 - CH_NB, 1);
+    hipLaunchKernelGGL(k_trsv_fwd, dim3((rest + 255) / 256), dim3(256), 0, st, n, k0, L, dinv, x, tmp);
   }
+  // backward: y = tmp (updated above each block) -> x
   const int nblk = (n + CH_NB - 1) / CH_NB;
-  for (int b = nblk - 1; b >= 0; b--) {
-    const int k0 = b * CH_NB;
-    hipLaunchKernelGGL(k_trsv_upper_diag, dim3(1), dim3(64), 0, st, n, k0, L, x);
-    if (k0 > 0) hipLaunchKernelGGL(k_trsv_upper_update, dim3((k0 + 255) / 256), dim3(256), 0, st, n, k0, L, x);
+  for (int blk = nblk - 1; blk >= 0; blk--) {
+    const int k0 = blk * CH_NB;
+    hipLaunchKernelGGL(k_trsv_bwd, dim3((std::max(k0, 1) + 255) / 256), dim3(256), 0, st, n, k0, L, dinv, tmp, x);
   }
 }
 
@@ -411,12 +476,12 @@ void launch_sr_pipeline(const SrPipeline& p, hipStream_t st) {
   hipLaunchKernelGGL(k_sr_prepare, dim3(1), dim3(1024), 0, st, n, p.nvr, p.S, p.fii, p.sc, p.fl);
   hipLaunchKernelGGL(k_sr_system, dim3(1024), dim3(256), 0, st, n, p.nvr, (const double*)p.sc, p.S, p.A, p.L, p.g,
                      p.x);
-  launch_cholesky(n, p.L, p.fl + 4, st);
-  launch_chol_solve(n, p.L, p.x, st);
+  launch_cholesky(n, p.L, p.dinv, p.fl + 4, st);
+  launch_chol_solve(n, p.L, p.dinv, p.x, p.ax, st);
   for (int it = 0; it < p.refine_iters; it++) {
     hipLaunchKernelGGL(k_residual, dim3((n + 3) / 4), dim3(256), 0, st, n, p.nvr, (const double*)p.sc, p.A, p.x,
                        p.g, p.r);
-    launch_chol_solve(n, p.L, p.r, st);
+    launch_chol_solve(n, p.L, p.dinv, p.r, p.ax, st);
     hipLaunchKernelGGL(k_axpy1, dim3((n + 255) / 256), dim3(256), 0, st, n, p.r, p.x);
   }
   hipLaunchKernelGGL(k_gemv, dim3((n + 3) / 4), dim3(256), 0, st, n, p.A, p.x, p.ax);

@@ -262,20 +262,145 @@ __global__ __launch_bounds__(256, 1) void k_syrk_packed(const unsigned char* __r
       }
 }
 
+// Split BF16 x 3 on 256 x 256 tiles: 8 waves of 64 x 128 (4 x 8 blocks),
+// twice the products per staged byte of the 128 x 128 kernel.  The FP32
+// accumulators (128 per lane) leave no room for FP64 ones, so every FLUSH
+// K-blocks they are added into the workgroup's own FP64 partial tile in
+// global memory (read-modify-write by the lane that owns each entry; the
+// first flush writes).
+constexpr int T2 = 256;
+struct SyrkSmem256 {
+  unsigned char p[2][2][T2 * LDP];
+};
+
+__global__ __launch_bounds__(512, 1) void k_syrk_bf16_256(const unsigned char* __restrict__ P, int nkb, int nvp,
+                                                          int ntile, int npair, int nsplit, int kb_per_split,
+                                                          int nlogical, double* __restrict__ part) {
+  __shared__ SyrkSmem256 L;
+  const int g = blockIdx.x, per = gridDim.x >> 3;
+  const int lg = (g & 7) * per + (g >> 3);
+  if (lg >= nlogical) return;
+  const int pair = lg % npair, split = lg / npair;
+  int p = pair, ti = 0;
+  while (p >= ntile - ti) { p -= ntile - ti; ti++; }
+  const int tj = ti + p;
+  const bool diag = ti == tj;
+  const int kb0 = split * kb_per_split, kb1 = min(nkb, kb0 + kb_per_split);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wi = wave >> 1, wj = wave & 1;  // rows wi * 64, columns wj * 128
+  const int npan = diag ? 1 : 2;
+  f32x4 a32[4][8];
+#pragma unroll
+  for (int x = 0; x < 4; x++)
+#pragma unroll
+    for (int y = 0; y < 8; y++) a32[x][y] = f32x4{0.f, 0.f, 0.f, 0.f};
+  u32x4 st[2][4];
+  auto gload = [&](int kb) {
+#pragma unroll
+    for (int pn = 0; pn < 2; pn++) {
+      if (pn < npan) {
+        const unsigned char* src = P + ((size_t)kb * nvp + (pn == 0 ? ti : tj) * T2) * COLB;
+#pragma unroll
+        for (int u = 0; u < 4; u++) st[pn][u] = *(const u32x4*)(src + (size_t)(tid + 512 * u) * 16);
+      }
+    }
+  };
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int pn = 0; pn < 2; pn++) {
+      if (pn < npan) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int q = tid + 512 * u;
+          *(u32x4*)&L.p[buf][pn][(q >> 3) * LDP + 16 * (q & 7)] = st[pn][u];
+        }
+      }
+    }
+  };
+  // the partial tile through a buffer resource: one per-lane byte offset,
+  // the (x, y, r) part of each address a scalar constant
+  const __amdgpu_buffer_rsrc_t prs = block_rsrc(part + ((size_t)split * npair + pair) * T2 * T2, T2 * T2 * 8);
+  const int m = lane & 15, ks = lane >> 4;
+  const int vbase = ((wi * 64 + 4 * ks) * T2 + wj * 128 + m) * 8;
+  bool written = false;
+  auto flush = [&]() {
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+      for (int y = 0; y < 8; y++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int soff = ((16 * x + r) * T2 + 16 * y) * 8;
+          double v = (double)a32[x][y][r];
+          if (written) v += bload1(prs, vbase, soff);
+          bstore1(v, prs, vbase, soff);
+          a32[x][y][r] = 0.f;
+        }
+    written = true;
+  };
+  const int ia = wi * 64 + m, jb = wj * 128 + m;
+  int nflush = 0;
+  if (kb0 < kb1) {
+    gload(kb0);
+    lstore(0);
+  }
+  __syncthreads();
+  int buf = 0;
+  for (int kb = kb0; kb < kb1; kb++) {
+    const bool more = kb + 1 < kb1;
+    if (more) gload(kb + 1);
+    const unsigned char* pa = L.p[buf][0];
+    const unsigned char* pb = L.p[buf][diag ? 0 : 1];
+    bf16x8 ah[4], al[4];
+#pragma unroll
+    for (int x = 0; x < 4; x++) {
+      ah[x] = *(const bf16x8*)(pa + (ia + 16 * x) * LDP + 16 * ks);
+      al[x] = *(const bf16x8*)(pa + (ia + 16 * x) * LDP + 64 + 16 * ks);
+    }
+#pragma unroll
+    for (int yh = 0; yh < 4; yh++) {
+      bf16x8 bh[2], bl[2];
+#pragma unroll
+      for (int y = 0; y < 2; y++) {
+        bh[y] = *(const bf16x8*)(pb + (jb + 16 * (2 * yh + y)) * LDP + 16 * ks);
+        bl[y] = *(const bf16x8*)(pb + (jb + 16 * (2 * yh + y)) * LDP + 64 + 16 * ks);
+      }
+#pragma unroll
+      for (int x = 0; x < 4; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++) {
+          f32x4& c = a32[x][2 * yh + y];
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[x], bh[y], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bl[y], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bh[y], c, 0, 0, 0);
+        }
+    }
+    if (++nflush == FLUSH) {
+      nflush = 0;
+      flush();
+    }
+    if (more) lstore(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+  }
+  if (nflush > 0 || !written) flush();
+}
+
 // Sum of the split partials of each tile pair (fixed order), scaled by
 // 1/norm, written to both triangles of S (row-major nv x nv).  In diagonal
 // tiles only the entries with i <= j are used, so S is exactly symmetric.
-__global__ void k_syrk_reduce(int nv, int ntile, int npair, int nsplit, const double* __restrict__ part,
+__global__ void k_syrk_reduce(int nv, int ts, int ntile, int npair, int nsplit, const double* __restrict__ part,
                               double inv_norm, double* __restrict__ S) {
   int p = blockIdx.x, ti = 0;
   while (p >= ntile - ti) { p -= ntile - ti; ti++; }
   const int tj = ti + p;
-  for (int e = threadIdx.x; e < TILE * TILE; e += blockDim.x) {
-    const int a = e / TILE, b = e - a * TILE;
-    const int i = ti * TILE + a, j = tj * TILE + b;
+  const size_t tt = (size_t)ts * ts;
+  for (int e = threadIdx.x; e < ts * ts; e += blockDim.x) {
+    const int a = e / ts, b = e - a * ts;
+    const int i = ti * ts + a, j = tj * ts + b;
     if (i >= nv || j >= nv || i > j) continue;
     double t = 0;
-    for (int z = 0; z < nsplit; z++) t += part[((size_t)z * npair + blockIdx.x) * TILE * TILE + e];
+    for (int z = 0; z < nsplit; z++) t += part[((size_t)z * npair + blockIdx.x) * tt + e];
     t *= inv_norm;
     S[(size_t)i * nv + j] = t;
     S[(size_t)j * nv + i] = t;
@@ -285,30 +410,37 @@ __global__ void k_syrk_reduce(int nv, int ntile, int npair, int nsplit, const do
 }  // namespace
 
 // ---------------------------------------------------------------- host side
-SyrkPlan syrk_plan(int precision, int nrows, int nv, int cus) {
+SyrkPlan syrk_plan(int precision, int nrows, int nv, int cus, int tile, int nsplit_want) {
   SyrkPlan pl;
   pl.mode = precision;
   pl.kb = kblock_rows(precision);
-  pl.nvp = (nv + TILE - 1) / TILE * TILE;
-  pl.ntile = pl.nvp / TILE;
+  // 256 x 256 tiles (split BF16 x 3 only) once the 128-tile grid has several
+  // tile pairs per CU
+  if (tile <= 0) tile = (precision == 2 && nv >= 1536) ? T2 : TILE;
+  if (precision != 2) tile = TILE;
+  pl.tile = tile;
+  pl.nvp = (nv + tile - 1) / tile * tile;
+  pl.ntile = pl.nvp / tile;
   pl.npair = pl.ntile * (pl.ntile + 1) / 2;
   pl.nkb = (nrows + pl.kb - 1) / pl.kb;
   // about two workgroups per CU, a multiple of 8 splits (one set per XCD)
   // unless the tile pairs alone fill the chip; at least 64 K-blocks a split
-  int nsplit = (2 * cus + pl.npair - 1) / pl.npair;
+  const int per_cu = tile == T2 ? 1 : 2;  // resident workgroups per CU
+  int nsplit = (per_cu * cus + pl.npair - 1) / pl.npair;
   if (nsplit > 1) nsplit = (nsplit + 7) / 8 * 8;
+  if (nsplit_want > 0) nsplit = nsplit_want;
   nsplit = std::max(1, std::min(nsplit, pl.nkb / 64));
   pl.kb_per_split = (pl.nkb + nsplit - 1) / nsplit;
   pl.nsplit = (pl.nkb + pl.kb_per_split - 1) / pl.kb_per_split;
   if (pl.nsplit < 1) pl.nsplit = 1;
   pl.packed_bytes = (size_t)pl.nkb * pl.nvp * COLB;
-  pl.part_doubles = (size_t)pl.nsplit * pl.npair * TILE * TILE;
+  pl.part_doubles = (size_t)pl.nsplit * pl.npair * tile * tile;
   return pl;
 }
 
 void launch_center_pack(const SyrkPlan& pl, int nrows, int nv, const double* orows, const double* kappa,
                         const double* obar, unsigned char* packed, hipStream_t st) {
-  const dim3 grid(pl.ntile, pl.nkb);
+  const dim3 grid(pl.nvp / TILE, pl.nkb);
   if (pl.mode == 2)
     hipLaunchKernelGGL(k_center_pack<2>, grid, dim3(256), 0, st, nrows, nv, pl.nvp, orows, kappa, obar, packed);
   else if (pl.mode == 1)
@@ -321,7 +453,10 @@ void launch_syrk_packed(const SyrkPlan& pl, int nv, const unsigned char* packed,
                         double* S, hipStream_t st) {
   const int nlogical = pl.npair * pl.nsplit;
   const int ngrid = (nlogical + 7) / 8 * 8;
-  if (pl.mode == 2)
+  if (pl.tile == T2)
+    hipLaunchKernelGGL(k_syrk_bf16_256, dim3(ngrid), dim3(512), 0, st, packed, pl.nkb, pl.nvp, pl.ntile, pl.npair,
+                       pl.nsplit, pl.kb_per_split, nlogical, part);
+  else if (pl.mode == 2)
     hipLaunchKernelGGL(k_syrk_packed<2>, dim3(ngrid), dim3(256), 0, st, packed, pl.nkb, pl.nvp, pl.ntile, pl.npair,
                        pl.nsplit, pl.kb_per_split, nlogical, part);
   else if (pl.mode == 1)
@@ -330,6 +465,6 @@ void launch_syrk_packed(const SyrkPlan& pl, int nv, const unsigned char* packed,
   else
     hipLaunchKernelGGL(k_syrk_packed<0>, dim3(ngrid), dim3(256), 0, st, packed, pl.nkb, pl.nvp, pl.ntile, pl.npair,
                        pl.nsplit, pl.kb_per_split, nlogical, part);
-  hipLaunchKernelGGL(k_syrk_reduce, dim3(pl.npair), dim3(256), 0, st, nv, pl.ntile, pl.npair, pl.nsplit, part,
-                     inv_norm, S);
+  hipLaunchKernelGGL(k_syrk_reduce, dim3(pl.npair), dim3(256), 0, st, nv, pl.tile, pl.ntile, pl.npair, pl.nsplit,
+                     part, inv_norm, S);
 }

@@ -43,6 +43,15 @@ module rbmpp_mod
     integer(c_int32_t) :: refine_iters, flags
   end type
 
+  type, bind(C), public :: rbmpp_tune_report
+    integer(c_int32_t) :: s_tile, s_nsplit, s_default_tile, s_default_nsplit
+    real(c_double)     :: s_build_s
+    integer(c_int32_t) :: measure_grid, measure_configurations
+    real(c_double)     :: measure_s
+  end type
+
+  integer(c_int), parameter, public :: RBMPP_TUNE_S_BUILD = 1, RBMPP_TUNE_MEASURE = 2
+
   type, bind(C), public :: rbmpp_sr_report
     real(c_double)     :: lambda, delw_max, scaling, sdiag_av, s_factor, residual, mean_abs_f
     integer(c_int32_t) :: k_nonzero, reserved
@@ -55,6 +64,9 @@ module rbmpp_mod
   public :: rbmpp_set_samples, rbmpp_get_samples, rbmpp_n_samples
   public :: rbmpp_get_walkers, rbmpp_get_walker_electrons, rbmpp_eval, rbmpp_ratio_exchange, rbmpp_local
   public :: rbmpp_get_smat, rbmpp_timings, rbmpp_get_measured, rbmpp_set_measured, rbmpp_local_noalloc
+  public :: rbmpp_tune
+  public :: rbmpp_batch_create, rbmpp_batch_make_sample, rbmpp_batch_sample_seconds
+  public :: rbmpp_batch_last_error, rbmpp_batch_destroy
   public :: rbmpp_error_message
 
   interface
@@ -198,6 +210,45 @@ module rbmpp_mod
       type(c_ptr), value :: h
       integer(c_int32_t), intent(out) :: ele_idx(*)
     end function
+
+    integer(c_int) function rbmpp_tune(h, what, rep) bind(C, name="rbmpp_tune")
+      import :: c_int, c_int32_t, c_ptr, rbmpp_tune_report
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: what
+      type(rbmpp_tune_report), intent(out) :: rep
+    end function
+
+    ! several handles sampled by one launch (handles: array of c_ptr)
+    integer(c_int) function rbmpp_batch_create(handles, n, b) bind(C, name="rbmpp_batch_create")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), intent(in) :: handles(*)
+      integer(c_int32_t), value :: n
+      type(c_ptr), intent(out) :: b
+    end function
+
+    integer(c_int) function rbmpp_batch_make_sample(b, first_iteration, shift, naccept) &
+        bind(C, name="rbmpp_batch_make_sample")
+      import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int32_t), intent(in) :: first_iteration(*)
+      real(c_double), intent(in) :: shift(*)
+      integer(c_int64_t), intent(out) :: naccept(*)
+    end function
+
+    real(c_double) function rbmpp_batch_sample_seconds(b) bind(C, name="rbmpp_batch_sample_seconds")
+      import :: c_double, c_ptr
+      type(c_ptr), value :: b
+    end function
+
+    type(c_ptr) function rbmpp_batch_last_error(b) bind(C, name="rbmpp_batch_last_error")
+      import :: c_ptr
+      type(c_ptr), value :: b
+    end function
+
+    subroutine rbmpp_batch_destroy(b) bind(C, name="rbmpp_batch_destroy")
+      import :: c_ptr
+      type(c_ptr), value :: b
+    end subroutine
 
     ! a measurement loaded for rbmpp_sr_step (orows = c_null_ptr: synthetic)
     integer(c_int) function rbmpp_set_measured(h, n, psi, eloc, rw, orows, seed) &

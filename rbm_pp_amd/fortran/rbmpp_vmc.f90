@@ -11,6 +11,10 @@
 ! (transverse_code/spin_projection.f90:30-38) and its reweighting-offset
 ! factor 0.003 (transverse_code/main.f90:451) in place of 0.001.
 !
+! mode 2 reads the run directory and writes the parameters it read with the
+! writer of optimized_W.txt and zqp_orbital_opt.dat, then stops (no device
+! work): the reader and writer against parameter.f90's.
+!
 ! mode 1 is the measurement run of Calc_Correlation (Calc_Correlation/
 ! main.f90:311 skips the optimisation, :499-640 measure): Nav iterations of
 ! energy, spin correlation Chi_sp and dimer correlation Chi_vbs with the
@@ -64,6 +68,12 @@ program rbmpp_vmc
 
   call read_inputs()
   nspw = (Nsample + walkers - 1) / walkers
+  if (mode == 2) then
+    ! the parameters as read (initialize_parameters) written back with the
+    ! writer of write_optimized_parameters; no device work
+    call write_optimized_parameters(packed_inputs())
+    stop
+  end if
 
   cfg%L = L; cfg%n_sym = P; cfg%alphac = alphac; cfg%alphar = alphar
   cfg%n_slater = nsl; cfg%spin_parity = spin_parity; cfg%J1 = J1; cfg%J2 = J2
@@ -109,6 +119,8 @@ program rbmpp_vmc
     write(6, '(a,2E20.5)')  '   wf_av, shift         :', wf_av, shift
     write(6, '(a,F20.10)')  '   norm                 :', norm / dble(walkers * nspw)
     write(10, *) it, etot, (etot2 - etot*etot) / (etot*etot)
+    flush(10)
+    flush(6)
     if (it > Nstep - Nav) then
       nav_chk = nav_chk + 1
       shift_av = shift_av + shift
@@ -152,6 +164,8 @@ program rbmpp_vmc
     e_data(it) = etot
     e2_data(it) = etot2 - etot*etot
     write(10, *) it, etot, (etot2 - etot*etot) / (etot*etot)
+    flush(10)
+    flush(6)
   end do
   close(10)
   call calc_av_er(Nav, e_data, e_av, e_er)
@@ -316,9 +330,16 @@ contains
       slater(k) = re
     end do
     close(u)
-    ! mean |f| = 0.25 (parameter.f90:121-140)
-    mean_abs = sum(abs(slater)) / dble(nsl)
-    slater = slater * (0.25d0 / mean_abs)
+    ! mean |f| = 0.25 (parameter.f90:121-140), the reference's arithmetic:
+    ! sequential sum, then f * 0.25 / mean
+    mean_abs = 0d0
+    do k = 1, nsl
+      mean_abs = mean_abs + abs(slater(k))
+    end do
+    mean_abs = mean_abs / dble(nsl)
+    do k = 1, nsl
+      slater(k) = slater(k) * 0.25d0 / mean_abs
+    end do
 
     allocate(wc(2, N, max(alphac, 1)), bc(2, max(alphac, 1)), wr(N, max(alphar, 1)), br(max(alphar, 1)))
     wc = 0d0; bc = 0d0; wr = 0d0; br = 0d0
@@ -342,6 +363,31 @@ contains
     end do
     close(u)
   end subroutine
+
+  ! the parameters read from the input files, packed as main.f90:263-294
+  function packed_inputs() result(g)
+    real(c_double), allocatable :: g(:)
+    integer :: f, iw, k
+    allocate(g(2*alphac*(N+1) + alphar*(N+1) + nsl))
+    k = 0
+    do f = 1, alphac
+      g(k+1) = bc(1, f); g(k+2) = bc(2, f); k = k + 2
+    end do
+    do f = 1, alphac
+      do iw = 1, N
+        g(k+1) = wc(1, iw, f); g(k+2) = wc(2, iw, f); k = k + 2
+      end do
+    end do
+    do f = 1, alphar
+      g(k+1) = br(f); k = k + 1
+    end do
+    do f = 1, alphar
+      do iw = 1, N
+        g(k+1) = wr(iw, f); k = k + 1
+      end do
+    end do
+    g(k+1:k+nsl) = slater(1:nsl)
+  end function
 
   ! parameter.f90:238-276, from the packed gamma (main.f90:263-294)
   subroutine write_optimized_parameters(g)

@@ -185,8 +185,15 @@ class Params:
 
 
 def renormalise_f(f: np.ndarray, slater_abs_av: float = 0.25) -> np.ndarray:
-    """parameter.f90:121-140: scale f so that mean|f| = slater_abs_av."""
-    return f * (slater_abs_av / np.mean(np.abs(f)))
+    """parameter.f90:121-140: scale f so that mean|f| = slater_abs_av, with the
+    reference's arithmetic (sequential sum of |f|, then f * slater_abs_av / mean)
+    so that the result is bit-identical to its reader's."""
+    f = np.asarray(f, np.float64)
+    r3 = 0.0
+    for v in f.tolist():
+        r3 += abs(v)
+    r3 /= float(len(f))
+    return f * slater_abs_av / r3
 
 
 def random_params(spec: ModelSpec, seed_w: int = 1, seed_f: int = 2,

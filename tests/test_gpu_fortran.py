@@ -98,3 +98,51 @@ def test_fortran_correlation_run_matches_python_mirror(tmp_path):
         np.testing.assert_allclose(vb[:, 1:], r["chi_vbs"].T, rtol=0, atol=1e-13 * max(1.0, np.abs(vb).max()))
         e = np.loadtxt(tmp_path / "energy_after_opt.txt")
         assert e[it - 1, 1] == pytest.approx(r["Etot"], rel=1e-12)
+
+
+def test_fortran_driver_against_oracle_loop(tmp_path, oracle_lib):
+    """Three SR iterations of the Fortran driver (4x4, 32 walkers) against the
+    same iterations restated on the CPU: the oracle's make_sample on each
+    walker's Philox stream (the draws the GPU sampler makes), its
+    calc_phys_quantity_opt over all samples and its SR step, with the shift
+    update of main.f90:441-451.  The energies of every iteration agree to
+    1e-10 (the first) and 1e-8 (later ones, whose parameters come from two
+    solves of the same system), and the final parameters to 1e-8."""
+    if not os.path.exists(DRIVER):
+        pytest.skip("Fortran driver not built (no amdflang at build time)")
+    W, nspw, K = 32, 12, 16
+    spec = lt.j1j2_model(4, alphar=1, J2=0.5)
+    spec.nsample, spec.nstep, spec.nav = W * nspw, 3, 1
+    spec.delta_tau, spec.s_factor0, spec.s_factor_scale = 0.05, 1e-3, 1.0
+    lt.write_run_dir(str(tmp_path), spec, lt.random_params(spec, seed_w=4, seed_f=5, sigma_w=0.05))
+    print(f"fortran driver vs oracle: 4x4 walkers={W} samples/walker={nspw}", flush=True)
+    out = subprocess.run([DRIVER, str(W), "0", str(K)], cwd=tmp_path, capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stderr + out.stdout[-2000:]
+    e_f = np.loadtxt(tmp_path / "Energy_vs_Iteration.txt")
+    spec2, p = lt.read_run_dir(str(tmp_path))
+    orc = oracle_lib.Oracle(spec2, p)
+    chains = [(oracle_lib.Rng("philox", 1, stream=w), orc.new_state()) for w in range(W)]
+    nfast = [None] * W
+    nwarm = max(30, nspw // 100)  # make_sample.f90:66
+    shift, wf_list, gam = 0.0, [0.0] * 10, orc.gamma()
+    for it in range(1, spec.nstep + 1):
+        xs = []
+        for w, (rng, st) in enumerate(chains):
+            o = orc.make_sample(rng, it == 1, nsample=nspw, nwarmup=nwarm, fixed_k=K, ref_draws=False,
+                                shift=shift, state=st, cond_guard=1e4, nfastup=nfast[w])
+            nfast[w] = o["nfastup"]
+            xs.append(o["x"])
+        m = orc.measure(np.concatenate(xs), shift=shift, want_s=True)
+        etot = m["Etot"] / m["norm"]
+        assert e_f[it - 1, 1] == pytest.approx(etot, rel=1e-10 if it == 1 else 1e-8), it
+        rc, gam, _, _ = orc.sr_step(etot, m["norm"], m["gvec"], m["ovec"], m["smat"], gam, s_factor0=1e-3)
+        assert rc == 0
+        orc.set_gamma(gam)
+        wf_list = wf_list[1:] + [m["wf_av"] / (W * nspw)]
+        shift = sum(wf_list) / 10.0 * 1e-3 / spec.N / spec.N
+    Wc, bc, Wr, br = lt.read_w(str(tmp_path / "optimized_W.txt"), spec.alphac, spec.alphar, spec.N)
+    f = lt.read_zqp_orbital(str(tmp_path / "zqp_orbital_opt.dat"), spec.n_slater)
+    po = lt.Params.from_gamma(spec, gam)
+    np.testing.assert_allclose(Wr, po.Wr, rtol=0, atol=1e-8 * np.abs(po.Wr).max())
+    np.testing.assert_allclose(f, po.f, rtol=0, atol=1e-8 * np.abs(po.f).max())

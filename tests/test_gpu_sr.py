@@ -108,3 +108,49 @@ def test_sr_step_16x16_synthetic_three_precisions():
     for name, bound in (("fp32", 1e-5), ("bf16x3", 1e-4)):
         err = np.linalg.norm(res[name][0] - S64) / np.linalg.norm(S64)
         assert err < bound, (name, err)
+
+
+@pytest.mark.parametrize("prec,bound", [("S_FP64", 1e-13), ("S_BF16X3", 2e-5)])
+def test_tune_keeps_results(prec, bound):
+    """rbmpp_tune only changes launch shapes: S built with the chosen tile and
+    split count agrees with the default build to summation order (FP64) or
+    to the precision's own error (split BF16 x 3, whose 256-wide tiles flush
+    FP32 sums differently), and a measurement after tuning the estimator
+    grid gives the same energy bit for bit and the same gradient to rounding
+    (its orbital-derivative bins are summed with LDS atomics, whose order
+    varies from run to run)."""
+    from rbm_pp_amd import api
+    spec = lt.j1j2_model(10, alphar=4, J2=0.5)
+    p = lt.random_params(spec, seed_w=1, seed_f=2, sigma_w=0.05)
+    n = 2048
+    xs = _configs(spec.N, n, seed=44)
+    eng = api.Engine(spec, params=p, n_walkers=1, n_samples=n, s_precision=getattr(api, prec))
+    eng.set_samples(xs)
+    r0 = eng.calc_phys_quantity_opt(shift=1e-3)
+    S0 = eng.smat()
+    rep = eng.tune(api.TUNE_S_BUILD)
+    assert rep["s_tile"] in (128, 256) and rep["s_nsplit"] >= 1 and rep["s_build_s"] > 0
+    S1 = eng.smat()
+    assert np.linalg.norm(S1 - S0) / np.linalg.norm(S0) < bound
+    assert np.array_equal(S1, S1.T)
+    rep = eng.tune(api.TUNE_MEASURE)
+    assert rep["measure_grid"] in (256, 512, 1024, 2048, 4096)
+    r1 = eng.calc_phys_quantity_opt(shift=1e-3)
+    assert r1["Etot"] == r0["Etot"]
+    assert np.linalg.norm(r1["gvec"] - r0["gvec"]) <= 1e-12 * np.linalg.norm(r0["gvec"])
+
+
+def test_bf16x3_256_tiles_16x16():
+    """Split BF16 x 3 with 256-wide tiles (the default at Nv = 5136, FP32 sums
+    flushed into FP64 partial tiles in global memory) against the FP64 S of
+    the same synthetic 16x16 measurement: within 1e-4, exactly symmetric."""
+    from rbm_pp_amd import api
+    spec = lt.j1j2_model(16, alphar=16, J2=0.55)
+    p = lt.random_params(spec, seed_w=1, seed_f=2, sigma_w=0.01)
+    eng = api.Engine(spec, params=p, n_walkers=1, n_samples=1, s_precision=api.S_FP64)
+    eng.set_measured(8192, seed=9)
+    S64 = eng.smat()
+    eng.set_s_precision(api.S_BF16X3)
+    S256 = eng.smat()
+    assert np.linalg.norm(S256 - S64) / np.linalg.norm(S64) < 1e-4
+    assert np.array_equal(S256, S256.T)
