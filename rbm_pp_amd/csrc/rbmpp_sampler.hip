@@ -16,6 +16,7 @@
 // per proposal).  All threads draw the same Philox numbers, so the control
 // flow is uniform across the workgroup and runs on the scalar unit.
 #include <algorithm>
+#include <cstdlib>
 
 #include "rbmpp_kernels.h"
 #include "rbmpp_measure.h"
@@ -59,24 +60,33 @@ void launch_slater(const DevModel& m, const double* f, double* se, int* bad, hip
 // shared-memory layout of the sampler workgroup.  Each wave (one copy)
 // keeps a per-row quad {s_j|a_j, t_j|b_j, p_j, q_j} for broadcasts; the quads
 // share storage with the LDS recompute matrix, which is only live between
-// proposals.
-template <int NMAX, bool LDSMAT>
+// proposals.  RL > 0 (block form without the fused measurement): rows
+// [RR, RR + RL) of every copy's F^-1 stay in LDS between proposals; the
+// recompute's stages overlap them (they are reloaded after every recompute),
+// and the amplitude tables are sized for C * ar <= 256, C * ac <= 64 and
+// ar N + 2 ac N <= 512 (the RBM tables stay in global memory past that).
+template <int NMAX, bool LDSMAT, int RL = 0>
 struct SweepSmem {
   static constexpr int NW = RBMPP_CMAX;
-  static constexpr int VEC = NW * 4 * NMAX;
+  static constexpr int HS = NMAX / 2;  // row stride of the resident F^-1 rows
+  static constexpr int VEC = RL > 0 ? NW * RL * HS : NW * 4 * NMAX;
+  static constexpr int MRL = RL > 0 ? 256 : RBMPP_MRMAX;  // real angles held
+  static constexpr int MCL = RL > 0 ? 64 : RBMPP_MCMAX;   // complex angles held
   static constexpr int MAT = LDSMAT ? (NMAX < RBMPP_NLDS ? NMAX * NMAX : RBMPP_NLDS * RBMPP_NLDS) : 0;
   // fused measurement: one copy's X^-1 (at offset 0) + per-wave a, b vectors
   static constexpr int MWAVES = 8;
-  static constexpr int MEAS = LDSMAT ? MAT + MWAVES * 2 * NMAX : 0;
+  // (RL > 0: the block recompute's stages, or the N x N recompute matrix)
+  static constexpr int MEAS =
+      LDSMAT ? (RL > 0 ? (NW * 8 * HS > MAT ? NW * 8 * HS : MAT) : MAT + MWAVES * 2 * NMAX) : 0;
   static constexpr int UNION = VEC > MEAS ? VEC : MEAS;
-  static constexpr int MB = LDSMAT ? 4 * NMAX : 1;  // bonds / orbital bins of the fused measurement
+  static constexpr int MB = (LDSMAT && RL == 0) ? 4 * NMAX : 1;  // bonds / orbital bins of the fused measurement
   double u[UNION];
   double2 psib[MB];
   double opp[2][MB];
   short bi1[MB], bi2[MB];
   int nb;
-  double thr[RBMPP_MRMAX];
-  double2 thc[RBMPP_MCMAX];
+  double thr[MRL];
+  double2 thc[MCL];
   double pf[RBMPP_CMAX], pfn[RBMPP_CMAX];
   double2 psiq[RBMPP_CMAX];
   double gjbuf[4 * NMAX];
@@ -92,7 +102,7 @@ struct SweepSmem {
   double gselv[2];
   // RBM tables staged for the proposals' amplitudes (Wr, Wc as doubles;
   // W_map; sublattice flags), used when they fit
-  static constexpr int TW = LDSMAT ? 1536 : 2;
+  static constexpr int TW = LDSMAT ? (RL > 0 ? 512 : 1536) : 2;
   static constexpr int TM = LDSMAT ? (RBMPP_CMAX / 2) * NMAX : 1;
   alignas(16) double tw[TW];
   int tmap[TM];
@@ -101,8 +111,8 @@ struct SweepSmem {
 };
 
 
-template <int NMAX, bool LDSMAT>
-__device__ __attribute__((noinline)) void wg_recompute(const DevModel& m, SweepSmem<NMAX, LDSMAT>& s, double* Mw,
+template <int NMAX, bool LDSMAT, int RL>
+__device__ __attribute__((noinline)) void wg_recompute(const DevModel& m, SweepSmem<NMAX, LDSMAT, RL>& s, double* Mw,
                              double* gscratch, int check, double* drift_pf, double* drift_inv) {
   const int tid = threadIdx.x, nth = blockDim.x;
   const int N = m.N, C = m.C;
@@ -124,7 +134,7 @@ __device__ __attribute__((noinline)) void wg_recompute(const DevModel& m, SweepS
     const int lane = tid & 63, wid = tid >> 6, np = m.npair;
     // pivot rows in gjbuf (4 NMAX doubles = 16 x NMAX/2 ints), the write-back
     // stages in u (16 x 8 x NMAX/2 doubles)
-    static_assert(!LDSMAT || (SweepSmem<NMAX, LDSMAT>::UNION >= RBMPP_CMAX * 8 * (NMAX / 2) &&
+    static_assert(!LDSMAT || (SweepSmem<NMAX, LDSMAT, RL>::UNION >= RBMPP_CMAX * 8 * (NMAX / 2) &&
                               4 * NMAX * 2 >= RBMPP_CMAX * (NMAX / 2)),
                   "stages and pivot rows of the block recompute fit u and gjbuf");
     int* piv = (int*)s.gjbuf + (size_t)wid * (NMAX / 2);
@@ -289,8 +299,8 @@ __device__ __attribute__((noinline)) void wg_recompute(const DevModel& m, SweepS
 }
 
 // x_to_Ele_Idx_Ele_Cfg (interface_for_PP.f90:5-33), serial on thread 0
-template <int NMAX, bool LDSMAT>
-__device__ void wg_x_to_ele(const DevModel& m, SweepSmem<NMAX, LDSMAT>& s) {
+template <int NMAX, bool LDSMAT, int RL>
+__device__ void wg_x_to_ele(const DevModel& m, SweepSmem<NMAX, LDSMAT, RL>& s) {
   if (threadIdx.x == 0) {
     int i1 = 0, i2 = m.npair;
     for (int i = 0; i < m.N; i++) {
@@ -304,8 +314,8 @@ __device__ void wg_x_to_ele(const DevModel& m, SweepSmem<NMAX, LDSMAT>& s) {
 // make_initial_x (make_sample.f90:298-362): random Sz=0 configuration with
 // |psi| > 1e-300.  The serial draws run on thread 0; the counter is then
 // broadcast so every thread's stream copy stays in step.
-template <int NMAX, bool LDSMAT>
-__device__ __attribute__((noinline)) int wg_initial_x(const DevModel& m, SweepSmem<NMAX, LDSMAT>& s, Rng& rng, double* Mw,
+template <int NMAX, bool LDSMAT, int RL>
+__device__ __attribute__((noinline)) int wg_initial_x(const DevModel& m, SweepSmem<NMAX, LDSMAT, RL>& s, Rng& rng, double* Mw,
                             double* gscratch) {
   for (int attempt = 0; attempt < 1000; attempt++) {
     if (threadIdx.x == 0) {
@@ -340,8 +350,8 @@ __device__ __attribute__((noinline)) int wg_initial_x(const DevModel& m, SweepSm
 // half scatter the Pfaffian derivative into the orbital bins
 // (calculate_derivative_Pfaffian_child).  Writes the reweighted E_loc,
 // amplitude, reweight factor and O row(s) of sample `row`.
-template <int NMAX, bool LDSMAT>
-__device__ __attribute__((noinline)) void wg_measure_sample(const DevModel& m, SweepSmem<NMAX, LDSMAT>& s,
+template <int NMAX, bool LDSMAT, int RL>
+__device__ __attribute__((noinline)) void wg_measure_sample(const DevModel& m, SweepSmem<NMAX, LDSMAT, RL>& s,
                                                             const double* Mw, size_t row, const SweepArgs& a) {
   constexpr int KMAX = NMAX / RBMPP_WAVE;
   const int tid = threadIdx.x, nth = blockDim.x;
@@ -349,7 +359,7 @@ __device__ __attribute__((noinline)) void wg_measure_sample(const DevModel& m, S
   const int wid = __builtin_amdgcn_readfirstlane(tid >> 6), nwv = nth >> 6;
   const int N = m.N, C = m.C, np = m.npair;
   const size_t NN = (size_t)N * N;
-  const int nbw = nwv >= 2 ? min(nwv / 2, SweepSmem<NMAX, LDSMAT>::MWAVES) : 1;  // bond waves
+  const int nbw = nwv >= 2 ? min(nwv / 2, SweepSmem<NMAX, LDSMAT, RL>::MWAVES) : 1;  // bond waves
   if (tid == 0) s.nb = bond_list(m, s);
   for (int k = tid; k < 4 * NMAX; k += nth) {
     s.psib[k] = make_double2(0.0, 0.0);
@@ -419,7 +429,7 @@ __device__ __attribute__((noinline)) void wg_measure_sample(const DevModel& m, S
       }
     }
     if (bond_wave && wid < nbw) {
-      double* va = s.u + SweepSmem<NMAX, LDSMAT>::MAT + (size_t)wid * 2 * NMAX;
+      double* va = s.u + SweepSmem<NMAX, LDSMAT, RL>::MAT + (size_t)wid * 2 * NMAX;
       double* vb = va + NMAX;
       for (int b = wid; b < nb; b += nbw) {
         const int i1 = s.bi1[b], i2 = s.bi2[b];
@@ -542,6 +552,22 @@ __device__ __attribute__((noinline)) void blk_update_pass(double* Mq, int np, in
   }
 }
 
+// The pending update of one row g = B[d][lane] (x_d by a wave sum), and of
+// four rows with their four wave sums taken together.
+__device__ __forceinline__ double blk_upd1(double g, double cv, int ua, double al, double be) {
+  const double xd = readlane_d(wave_sum_to63(g * cv), 63);
+  return fma(readlane_d(g, ua), al, fma(xd, be, g));
+}
+__device__ __forceinline__ void blk_upd4(double& g0, double& g1, double& g2, double& g3, double cv, int ua,
+                                         double al, double be) {
+  const double v = wave_sum4(g0 * cv, g1 * cv, g2 * cv, g3 * cv);
+  const double x0 = readlane_d(v, 15), x1 = readlane_d(v, 47), x2 = readlane_d(v, 31), x3 = readlane_d(v, 63);
+  g0 = fma(readlane_d(g0, ua), al, fma(x0, be, g0));
+  g1 = fma(readlane_d(g1, ua), al, fma(x1, be, g1));
+  g2 = fma(readlane_d(g2, ua), al, fma(x2, be, g2));
+  g3 = fma(readlane_d(g3, ua), al, fma(x3, be, g3));
+}
+
 // Every thread of the sweep draws the same Philox numbers and reads the same
 // shared configuration, so these values are equal in all lanes.  Saying so
 // keeps the counter, the draws and the branches on them on the scalar unit
@@ -555,8 +581,19 @@ __device__ __forceinline__ uint64_t uni64(uint64_t v) {
 // make_sample for walker w of a.ws (one workgroup of one wave per copy).
 // Called by k_sweep (one model) and k_sweep_batch (models of several
 // lattices in one launch); s is the workgroup's shared memory.
-template <int NMAX, bool LDSMAT, bool MEAS, bool BLK>
-__device__ __forceinline__ void sweep_walker(const SweepArgs& a, const int w, SweepSmem<NMAX, LDSMAT>& s) {
+//
+// RR, RL > 0 (block form, no fused measurement, RBMPP_CMAX copies): rows
+// [0, RR) of each copy's F^-1 stay in the wave's registers (lane u holds
+// column u) and rows [RR, RR + RL) in LDS for the whole launch; only the rest
+// is streamed from global memory by each ratio pass.  Global memory is made
+// current before a recompute with drift check and at the end of the launch,
+// and the resident rows are reloaded after every recompute.
+template <int NMAX, bool LDSMAT, bool MEAS, bool BLK, int RR = 0, int RL = 0>
+__device__ __forceinline__ void sweep_walker(const SweepArgs& a, const int w, SweepSmem<NMAX, LDSMAT, RL>& s) {
+  constexpr bool RES = RR + RL > 0;
+  static_assert(!RES || (BLK && !MEAS && LDSMAT), "resident F^-1 rows: block form without the fused measurement");
+  static_assert(RR % 4 == 0, "register rows are updated four at a time");
+  constexpr int HS = SweepSmem<NMAX, LDSMAT, RL>::HS;
   constexpr int KD = NMAX >= 128 ? NMAX / 128 : 1;
   constexpr int UN = 8;  // rows in flight per wave
   const DevModel& m = *a.m;
@@ -580,7 +617,7 @@ __device__ __forceinline__ void sweep_walker(const SweepArgs& a, const int w, Sw
   for (int g = tid; g < P; g += nth) s.sgn[g] = a.ws.sgn[(size_t)w * P + g];
   RbmTables tb = model_tables(m);
   if constexpr (LDSMAT) {
-    using SS = SweepSmem<NMAX, LDSMAT>;
+    using SS = SweepSmem<NMAX, LDSMAT, RL>;
     const int nwr = m.ar * N, nwc = 2 * m.ac * N;
     if (nwr + nwc <= SS::TW && P * N <= SS::TM) {
       for (int i = tid; i < nwr; i += nth) s.tw[i] = m.Wr[i];
@@ -619,7 +656,7 @@ __device__ __forceinline__ void sweep_walker(const SweepArgs& a, const int w, Sw
   int nrec = 0;
   // this wave's projection copy and its broadcast quads
   const int q = __builtin_amdgcn_readfirstlane(wid);  // wave-uniform
-  double* wv = s.u + (size_t)wid * 4 * NMAX;
+  double* wv = s.u + (size_t)wid * 4 * NMAX;  // (unused by the resident form)
   double* Mq = Mw + (size_t)q * fold_size(N);
   const double* Sq = m.se + (size_t)q * 4 * NN;
   const int H = N >> 1;
@@ -631,6 +668,52 @@ __device__ __forceinline__ void sweep_walker(const SweepArgs& a, const int w, Sw
   bool pend = false;
   int pua = 0;
   double pal = 0.0, pbe = 0.0, pcv = 0.0;
+  // resident rows: breg (registers), lres (LDS, row d - RR at (d - RR) * HS);
+  // the host launches this form only with npair >= RR + RL
+  double breg[RR > 0 ? RR : 1];
+  double* lres = s.u + (size_t)q * RL * HS;
+  const int bnp = m.npair;
+  const bool blive = lane < bnp;
+  const int blo = blive ? lane : 0;  // an in-range column for the lanes past npair
+  auto res_load = [&]() {
+    if constexpr (RES) {
+      const __amdgpu_buffer_rsrc_t rs = block_rsrc(Mq, bnp * bnp * 8);
+      const int vo = blive ? 8 * lane : RBMPP_OOB;
+#pragma unroll
+      for (int d = 0; d < RR; d++) breg[d] = bload1(rs, vo, d * 8 * bnp);
+#pragma unroll
+      for (int d = 0; d < RL; d++) {
+        const double v = bload1(rs, vo, (RR + d) * 8 * bnp);
+        if (lane < HS) lres[d * HS + lane] = v;
+      }
+    }
+  };
+  // global memory made current: the pending update applied everywhere and
+  // the resident rows written back
+  auto res_flush = [&]() {
+    if constexpr (RES) {
+      const __amdgpu_buffer_rsrc_t rs = block_rsrc(Mq, bnp * bnp * 8);
+      const int vo = blive ? 8 * lane : RBMPP_OOB;
+      const int rowb = 8 * bnp;
+      if (pend) {
+#pragma unroll
+        for (int d = 0; d < RR; d += 4) blk_upd4(breg[d], breg[d + 1], breg[d + 2], breg[d + 3], pcv, pua, pal, pbe);
+      }
+#pragma unroll
+      for (int d = 0; d < RR; d++) bstore1(breg[d], rs, vo, d * rowb);
+      for (int d = 0; d < RL; d++) {
+        const double v = lres[d * HS + blo];
+        double g = blive ? v : 0.0;
+        if (pend) g = blk_upd1(g, pcv, pua, pal, pbe);
+        bstore1(g, rs, vo, (RR + d) * rowb);
+      }
+      if (pend)
+        for (int d = RR + RL; d < bnp; d++)
+          bstore1(blk_upd1(bload1(rs, vo, d * rowb), pcv, pua, pal, pbe), rs, vo, d * rowb);
+      pend = false;
+    }
+  };
+  res_load();
 
   for (int i0 = -a.nwarm; i0 <= a.nsamp && status == WS_OK; i0++) {
     // ---- periodic recompute (make_sample.f90:74-128)
@@ -638,7 +721,8 @@ __device__ __forceinline__ void sweep_walker(const SweepArgs& a, const int w, Sw
     if (nacc_tmp > limit) {
       if constexpr (BLK) {
         // the drift check compares with the fast-updated F^-1
-        if (pend && q < C) blk_update_pass(Mq, m.npair, lane, pcv, pua, pal, pbe);
+        if constexpr (RES) res_flush();
+        else if (pend && q < C) blk_update_pass(Mq, m.npair, lane, pcv, pua, pal, pbe);
         pend = false;
       }
       double dpf = 0.0, dinv = 0.0;
@@ -651,6 +735,7 @@ __device__ __forceinline__ void sweep_walker(const SweepArgs& a, const int w, Sw
         else nfast = min(100, (int)llround((double)nfast * 1.1));
       }
       if (status) break;
+      res_load();
     }
     for (int iu = 0; iu < a.nupdate; iu++) {
       rng.ctr = uni64(rng.ctr);
@@ -671,6 +756,7 @@ __device__ __forceinline__ void sweep_walker(const SweepArgs& a, const int w, Sw
           wg_recompute(m, s, Mw, gscr, 0, nullptr, nullptr);
           if (uni(s.status)) { status = uni(s.status); break; }
         }
+        res_load();
       }
       // ---- exchange candidate (make_sample.f90:244-294)
       int i1 = -1, i2 = -1;
@@ -732,18 +818,93 @@ __device__ __forceinline__ void sweep_walker(const SweepArgs& a, const int w, Sw
             rv = lane == db ? S2[s1 + N] - S1[s2 + N] : S2[sd + N] - S1[sd + N];
             bcv = lane == bua ? 0.0 : T2[su] - T1[su];
           }
-          double* rt = wv;  // r(d) for every d, zero past np
-          rt[lane] = rv;
-          wave_sync();
+          double* rt = wv;  // r(d) for every d, zero past np (the resident form reads lane d)
+          if constexpr (!RES) {
+            rt[lane] = rv;
+            wave_sync();
+          }
           const __amdgpu_buffer_rsrc_t rs = block_rsrc(Mq, np * np * 8);
           const int vo = lane < np ? 8 * lane : RBMPP_OOB;
           const int rowb = 8 * np;
+          double y = 0.0;
+          if constexpr (RES) {
+            // the same pass over the three tiers: registers, LDS, global
+            if (pend) {
+#pragma unroll
+              for (int d = 0; d < RR; d += 4)
+                blk_upd4(breg[d], breg[d + 1], breg[d + 2], breg[d + 3], pcv, pua, pal, pbe);
+            }
+            double bsel = 0.0;
+#pragma unroll
+            for (int d = 0; d < RR; d++) {
+              y = fma(readlane_d(rv, d), breg[d], y);
+              bsel = d == db ? breg[d] : bsel;
+            }
+#pragma unroll
+            for (int d0 = 0; d0 < RL; d0 += 4) {
+              double g[4];
+#pragma unroll
+              for (int j = 0; j < 4; j++) {
+                if (d0 + j < RL) {
+                  const double v = lres[(d0 + j) * HS + blo];
+                  g[j] = blive ? v : 0.0;
+                }
+              }
+              if (pend) {
+                if (d0 + 4 <= RL) {
+                  blk_upd4(g[0], g[1], g[2], g[3], pcv, pua, pal, pbe);
+                } else {
+#pragma unroll
+                  for (int j = 0; j < 4; j++)
+                    if (d0 + j < RL) g[j] = blk_upd1(g[j], pcv, pua, pal, pbe);
+                }
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                  if (d0 + j < RL && blive) lres[(d0 + j) * HS + lane] = g[j];
+              }
+#pragma unroll
+              for (int j = 0; j < 4; j++)
+                if (d0 + j < RL) y = fma(readlane_d(rv, RR + d0 + j), g[j], y);
+            }
+            constexpr int UG = 16;  // streamed rows in flight
+            int d = RR + RL;
+            for (; d + UG <= np; d += UG) {
+              double g[UG];
+#pragma unroll
+              for (int u = 0; u < UG; u++) g[u] = bload1(rs, vo, (d + u) * rowb);
+              if (pend) {
+#pragma unroll
+                for (int u = 0; u < UG; u += 4) blk_upd4(g[u], g[u + 1], g[u + 2], g[u + 3], pcv, pua, pal, pbe);
+#pragma unroll
+                for (int u = 0; u < UG; u++) bstore1(g[u], rs, vo, (d + u) * rowb);
+              }
+#pragma unroll
+              for (int u = 0; u < UG; u++) y = fma(readlane_d(rv, d + u), g[u], y);
+            }
+            for (; d < np; d++) {
+              double g = bload1(rs, vo, d * rowb);
+              if (pend) {
+                g = blk_upd1(g, pcv, pua, pal, pbe);
+                bstore1(g, rs, vo, d * rowb);
+              }
+              y = fma(readlane_d(rv, d), g, y);
+            }
+            // row db as updated above (LDS and global rows: each lane reads
+            // back its own store)
+            if (db < RR) {
+              brow = bsel;
+            } else if (db < RR + RL) {
+              const double v = lres[(db - RR) * HS + blo];
+              brow = blive ? v : 0.0;
+            } else {
+              brow = bload1(rs, vo, db * rowb);
+            }
+          } else {
           // one pass over the rows of F^-1: the pending update of the last
           // accept first (stored back), then y += r(d) B[d]; row db is read
           // again after the pass (a reload is cheaper than selecting it from
           // every row)
           constexpr int UB = 25;  // rows in flight (N = 100: two groups)
-          double y = 0.0;
           int d = 0;
           for (; d + UB <= np; d += UB) {
             double g[UB];
@@ -786,6 +947,7 @@ __device__ __forceinline__ void sweep_walker(const SweepArgs& a, const int w, Sw
           }
           // row db as updated above (each lane reads back its own store)
           brow = bload1(rs, vo, db * rowb);
+          }
           pend = false;
           by = y;
           bgam = readlane_d(y, bua);
@@ -1122,6 +1284,7 @@ __device__ __forceinline__ void sweep_walker(const SweepArgs& a, const int w, Sw
           if (tid == 0) s.guard = 0;
           __syncthreads();
           if (status) break;
+          res_load();
         }
       }
     }
@@ -1141,7 +1304,8 @@ __device__ __forceinline__ void sweep_walker(const SweepArgs& a, const int w, Sw
       }
     }
   }
-  if constexpr (BLK) {
+  if constexpr (RES) res_flush();
+  else if constexpr (BLK) {
     if (pend && q < C) blk_update_pass(Mq, m.npair, lane, pcv, pua, pal, pbe);
   }
   __syncthreads();
@@ -1166,13 +1330,20 @@ __device__ __forceinline__ void sweep_walker(const SweepArgs& a, const int w, Sw
   }
 }
 
-template <int NMAX, bool LDSMAT, bool MEAS, bool BLK>
+template <int NMAX, bool LDSMAT, bool MEAS, bool BLK, int RR = 0, int RL = 0>
 __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
-  __shared__ SweepSmem<NMAX, LDSMAT> s;
+  __shared__ SweepSmem<NMAX, LDSMAT, RL> s;
   const int w = blockIdx.x;
   if (w >= a.nwalk) return;
-  sweep_walker<NMAX, LDSMAT, MEAS, BLK>(a, w, s);
+  sweep_walker<NMAX, LDSMAT, MEAS, BLK, RR, RL>(a, w, s);
 }
+
+// Resident form for block-form models with 16 copies, N <= 104 and N/2 >= 22:
+// rows [0, 22) of every copy's F^-1 in LDS, the rest streamed.  (Rows kept in
+// registers as well spill: the sweep already fills the 128 registers that a
+// 1024-thread workgroup allows.)
+constexpr int RES_NMAX = 104, RES_RR = 0, RES_RL = 22;
+static_assert(sizeof(SweepSmem<RES_NMAX, true, RES_RL>) <= 160 * 1024, "resident rows fit the LDS");
 
 // Walkers of several models (lattices of different sizes, couplings and
 // parameters) in one launch: workgroup b belongs to the entry whose block
@@ -1231,12 +1402,27 @@ static void launch_sweep_t(int C, bool blk, const SweepArgs& a, hipStream_t st) 
   hipLaunchKernelGGL((k_sweep<NMAX, LDSMAT, false, false>), dim3(a.nwalk), dim3(64 * C), 0, st, a);
 }
 
+// RBMPP_SWEEP_RESIDENT=1 in the environment selects the resident form where
+// it applies.  It is not the default: at 10x10 it takes 3.31 s per 80-sweep
+// launch against 2.68 s for the streaming form (the pass over the LDS rows
+// adds to the latency of the streamed rows instead of hiding it).
+static bool resident_enabled() {
+  const char* e = getenv("RBMPP_SWEEP_RESIDENT");
+  return e && e[0] == '1';
+}
+
 int launch_sweep(const DevModel& hm, const SweepArgs& a, hipStream_t st) {
   const int N = hm.N;
   if (hm.C > RBMPP_CMAX || hm.mr > RBMPP_MRMAX || hm.mc > RBMPP_MCMAX || (N & 1)) return -1;
   if (N > RBMPP_NLDS && !a.ws.scratch) return -1;
   // block form needs the register recompute (16 waves, N <= RBMPP_NLDS)
   const bool blk = hm.block && hm.C == RBMPP_CMAX && N <= RBMPP_NLDS;
+  using RS = SweepSmem<RES_NMAX, true, RES_RL>;
+  if (blk && !a.measure && N <= RES_NMAX && hm.npair >= RES_RR + RES_RL && hm.C * hm.ar <= RS::MRL &&
+      hm.C * hm.ac <= RS::MCL && resident_enabled()) {
+    hipLaunchKernelGGL((k_sweep<RES_NMAX, true, false, true, RES_RR, RES_RL>), dim3(a.nwalk), dim3(1024), 0, st, a);
+    return 0;
+  }
   if (N <= 64) launch_sweep_t<64, true>(hm.C, blk, a, st);
   else if (N <= RBMPP_NLDS) launch_sweep_t<128, true>(hm.C, blk, a, st);
   else if (N <= 256) launch_sweep_t<256, false>(hm.C, false, a, st);

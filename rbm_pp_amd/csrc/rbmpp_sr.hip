@@ -90,59 +90,69 @@ __global__ void k_gemv(int n, const double* __restrict__ A, const double* __rest
 }
 
 // ------------------------------------------------------- blocked Cholesky
-// Right-looking, lower triangle of a row-major SPD matrix, block NB=32.
+// Right-looking, lower triangle of a row-major SPD matrix, block NB=64.
 #define CH_NB 64
 
-// Factor the diagonal block [k0, k0+nb)^2 and invert its factor: one wave,
-// the block in LDS.  Factor: lane i owns row i (right-looking, column j
-// scaled by sqrt of the pivot, then the rank-1 update of the rows below).
-// Inverse X = L^-1: lane c owns column c (forward substitution on e_c).
-// L goes back into A's lower triangle, X (row-major, zero above the diagonal)
-// into dinv.  A pivot <= 0 or NaN sets info (first failing pivot, 1-based).
+__device__ __forceinline__ double lane_value(double v, int l) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+
+// Factor the diagonal block [k0, k0+nb)^2 and invert its factor with one
+// wave.  Factor: lane i keeps row i in registers (rows and columns past nb
+// hold the identity); step j takes the pivot from lane j, scales column j and
+// applies the rank-1 update, reading L(k, j) from lane k (the textbook
+// right-looking loop, the same operations in the same order).  Column j of L
+// also goes to LDS.  Inverse X = L^-1: lane c forms column c by column-wise
+// forward substitution on e_c, reading column k of L from LDS as a broadcast.
+// L goes back into A's lower triangle, X (row-major, zero above the diagonal
+// and past nb) into dinv.  A pivot <= 0 or NaN sets info (first failing
+// pivot, 1-based) and leaves the block as it is.
 __global__ __launch_bounds__(64) void k_potrf_inv(int n, int k0, double* A, double* dinv, int* info) {
-  __shared__ double B[CH_NB][CH_NB + 1];
-  __shared__ double X[CH_NB][CH_NB + 1];
+  __shared__ double Lc[CH_NB][CH_NB + 1];  // Lc[j][i] = L(i, j)
   const int nb = min(CH_NB, n - k0), i = threadIdx.x;
-  for (int e = i; e < CH_NB * CH_NB; e += 64) {
-    const int r = e / CH_NB, c = e - r * CH_NB;
-    B[r][c] = (r < nb && c < nb) ? A[(size_t)(k0 + r) * n + k0 + c] : 0.0;
-    X[r][c] = 0.0;
+  double r[CH_NB];
+  if (i < nb) {
+    const double* Ar = A + (size_t)(k0 + i) * n + k0;
+#pragma unroll
+    for (int c = 0; c < CH_NB; c++) r[c] = c < nb ? Ar[c] : 0.0;
+  } else {
+#pragma unroll
+    for (int c = 0; c < CH_NB; c++) r[c] = c == i ? 1.0 : 0.0;
   }
-  __syncthreads();
-  for (int j = 0; j < nb; j++) {
-    const double p = B[j][j];
+#pragma unroll
+  for (int j = 0; j < CH_NB; j++) {
+    const double p = lane_value(r[j], j);
     if (!(p > 0.0)) {
       if (i == 0 && *info == 0) *info = k0 + j + 1;
       return;
     }
     const double d = sqrt(p);
-    __syncthreads();
-    double lij = 0.0;
-    if (i == j) B[j][j] = d;
-    if (i > j && i < nb) {
-      lij = B[i][j] / d;
-      B[i][j] = lij;
-    }
-    __syncthreads();
-    if (i > j && i < nb)
-      for (int k = j + 1; k <= i; k++) B[i][k] = fma(-lij, B[k][j], B[i][k]);
-    __syncthreads();
-  }
-  // X = L^-1, column c by lane c
-  if (i < nb) {
-    X[i][i] = 1.0 / B[i][i];
-    for (int r = i + 1; r < nb; r++) {
-      double t = 0.0;
-      for (int k = i; k < r; k++) t = fma(B[r][k], X[k][i], t);
-      X[r][i] = -t / B[r][r];
-    }
+    // lane j carries the pivot; its row's entries right of the diagonal
+    // are never read
+    const double lij = i == j ? d : (i > j ? r[j] / d : 0.0);
+    Lc[j][i] = lij;
+#pragma unroll
+    for (int k = j + 1; k < CH_NB; k++) r[k] = fma(-lij, lane_value(lij, k), r[k]);
   }
   __syncthreads();
+  double x[CH_NB];
+#pragma unroll
+  for (int k = 0; k < CH_NB; k++) x[k] = k == i ? 1.0 : 0.0;
+#pragma unroll
+  for (int k = 0; k < CH_NB; k++) {
+    x[k] = x[k] / Lc[k][k];
+#pragma unroll
+    for (int rr = k + 1; rr < CH_NB; rr++) x[rr] = fma(-Lc[k][rr], x[k], x[rr]);
+  }
   double* D = dinv + (size_t)(k0 / CH_NB) * CH_NB * CH_NB;
+#pragma unroll
+  for (int rr = 0; rr < CH_NB; rr++) D[rr * CH_NB + i] = (i <= rr && rr < nb) ? x[rr] : 0.0;
   for (int e = i; e < CH_NB * CH_NB; e += 64) {
-    const int r = e / CH_NB, c = e - r * CH_NB;
-    D[e] = c <= r ? X[r][c] : 0.0;
-    if (r < nb && c <= r) A[(size_t)(k0 + r) * n + k0 + c] = B[r][c];
+    const int rr = e / CH_NB, c = e - rr * CH_NB;
+    if (rr < nb && c <= rr) A[(size_t)(k0 + rr) * n + k0 + c] = Lc[c][rr];
   }
 }
 
