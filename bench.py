@@ -113,6 +113,12 @@ def synthetic_orbitals(L, nsl):
     return orb
 
 
+def refine_sweeps(name):
+    """Refinement sweeps of the SR solve: one for FP64 S; two for a
+    mixed-precision S, refined against the FP64 operator of the O rows."""
+    return 1 if name == "fp64" else 2
+
+
 def sr_shape_bench(api, lt, L, alphar, J2, ns, nsl=None, seed=5):
     """The SR step (S build + system solve) on a synthetic measurement of ns
     configurations in the three S precisions: times, FLOP rate of the S build
@@ -132,12 +138,17 @@ def sr_shape_bench(api, lt, L, alphar, J2, ns, nsl=None, seed=5):
         eng.set_s_precision(prec)
         best = None
         for _ in range(2):  # the first call also captures the solve's graph
-            g, rep = eng.sr_step(p.gamma(), s_factor0=1e-4, delta_tau=0.05, refine_iters=1)
+            g, rep = eng.sr_step(p.gamma(), s_factor0=1e-4, delta_tau=0.05, refine_iters=refine_sweeps(name))
             t = eng.timings()
             if best is None or t["sr_total"] < best[1]["sr_total"]:
                 best = (g - p.gamma(), t, rep)
+        # the same step refined against the assembled mixed-precision matrix
+        ga = None
+        if name != "fp64":
+            ga, _ = eng.sr_step(p.gamma(), s_factor0=1e-4, delta_tau=0.05, refine_iters=1, assembled_refine=True)
+            ga = ga - p.gamma()
         S = eng.smat() if nv <= 6000 else None
-        res[name] = dict(step=best[0], t=best[1], rep=best[2], S=S)
+        res[name] = dict(step=best[0], t=best[1], rep=best[2], S=S, step_assembled=ga)
         log(f"SR step nv={nv} ns={ns} {name}: S build {best[1]['s_build'] * 1e3:.1f} ms, solve "
             f"{best[1]['solve'] * 1e3:.1f} ms, residual {best[2]['residual']:.1e}")
     base = res["fp64"]
@@ -148,7 +159,11 @@ def sr_shape_bench(api, lt, L, alphar, J2, ns, nsl=None, seed=5):
                  tflops=ach, peak_tflops=peaks[name], mfma_frac=ach / peaks[name],
                  residual=r["rep"]["residual"],
                  step_rel_err=float(np.linalg.norm(r["step"] - base["step"]) / np.linalg.norm(base["step"])),
-                 speedup_vs_fp64=base["t"]["sr_total"] / r["t"]["sr_total"])
+                 speedup_vs_fp64=base["t"]["sr_total"] / r["t"]["sr_total"],
+                 refine_sweeps=refine_sweeps(name))
+        if r["step_assembled"] is not None:
+            d["step_rel_err_assembled_refine"] = float(np.linalg.norm(r["step_assembled"] - base["step"]) /
+                                                       np.linalg.norm(base["step"]))
         if r["S"] is not None:
             d["s_rel_err"] = float(np.linalg.norm(r["S"] - base["S"]) / np.linalg.norm(base["S"]))
         out[name] = d
@@ -246,7 +261,7 @@ def batch_run(args):
             a = time.perf_counter()
             r = e.calc_phys_quantity_opt(shift=shifts[k])
             b = time.perf_counter()
-            e.sr_step(e.gamma(), s_factor0=1e-4, delta_tau=0.05, refine_iters=1)
+            e.sr_step(e.gamma(), s_factor0=1e-4, delta_tau=0.05, refine_iters=refine_sweeps("bf16x3"))
             tm += b - a
             ts += time.perf_counter() - b
             E.append(r["Etot"] / 4 / e.N)
@@ -438,7 +453,7 @@ def main():
         r = eng.calc_phys_quantity_opt(shift=shift)
         t2 = time.perf_counter()
         gam_prev = gam.copy()
-        gam, rep = eng.sr_step(gam, s_factor0=1e-4, delta_tau=0.05, refine_iters=1)
+        gam, rep = eng.sr_step(gam, s_factor0=1e-4, delta_tau=0.05, refine_iters=refine_sweeps(args.s_precision))
         t3 = time.perf_counter()
         tt2 = eng.timings()
         # main.f90:441-451: reweighting shift from the running mean of wf_av
@@ -480,9 +495,14 @@ def main():
             eng.set_s_precision(prec)
             S = eng.smat()
             t_s = eng.timings()["s_build"]
-            g_new, _ = eng.sr_step(gam_prev, s_factor0=1e-4, delta_tau=0.05, refine_iters=1)
+            g_new, _ = eng.sr_step(gam_prev, s_factor0=1e-4, delta_tau=0.05, refine_iters=refine_sweeps(name))
             t_sr = eng.timings()["sr_total"]
-            res[name] = dict(S=S, t_s=t_s, t_sr=t_sr, step=g_new - gam_prev)
+            g_asm = None
+            if name != "fp64":
+                g_asm, _ = eng.sr_step(gam_prev, s_factor0=1e-4, delta_tau=0.05, refine_iters=1,
+                                       assembled_refine=True)
+            res[name] = dict(S=S, t_s=t_s, t_sr=t_sr, step=g_new - gam_prev,
+                             step_asm=None if g_asm is None else g_asm - gam_prev)
         base = res["fp64"]
         fp64 = {}
         for name, r in res.items():
@@ -491,7 +511,11 @@ def main():
                               mfma_frac=ach / peaks[name],
                               s_rel_err=float(np.linalg.norm(r["S"] - base["S"]) / np.linalg.norm(base["S"])),
                               step_rel_err=float(np.linalg.norm(r["step"] - base["step"]) /
-                                                 np.linalg.norm(base["step"])))
+                                                 np.linalg.norm(base["step"])),
+                              refine_sweeps=refine_sweeps(name))
+            if r["step_asm"] is not None:
+                fp64[name]["step_rel_err_assembled_refine"] = float(
+                    np.linalg.norm(r["step_asm"] - base["step"]) / np.linalg.norm(base["step"]))
             log(f"S build {name}: {r['t_s'] * 1e3:.3f} ms ({ach:.1f} TFLOP/s, {100 * ach / peaks[name]:.1f}% of "
                 f"{peaks[name]:.0f}), SR step {r['t_sr'] * 1e3:.2f} ms, S err {fp64[name]['s_rel_err']:.2e}, "
                 f"step err {fp64[name]['step_rel_err']:.2e}")

@@ -98,6 +98,12 @@ typedef struct {
 /* rbmpp_sr_config.flags: run the SR system's launches eagerly instead of
    replaying the hipGraph they are captured into on the first call */
 #define RBMPP_SR_NO_GRAPH 1
+/* With S assembled from FP32 or split-BF16 products, the refinement sweeps
+   and lambda use by default the FP64 operator of the stored O rows,
+   D (C^T C / norm) D + shift with C = O - kappa <O>^T (two passes over O per
+   sweep), so that the step converges to the FP64 system's solution; this
+   flag refines against the assembled mixed-precision matrix instead. */
+#define RBMPP_SR_ASSEMBLED_REFINE 2
 
 typedef struct {
   double lambda;             /* sqrt(x^T S x) * 5/L (SR_pdposv.f90:309) */
@@ -105,7 +111,7 @@ typedef struct {
   double scaling;            /* PP block scale (SR_pdposv.f90:40) */
   double sdiag_av;           /* mean positive diagonal (SR_pdposv.f90:133) */
   double s_factor;           /* max(1e-4, s_factor0) */
-  double residual;           /* ||S x - g|| / ||g|| after refinement */
+  double residual;           /* ||S x - g|| / ||g|| after refinement (S: the operator refined against) */
   double mean_abs_f;         /* mean |f| before renormalisation */
   int32_t k_nonzero;         /* SR_pdposv.f90:337-341 */
   int32_t reserved;

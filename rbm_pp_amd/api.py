@@ -25,6 +25,7 @@ LIB = os.path.join(HERE, "librbmpp.so")
 S_FP64, S_FP32, S_BF16X3 = 0, 1, 2
 PROJ_NORMAL, PROJ_TRANSVERSE, PROJ_NORMAL_EXACT = 0, 1, 2
 SR_NO_GRAPH = 1
+SR_ASSEMBLED_REFINE = 2
 
 
 TUNE_S_BUILD, TUNE_MEASURE = 1, 2
@@ -222,11 +223,14 @@ class Engine:
 
     def sr_step(self, gamma=None, s_factor0: float = 1e-3, delta_tau: float = 0.05,
                 max_change: float = 0.04, slater_abs_av: float = 0.25, refine_iters: int = 1,
-                graph: bool = True):
+                graph: bool = True, assembled_refine: bool = False):
         """graph=False runs the SR system's launches eagerly instead of
-        replaying the captured graph (RBMPP_SR_NO_GRAPH)."""
+        replaying the captured graph (RBMPP_SR_NO_GRAPH); assembled_refine
+        refines a mixed-precision S against the assembled matrix instead of
+        the FP64 operator of the O rows (RBMPP_SR_ASSEMBLED_REFINE)."""
         g = np.ascontiguousarray(self.gamma() if gamma is None else gamma, np.float64).copy()
-        c = SrConfig(s_factor0, delta_tau, max_change, slater_abs_av, refine_iters, 0 if graph else SR_NO_GRAPH)
+        flags = (0 if graph else SR_NO_GRAPH) | (SR_ASSEMBLED_REFINE if assembled_refine else 0)
+        c = SrConfig(s_factor0, delta_tau, max_change, slater_abs_av, refine_iters, flags)
         rep = SrReport()
         self._check(lib().rbmpp_sr_step(self.h, C.byref(c), _d(g), C.byref(rep)))
         return g, {k: getattr(rep, k) for k, _ in SrReport._fields_}

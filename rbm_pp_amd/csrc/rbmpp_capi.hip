@@ -77,6 +77,12 @@ struct rbmpp_handle {
   hipGraph_t sr_graph = nullptr;   // captured SR pipeline
   hipGraphExec_t sr_exec = nullptr;
   int sr_graph_iters = -1;         // refine_iters it was captured with
+  // and the O-row operator it refines against (null: the assembled matrix)
+  const double *sr_graph_orows = nullptr, *sr_graph_kappa = nullptr;
+  int sr_graph_rows = 0;
+  double* sr_graph_t = nullptr;
+  double* d_op = nullptr;  // scratch of the O-row operator
+  size_t op_cap = 0;
   unsigned char* d_packed = nullptr;  // centred O rows packed for the S build
   size_t packed_cap = 0;
   int n_cu = 256;                     // compute units of the device
@@ -753,21 +759,36 @@ __global__ void k_synth_measurement(int64_t n, int nv, int cplx_rows, uint64_t s
   }
 }
 
+// scratch of the FP64 O-row passes (diagonal of a mixed S, SR refinement)
+static int ensure_op(rbmpp_handle* h, int rows) {
+  const size_t need = orow_op_scratch(rows, h->nv);
+  if (need <= h->op_cap) return 0;
+  double* op = nullptr;
+  int rc = dalloc(h, &op, need);
+  if (rc) return rc;
+  dfree(h, h->d_op);
+  h->d_op = op;
+  h->op_cap = need;
+  return 0;
+}
+
 // centred S = <O O^T>/norm - <O><O>^T in the configured precision: the O
 // rows are centred once in FP64 and packed in the precision of the products
-// (rbmpp_syrk.hip), then multiplied on the matrix cores
+// (rbmpp_syrk.hip), then multiplied on the matrix cores; a mixed-precision
+// S gets its diagonal from an FP64 pass over the O rows
 static int build_s(rbmpp_handle* h) {
   const int nv = h->nv, n = (int)h->n_meas;
   int rc;
   if (!h->d_S && ((rc = dalloc(h, &h->d_S, (size_t)nv * nv)) || (rc = dalloc(h, &h->d_A, (size_t)nv * nv)) ||
-                  (rc = dalloc(h, &h->d_L, (size_t)nv * nv)) || (rc = dalloc(h, &h->d_vec, (size_t)4 * nv)) ||
-                  (rc = dalloc(h, &h->d_info, 1)) || (rc = dalloc(h, &h->d_sc, 8)) || (rc = dalloc(h, &h->d_fl, 8)) ||
+                  (rc = dalloc(h, &h->d_L, (size_t)nv * nv)) || (rc = dalloc(h, &h->d_vec, (size_t)5 * nv)) ||
+                  (rc = dalloc(h, &h->d_info, 1)) || (rc = dalloc(h, &h->d_sc, 16)) || (rc = dalloc(h, &h->d_fl, 8)) ||
                   (rc = dalloc(h, &h->d_dinv, chol_dinv_size(nv)))))
     return rc;
   const int rows = h->real_only ? n : 2 * n;
   const int prec = h->cfg.s_precision;
   const SyrkPlan pl = syrk_plan(prec, rows, nv, h->n_cu, h->tune_tile[prec], h->tune_nsplit[prec]);
   if ((rc = ensure_part(h, pl.part_doubles))) return rc;
+  if (prec != RBMPP_S_FP64 && (rc = ensure_op(h, rows))) return rc;
   if (pl.packed_bytes > h->packed_cap) {
     unsigned char* pk = nullptr;
     if ((rc = dalloc(h, &pk, pl.packed_bytes))) return rc;
@@ -775,12 +796,15 @@ static int build_s(rbmpp_handle* h) {
     h->d_packed = pk;
     h->packed_cap = pl.packed_bytes;
   }
-  // obar to device
-  HIPCHK(h, hipMemcpyAsync(h->d_vec, h->m_o.data(), sizeof(double) * nv, hipMemcpyHostToDevice, h->st));
+  // obar to device (kept after the S build for the SR refinement)
+  double* d_obar = h->d_vec + 4 * nv;
+  HIPCHK(h, hipMemcpyAsync(d_obar, h->m_o.data(), sizeof(double) * nv, hipMemcpyHostToDevice, h->st));
   launch_row_kappa(n, h->real_only ? 0 : 1, h->d_mpsi, h->d_rw, h->d_kappa, h->st);
   tic(h);
-  launch_center_pack(pl, rows, nv, h->d_orows, h->d_kappa, h->d_vec, h->d_packed, h->st);
+  launch_center_pack(pl, rows, nv, h->d_orows, h->d_kappa, d_obar, h->d_packed, h->st);
   launch_syrk_packed(pl, nv, h->d_packed, h->d_part, 1.0 / h->m_norm, h->d_S, h->st);
+  if (prec != RBMPP_S_FP64)
+    launch_s_diag_fp64(rows, nv, h->d_orows, h->d_kappa, d_obar, 1.0 / h->m_norm, h->d_op, h->d_S, h->st);
   HIPCHK(h, hipGetLastError());
   h->t[3] = toc(h);
   h->s_valid = 1;
@@ -822,8 +846,17 @@ int rbmpp_sr_step(rbmpp_handle* h, const rbmpp_sr_config* c, double* gamma, rbmp
   double* d_g = h->d_vec + 3 * nv;
   p.g = d_g;
   p.sc = h->d_sc; p.fl = h->d_fl; p.dinv = h->d_dinv;
+  // a mixed-precision S is refined against the FP64 operator of the O rows
+  if (h->cfg.s_precision != RBMPP_S_FP64 && !(c->flags & RBMPP_SR_ASSEMBLED_REFINE)) {
+    const int rows = h->real_only ? (int)h->n_meas : 2 * (int)h->n_meas;
+    if ((rc = ensure_op(h, rows))) return rc;
+    p.orows = h->d_orows; p.kappa = h->d_kappa; p.obar = h->d_vec + 4 * nv;
+    p.rows = rows; p.t = h->d_op; p.part = h->d_op + rows;
+  }
+  const double inv_norm = 1.0 / h->m_norm;
   HIPCHK(h, hipMemcpyAsync(d_g, g.data(), sizeof(double) * nv, hipMemcpyHostToDevice, h->st));
   HIPCHK(h, hipMemcpyAsync(h->d_sc + 3, &sf, sizeof(double), hipMemcpyHostToDevice, h->st));
+  HIPCHK(h, hipMemcpyAsync(h->d_sc + 7, &inv_norm, sizeof(double), hipMemcpyHostToDevice, h->st));
   // the launch sequence (scalars, system, Cholesky, solves, refinement,
   // lambda) has no host synchronisation; it is captured once into a graph
   // and replayed (RBMPP_SR_NO_GRAPH runs it eagerly)
@@ -831,7 +864,8 @@ int rbmpp_sr_step(rbmpp_handle* h, const rbmpp_sr_config* c, double* gamma, rbmp
     launch_sr_pipeline(p, h->st);
     HIPCHK(h, hipGetLastError());
   } else {
-    if (!h->sr_exec || h->sr_graph_iters != c->refine_iters) {
+    if (!h->sr_exec || h->sr_graph_iters != c->refine_iters || h->sr_graph_orows != p.orows ||
+        h->sr_graph_kappa != p.kappa || h->sr_graph_rows != p.rows || h->sr_graph_t != p.t) {
       if (h->sr_exec) { hipGraphExecDestroy(h->sr_exec); h->sr_exec = nullptr; }
       if (h->sr_graph) { hipGraphDestroy(h->sr_graph); h->sr_graph = nullptr; }
       HIPCHK(h, hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
@@ -841,6 +875,10 @@ int rbmpp_sr_step(rbmpp_handle* h, const rbmpp_sr_config* c, double* gamma, rbmp
       HIPCHK(h, le);
       HIPCHK(h, hipGraphInstantiate(&h->sr_exec, h->sr_graph, nullptr, nullptr, 0));
       h->sr_graph_iters = c->refine_iters;
+      h->sr_graph_orows = p.orows;
+      h->sr_graph_kappa = p.kappa;
+      h->sr_graph_rows = p.rows;
+      h->sr_graph_t = p.t;
     }
     HIPCHK(h, hipGraphLaunch(h->sr_exec, h->st));
   }

@@ -53,7 +53,23 @@ struct SrPipeline {
   double *A, *L;             // nv x nv: D S D + shift I and its factor
   double *x, *r, *ax;        // [nv]: solution, residual / correction, A x (also solve scratch)
   double* dinv;              // inverses of the factor's diagonal blocks (chol_dinv_size)
-  double* sc;                // [8] scalars (see k_sr_prepare)
+  double* sc;                // [16] scalars (see k_sr_prepare; sc[7] = 1/norm, sc[8] scratch)
   int* fl;                   // [8] flags and Cholesky info
+  // orows != nullptr: the refinement residuals and lambda use the FP64
+  // operator of the O rows, D (C^T C / norm) D + shift I with
+  // C = O - kappa obar^T, instead of the assembled A (for S assembled from
+  // FP32 or split-BF16 products)
+  const double* orows = nullptr;  // [rows][nv]
+  const double* kappa = nullptr;  // [rows]
+  const double* obar = nullptr;   // [nv]
+  int rows = 0;
+  double* t = nullptr;            // orow_op_scratch(rows, nv) doubles
+  double* part = nullptr;         // (inside t's allocation)
 };
 void launch_sr_pipeline(const SrPipeline& p, hipStream_t st);
+// doubles of scratch of the O-row operator
+size_t orow_op_scratch(int rows, int nv);
+// S_jj = inv_norm sum_k (O_kj - kappa_k obar_j)^2 in FP64 over the diagonal
+// of S (part: orow_op_scratch(rows, nv) doubles of scratch)
+void launch_s_diag_fp64(int rows, int nv, const double* orows, const double* kappa, const double* obar,
+                        double inv_norm, double* part, double* S, hipStream_t st);

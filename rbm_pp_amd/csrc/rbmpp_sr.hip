@@ -481,6 +481,149 @@ void launch_chol_solve(int n, const double* L, const double* dinv, double* x, do
   }
 }
 
+// ------------------------------------------- the FP64 operator of the O rows
+// y = (D S D + shift I) x with S = inv_norm C^T C and C = O - kappa obar^T,
+// the centred rows taken from the FP64 O rows themselves (two passes over O),
+// so that a refinement against it converges to the FP64 system's solution
+// whatever the precision of the assembled S.  sc[7] = inv_norm, sc[8] holds
+// obar . D x.
+#define OP_CHUNKS 64
+
+__global__ __launch_bounds__(1024) void k_op_obar(int nv, int nvr, double* __restrict__ sc,
+                                                  const double* __restrict__ obar, const double* __restrict__ x) {
+  __shared__ double rs[1024];
+  const int tid = threadIdx.x;
+  const double scal = sc[0];
+  double t = 0.0;
+  for (int j = tid; j < nv; j += 1024) t = fma(obar[j], j >= nvr ? x[j] * scal : x[j], t);
+  rs[tid] = t;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if (tid < o) rs[tid] += rs[tid + o];
+    __syncthreads();
+  }
+  if (tid == 0) sc[8] = rs[0];
+}
+
+// t_k = C_k . D x, one wave per row
+__global__ void k_op_rows(int rows, int nv, int nvr, const double* __restrict__ sc, const double* __restrict__ O,
+                          const double* __restrict__ kappa, const double* __restrict__ x, double* __restrict__ t) {
+  const int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= rows) return;
+  const double scal = sc[0];
+  const double* Or = O + (size_t)row * nv;
+  double a = 0.0;
+  for (int j = lane; j < nv; j += 64) a = fma(Or[j], j >= nvr ? x[j] * scal : x[j], a);
+  a = wave_sum(a);
+  if (lane == 0) t[row] = a - kappa[row] * sc[8];
+}
+
+// column partial sums over a chunk of rows: part[c][j] = sum_k O[k][j] t_k
+// (j < nv) and part[c][nv] = sum_k kappa_k t_k, each in a fixed order
+__global__ __launch_bounds__(256) void k_op_cols(int rows, int nv, const double* __restrict__ O,
+                                                 const double* __restrict__ kappa, const double* __restrict__ t,
+                                                 double* __restrict__ part) {
+  __shared__ double rs[256];
+  const int c = blockIdx.y, j = blockIdx.x * 256 + threadIdx.x;
+  const int per = (rows + OP_CHUNKS - 1) / OP_CHUNKS;
+  const int r0 = c * per, r1 = min(rows, r0 + per);
+  double* pc = part + (size_t)c * (nv + 1);
+  if (j < nv) {
+    double a = 0.0;
+    int k = r0;
+    for (; k + 4 <= r1; k += 4) {
+      const double o0 = O[(size_t)k * nv + j], o1 = O[(size_t)(k + 1) * nv + j];
+      const double o2 = O[(size_t)(k + 2) * nv + j], o3 = O[(size_t)(k + 3) * nv + j];
+      a = fma(o0, t[k], a);
+      a = fma(o1, t[k + 1], a);
+      a = fma(o2, t[k + 2], a);
+      a = fma(o3, t[k + 3], a);
+    }
+    for (; k < r1; k++) a = fma(O[(size_t)k * nv + j], t[k], a);
+    pc[j] = a;
+  }
+  if (blockIdx.x == 0) {
+    double a = 0.0;
+    for (int k = r0 + (int)threadIdx.x; k < r1; k += 256) a = fma(kappa[k], t[k], a);
+    rs[threadIdx.x] = a;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) rs[threadIdx.x] += rs[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) pc[nv] = rs[0];
+  }
+}
+
+// y = D inv_norm (sum_c part_c - obar sum_c part_c[nv]) + shift x, and with
+// r: r = D g - y
+__global__ void k_op_finish(int nv, int nvr, const double* __restrict__ sc, const double* __restrict__ obar,
+                            const double* __restrict__ part, const double* __restrict__ x,
+                            const double* __restrict__ g, double* __restrict__ y, double* __restrict__ r) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nv) return;
+  double w = 0.0, kt = 0.0;
+  for (int c = 0; c < OP_CHUNKS; c++) {
+    w += part[(size_t)c * (nv + 1) + j];
+    kt += part[(size_t)c * (nv + 1) + nv];
+  }
+  const double scal = sc[0], shift = sc[1], inv_norm = sc[7];
+  const double d = j >= nvr ? scal : 1.0;
+  const double yj = d * (inv_norm * fma(-obar[j], kt, w)) + shift * x[j];
+  y[j] = yj;
+  if (r) r[j] = (j >= nvr ? g[j] * scal : g[j]) - yj;
+}
+
+static void launch_orow_op(const SrPipeline& p, double* y, double* r, hipStream_t st) {
+  const int n = p.nv;
+  hipLaunchKernelGGL(k_op_obar, dim3(1), dim3(1024), 0, st, n, p.nvr, p.sc, p.obar, (const double*)p.x);
+  hipLaunchKernelGGL(k_op_rows, dim3((p.rows + 3) / 4), dim3(256), 0, st, p.rows, n, p.nvr, (const double*)p.sc,
+                     p.orows, p.kappa, (const double*)p.x, p.t);
+  hipLaunchKernelGGL(k_op_cols, dim3((n + 255) / 256, OP_CHUNKS), dim3(256), 0, st, p.rows, n, p.orows, p.kappa,
+                     (const double*)p.t, p.part);
+  hipLaunchKernelGGL(k_op_finish, dim3((n + 255) / 256), dim3(256), 0, st, n, p.nvr, (const double*)p.sc, p.obar,
+                     (const double*)p.part, (const double*)p.x, p.g, y, r);
+}
+
+size_t orow_op_scratch(int rows, int nv) { return (size_t)rows + (size_t)OP_CHUNKS * (nv + 1); }
+
+// The diagonal of S in FP64 from the O rows, S_jj = inv_norm sum_k C_kj^2
+// (column chunks as k_op_cols, then a fixed-order sum), written over the
+// diagonal of an S assembled from FP32 or split-BF16 products: the PP
+// scaling, the shift and the diagonal checks then see FP64 values, and the
+// operator of the O rows and the assembled matrix share D and the shift.
+__global__ __launch_bounds__(256) void k_diag_cols(int rows, int nv, const double* __restrict__ O,
+                                                   const double* __restrict__ kappa, const double* __restrict__ obar,
+                                                   double* __restrict__ part) {
+  const int c = blockIdx.y, j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= nv) return;
+  const int per = (rows + OP_CHUNKS - 1) / OP_CHUNKS;
+  const int r0 = c * per, r1 = min(rows, r0 + per);
+  const double ob = obar[j];
+  double a = 0.0;
+  for (int k = r0; k < r1; k++) {
+    const double v = fma(-kappa[k], ob, O[(size_t)k * nv + j]);
+    a = fma(v, v, a);
+  }
+  part[(size_t)c * (nv + 1) + j] = a;
+}
+
+__global__ void k_diag_finish(int nv, double inv_norm, const double* __restrict__ part, double* __restrict__ S) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nv) return;
+  double a = 0.0;
+  for (int c = 0; c < OP_CHUNKS; c++) a += part[(size_t)c * (nv + 1) + j];
+  S[(size_t)j * nv + j] = a * inv_norm;
+}
+
+void launch_s_diag_fp64(int rows, int nv, const double* orows, const double* kappa, const double* obar,
+                        double inv_norm, double* part, double* S, hipStream_t st) {
+  hipLaunchKernelGGL(k_diag_cols, dim3((nv + 255) / 256, OP_CHUNKS), dim3(256), 0, st, rows, nv, orows, kappa, obar,
+                     part);
+  hipLaunchKernelGGL(k_diag_finish, dim3((nv + 255) / 256), dim3(256), 0, st, nv, inv_norm, (const double*)part, S);
+}
+
 void launch_sr_pipeline(const SrPipeline& p, hipStream_t st) {
   const int n = p.nv;
   hipLaunchKernelGGL(k_sr_prepare, dim3(1), dim3(1024), 0, st, n, p.nvr, p.S, p.fii, p.sc, p.fl);
@@ -488,6 +631,17 @@ void launch_sr_pipeline(const SrPipeline& p, hipStream_t st) {
                      p.x);
   launch_cholesky(n, p.L, p.dinv, p.fl + 4, st);
   launch_chol_solve(n, p.L, p.dinv, p.x, p.ax, st);
+  if (p.orows) {
+    // refinement and lambda against the FP64 operator of the O rows
+    for (int it = 0; it < p.refine_iters; it++) {
+      launch_orow_op(p, p.ax, p.r, st);
+      launch_chol_solve(n, p.L, p.dinv, p.r, p.ax, st);
+      hipLaunchKernelGGL(k_axpy1, dim3((n + 255) / 256), dim3(256), 0, st, n, p.r, p.x);
+    }
+    launch_orow_op(p, p.ax, nullptr, st);
+    hipLaunchKernelGGL(k_sr_final, dim3(1), dim3(1024), 0, st, n, p.nvr, p.sc, p.x, p.ax, p.g);
+    return;
+  }
   for (int it = 0; it < p.refine_iters; it++) {
     hipLaunchKernelGGL(k_residual, dim3((n + 3) / 4), dim3(256), 0, st, n, p.nvr, (const double*)p.sc, p.A, p.x,
                        p.g, p.r);

@@ -27,6 +27,7 @@ module rbmpp_mod
   integer(c_int), parameter, public :: RBMPP_PROJ_NORMAL = 0, RBMPP_PROJ_TRANSVERSE = 1, &
                                         RBMPP_PROJ_NORMAL_EXACT = 2
   integer(c_int), parameter, public :: RBMPP_SR_NO_GRAPH = 1
+  integer(c_int), parameter, public :: RBMPP_SR_ASSEMBLED_REFINE = 2
 
   ! struct rbmpp_config (field order and kinds as in rbmpp.h)
   type, bind(C), public :: rbmpp_config
