@@ -210,6 +210,8 @@ def energy_run(args):
             it = line.split(":")[-1].strip()
         elif "Total Energy" in line or "total energy" in line:
             log(f"iteration {it}: E/N = {float(line.split(':')[-1]) / 4 / N:.6f}")
+        elif "failed" in line or "error" in line.lower():
+            log(f"driver: {line.rstrip()}")
     rc = proc.wait()
     if rc != 0:
         log(f"driver exit {rc}")

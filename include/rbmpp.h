@@ -91,18 +91,23 @@ typedef struct {
   double delta_tau;          /* step (main.f90:79) */
   double max_change;         /* 0.04 (main.f90:50) */
   double slater_abs_av;      /* 0.25 (module.f90:66) */
-  int32_t refine_iters;      /* FP64 iterative-refinement sweeps after the Cholesky solve (0..8) */
+  int32_t refine_iters;      /* FP64 iterative-refinement sweeps after the Cholesky solve (0..8); with a
+                                mixed-precision S the conjugate-gradient iterations captured with it */
   int32_t flags;             /* RBMPP_SR_* bits; 0 by default */
 } rbmpp_sr_config;
 
 /* rbmpp_sr_config.flags: run the SR system's launches eagerly instead of
    replaying the hipGraph they are captured into on the first call */
 #define RBMPP_SR_NO_GRAPH 1
-/* With S assembled from FP32 or split-BF16 products, the refinement sweeps
-   and lambda use by default the FP64 operator of the stored O rows,
+/* With S assembled from FP32 or split-BF16 products, the system is solved by
+   default with conjugate gradients on the FP64 operator of the stored O rows,
    D (C^T C / norm) D + shift with C = O - kappa <O>^T (two passes over O per
-   sweep), so that the step converges to the FP64 system's solution; this
-   flag refines against the assembled mixed-precision matrix instead. */
+   iteration), preconditioned by the Cholesky factor of the assembled matrix:
+   refine_iters iterations inside the captured launch sequence, then more in
+   fours (up to 40) until the relative residual is below 1e-11, so that the
+   step is the FP64 system's solution.  The diagonal of a mixed S is formed in
+   FP64 from the O rows.  This flag refines against the assembled matrix
+   instead (refine_iters sweeps). */
 #define RBMPP_SR_ASSEMBLED_REFINE 2
 
 typedef struct {
@@ -114,7 +119,10 @@ typedef struct {
   double residual;           /* ||S x - g|| / ||g|| after refinement (S: the operator refined against) */
   double mean_abs_f;         /* mean |f| before renormalisation */
   int32_t k_nonzero;         /* SR_pdposv.f90:337-341 */
-  int32_t reserved;
+  int32_t cg_iters;          /* conjugate-gradient iterations on the O-row operator (0: FP64 S or
+                                RBMPP_SR_ASSEMBLED_REFINE) */
+  double precond_boost;      /* diagonal added to the assembled mixed-precision matrix to factor it
+                                as the preconditioner (0 when it was positive definite) */
 } rbmpp_sr_report;
 
 /* Creates the device state.  Tables in the reference's layouts:

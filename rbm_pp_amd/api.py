@@ -61,7 +61,8 @@ class SrConfig(C.Structure):
 class SrReport(C.Structure):
     _fields_ = [("lambda_", C.c_double), ("delw_max", C.c_double), ("scaling", C.c_double),
                 ("sdiag_av", C.c_double), ("s_factor", C.c_double), ("residual", C.c_double),
-                ("mean_abs_f", C.c_double), ("k_nonzero", C.c_int32), ("reserved", C.c_int32)]
+                ("mean_abs_f", C.c_double), ("k_nonzero", C.c_int32), ("cg_iters", C.c_int32),
+                ("precond_boost", C.c_double)]
 
 
 _lib = None

@@ -82,6 +82,8 @@ struct rbmpp_handle {
   int sr_graph_rows = 0;
   double* sr_graph_t = nullptr;
   double* d_op = nullptr;  // scratch of the O-row operator
+  int sr_pcg_iters = 0;    // conjugate-gradient iterations of the last SR step (0: FP64 S)
+  double sr_boost = 0;     // extra diagonal its preconditioner needed
   size_t op_cap = 0;
   unsigned char* d_packed = nullptr;  // centred O rows packed for the S build
   size_t packed_cap = 0;
@@ -780,7 +782,7 @@ static int build_s(rbmpp_handle* h) {
   const int nv = h->nv, n = (int)h->n_meas;
   int rc;
   if (!h->d_S && ((rc = dalloc(h, &h->d_S, (size_t)nv * nv)) || (rc = dalloc(h, &h->d_A, (size_t)nv * nv)) ||
-                  (rc = dalloc(h, &h->d_L, (size_t)nv * nv)) || (rc = dalloc(h, &h->d_vec, (size_t)5 * nv)) ||
+                  (rc = dalloc(h, &h->d_L, (size_t)nv * nv)) || (rc = dalloc(h, &h->d_vec, (size_t)8 * nv)) ||
                   (rc = dalloc(h, &h->d_info, 1)) || (rc = dalloc(h, &h->d_sc, 16)) || (rc = dalloc(h, &h->d_fl, 8)) ||
                   (rc = dalloc(h, &h->d_dinv, chol_dinv_size(nv)))))
     return rc;
@@ -843,6 +845,7 @@ int rbmpp_sr_step(rbmpp_handle* h, const rbmpp_sr_config* c, double* gamma, rbmp
   p.nv = nv; p.nvr = nvr; p.refine_iters = c->refine_iters;
   p.S = h->d_S; p.fii = h->d_fii; p.A = h->d_A; p.L = h->d_L;
   p.x = h->d_vec; p.r = h->d_vec + nv; p.ax = h->d_vec + 2 * nv;
+  p.z = h->d_vec + 5 * nv; p.pd = h->d_vec + 6 * nv; p.tmp = h->d_vec + 7 * nv;
   double* d_g = h->d_vec + 3 * nv;
   p.g = d_g;
   p.sc = h->d_sc; p.fl = h->d_fl; p.dinv = h->d_dinv;
@@ -860,10 +863,12 @@ int rbmpp_sr_step(rbmpp_handle* h, const rbmpp_sr_config* c, double* gamma, rbmp
   // the launch sequence (scalars, system, Cholesky, solves, refinement,
   // lambda) has no host synchronisation; it is captured once into a graph
   // and replayed (RBMPP_SR_NO_GRAPH runs it eagerly)
-  if (c->flags & RBMPP_SR_NO_GRAPH) {
-    launch_sr_pipeline(p, h->st);
-    HIPCHK(h, hipGetLastError());
-  } else {
+  auto run_pipeline = [&]() -> int {
+    if (c->flags & RBMPP_SR_NO_GRAPH) {
+      launch_sr_pipeline(p, h->st);
+      HIPCHK(h, hipGetLastError());
+      return 0;
+    }
     if (!h->sr_exec || h->sr_graph_iters != c->refine_iters || h->sr_graph_orows != p.orows ||
         h->sr_graph_kappa != p.kappa || h->sr_graph_rows != p.rows || h->sr_graph_t != p.t) {
       if (h->sr_exec) { hipGraphExecDestroy(h->sr_exec); h->sr_exec = nullptr; }
@@ -881,13 +886,41 @@ int rbmpp_sr_step(rbmpp_handle* h, const rbmpp_sr_config* c, double* gamma, rbmp
       h->sr_graph_t = p.t;
     }
     HIPCHK(h, hipGraphLaunch(h->sr_exec, h->st));
-  }
-  std::vector<double> x(nv);
+    return 0;
+  };
   double sc[8];
   int fl[8];
+  auto read_scalars = [&]() -> int {
+    HIPCHK(h, hipMemcpyAsync(sc, h->d_sc, sizeof sc, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(h, hipMemcpyAsync(fl, h->d_fl, sizeof fl, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    return 0;
+  };
+  // With the operator of the O rows, an assembled mixed-precision matrix
+  // that is not positive definite only preconditions: its factor is retried
+  // with a larger diagonal (10, 100, 1000, 10^4 times the shift), and the
+  // conjugate gradients still solve the FP64 system.
+  double boost = 0.0;
+  for (int attempt = 0;; attempt++) {
+    HIPCHK(h, hipMemcpyAsync(h->d_sc + 12, &boost, sizeof(double), hipMemcpyHostToDevice, h->st));
+    if ((rc = run_pipeline())) return rc;
+    if ((rc = read_scalars())) return rc;
+    if (fl[4] == 0 || !p.orows || fl[3] != 0 || attempt == 4) break;
+    boost = (boost == 0.0 ? 10.0 : 10.0 * boost / sc[1]) * sc[1];
+  }
+  // more conjugate-gradient iterations (in fours, up to 40) until the
+  // residual against the operator of the O rows is below 1e-11
+  int pcg_iters = c->refine_iters;
+  if (p.orows && fl[4] == 0 && fl[3] == 0)
+    while (std::sqrt(sc[5] / std::max(sc[6], 1e-300)) > 1e-11 && pcg_iters < c->refine_iters + 40) {
+      launch_pcg_iters(p, pcg_iters, pcg_iters + 4, h->st);
+      launch_pcg_final(p, h->st);
+      HIPCHK(h, hipGetLastError());
+      pcg_iters += 4;
+      if ((rc = read_scalars())) return rc;
+    }
+  std::vector<double> x(nv);
   HIPCHK(h, hipMemcpyAsync(x.data(), p.x, sizeof(double) * nv, hipMemcpyDeviceToHost, h->st));
-  HIPCHK(h, hipMemcpyAsync(sc, h->d_sc, sizeof sc, hipMemcpyDeviceToHost, h->st));
-  HIPCHK(h, hipMemcpyAsync(fl, h->d_fl, sizeof fl, hipMemcpyDeviceToHost, h->st));
   HIPCHK(h, hipEventRecord(eb, h->st));
   HIPCHK(h, hipEventSynchronize(eb));
   float ms = 0;
@@ -895,6 +928,8 @@ int rbmpp_sr_step(rbmpp_handle* h, const rbmpp_sr_config* c, double* gamma, rbmp
   h->t[4] = ms * 1e-3;
   hipEventDestroy(ea);
   hipEventDestroy(eb);
+  h->sr_pcg_iters = p.orows ? pcg_iters : 0;
+  h->sr_boost = boost;
   // SR_pdposv.f90:32-33, 78-100 : the structural checks of the diagonal
   if (fl[0] != nvr) return fail(h, RBMPP_E_SDIAG, "i1/=Nv_RBM (SR_pdposv.f90:32): " + std::to_string(fl[0]) + " positive");
   if (fl[1] != nsl - nsl / N)
@@ -927,7 +962,8 @@ int rbmpp_sr_step(rbmpp_handle* h, const rbmpp_sr_config* c, double* gamma, rbmp
   for (int k = nv - nsl; k < nv; k++) gamma[k] = gamma[k] * c->slater_abs_av / r1;
   if (rep) {
     rep->lambda = lam; rep->delw_max = dmax; rep->scaling = scal; rep->sdiag_av = sav; rep->s_factor = sf;
-    rep->residual = resid; rep->mean_abs_f = r1; rep->k_nonzero = knz; rep->reserved = 0;
+    rep->residual = resid; rep->mean_abs_f = r1; rep->k_nonzero = knz; rep->cg_iters = h->sr_pcg_iters;
+    rep->precond_boost = h->sr_boost;
   }
   rc = rbmpp_set_gamma(h, gamma);
   h->t[5] = h->t[3] + h->t[4];

@@ -53,7 +53,8 @@ struct SrPipeline {
   double *A, *L;             // nv x nv: D S D + shift I and its factor
   double *x, *r, *ax;        // [nv]: solution, residual / correction, A x (also solve scratch)
   double* dinv;              // inverses of the factor's diagonal blocks (chol_dinv_size)
-  double* sc;                // [16] scalars (see k_sr_prepare; sc[7] = 1/norm, sc[8] scratch)
+  double* sc;                // [16] scalars: k_sr_prepare's, sc[7] = 1/norm, sc[8..11] scratch of the
+                             // O-row operator and the conjugate gradients, sc[12] extra diagonal of L
   int* fl;                   // [8] flags and Cholesky info
   // orows != nullptr: the refinement residuals and lambda use the FP64
   // operator of the O rows, D (C^T C / norm) D + shift I with
@@ -65,8 +66,16 @@ struct SrPipeline {
   int rows = 0;
   double* t = nullptr;            // orow_op_scratch(rows, nv) doubles
   double* part = nullptr;         // (inside t's allocation)
+  double *z = nullptr, *pd = nullptr, *tmp = nullptr;  // [nv] conjugate-gradient vectors, solve scratch
 };
+// With orows set, refine_iters counts conjugate-gradient iterations on the
+// operator of the O rows, preconditioned by the assembled matrix's factor
+// (whose diagonal gets sc[12] added: a boost when the assembled matrix is
+// not positive definite); launch_pcg_iters and launch_pcg_final continue
+// such a solve from iteration k0.
 void launch_sr_pipeline(const SrPipeline& p, hipStream_t st);
+void launch_pcg_iters(const SrPipeline& p, int k0, int k1, hipStream_t st);
+void launch_pcg_final(const SrPipeline& p, hipStream_t st);
 // doubles of scratch of the O-row operator
 size_t orow_op_scratch(int rows, int nv);
 // S_jj = inv_norm sum_k (O_kj - kappa_k obar_j)^2 in FP64 over the diagonal

@@ -331,7 +331,7 @@ __global__ __launch_bounds__(1024) void k_sr_prepare(int nv, int nvr, const doub
 __global__ void k_sr_system(int nv, int nvr, const double* __restrict__ sc, const double* __restrict__ S,
                             double* __restrict__ A, double* __restrict__ L, const double* __restrict__ g,
                             double* __restrict__ x) {
-  const double scal = sc[0], shift = sc[1];
+  const double scal = sc[0], shift = sc[1], boost = sc[12];
   const size_t n2 = (size_t)nv * nv;
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n2; e += (size_t)gridDim.x * blockDim.x) {
     const int i = (int)(e / nv), j = (int)(e - (size_t)i * nv);
@@ -340,7 +340,7 @@ __global__ void k_sr_system(int nv, int nvr, const double* __restrict__ sc, cons
     if (j >= nvr) v *= scal;
     if (i == j) v += shift;
     A[e] = v;
-    L[e] = v;
+    L[e] = i == j ? v + boost : v;
     if (e < (size_t)nv) x[e] = e >= (size_t)nvr ? g[e] * scal : g[e];
   }
 }
@@ -575,15 +575,101 @@ __global__ void k_op_finish(int nv, int nvr, const double* __restrict__ sc, cons
   if (r) r[j] = (j >= nvr ? g[j] * scal : g[j]) - yj;
 }
 
-static void launch_orow_op(const SrPipeline& p, double* y, double* r, hipStream_t st) {
+// y = A v for the operator of the O rows; with r: r = D g - y
+static void launch_orow_op(const SrPipeline& p, const double* v, double* y, double* r, hipStream_t st) {
   const int n = p.nv;
-  hipLaunchKernelGGL(k_op_obar, dim3(1), dim3(1024), 0, st, n, p.nvr, p.sc, p.obar, (const double*)p.x);
+  hipLaunchKernelGGL(k_op_obar, dim3(1), dim3(1024), 0, st, n, p.nvr, p.sc, p.obar, v);
   hipLaunchKernelGGL(k_op_rows, dim3((p.rows + 3) / 4), dim3(256), 0, st, p.rows, n, p.nvr, (const double*)p.sc,
-                     p.orows, p.kappa, (const double*)p.x, p.t);
+                     p.orows, p.kappa, v, p.t);
   hipLaunchKernelGGL(k_op_cols, dim3((n + 255) / 256, OP_CHUNKS), dim3(256), 0, st, p.rows, n, p.orows, p.kappa,
                      (const double*)p.t, p.part);
   hipLaunchKernelGGL(k_op_finish, dim3((n + 255) / 256), dim3(256), 0, st, n, p.nvr, (const double*)p.sc, p.obar,
-                     (const double*)p.part, (const double*)p.x, p.g, y, r);
+                     (const double*)p.part, v, p.g, y, r);
+}
+
+// ---------------------------------- conjugate gradients on the O-row operator
+// preconditioned by the Cholesky factor of the assembled matrix.  Scalar
+// slots: sc[9], sc[11] alternate as r.z of successive iterations, sc[10]
+// holds p.q.  A step whose p.q or r.z is not positive (converged, or lost
+// to rounding) is skipped rather than divided by.
+__global__ void k_copy(int n, const double* __restrict__ a, double* __restrict__ b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
+__global__ __launch_bounds__(1024) void k_dot(int n, const double* __restrict__ a, const double* __restrict__ b,
+                                              double* __restrict__ out) {
+  __shared__ double rs[1024];
+  const int tid = threadIdx.x;
+  double t = 0.0;
+  for (int k = tid; k < n; k += 1024) t = fma(a[k], b[k], t);
+  rs[tid] = t;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if (tid < o) rs[tid] += rs[tid + o];
+    __syncthreads();
+  }
+  if (tid == 0) *out = rs[0];
+}
+
+// x += alpha p, r -= alpha q, alpha = rz / pq
+__global__ void k_cg_step(int n, const double* __restrict__ rz, const double* __restrict__ pq,
+                          const double* __restrict__ pd, const double* __restrict__ q, double* __restrict__ x,
+                          double* __restrict__ r) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double a = (*pq > 0.0 && *rz > 0.0) ? *rz / *pq : 0.0;
+  x[i] = fma(a, pd[i], x[i]);
+  r[i] = fma(-a, q[i], r[i]);
+}
+
+// p = z + beta p, beta = rz_new / rz_old
+__global__ void k_cg_dir(int n, const double* __restrict__ rz_new, const double* __restrict__ rz_old,
+                         const double* __restrict__ z, double* __restrict__ pd) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double b = (*rz_old > 0.0 && *rz_new > 0.0) ? *rz_new / *rz_old : 0.0;
+  pd[i] = fma(b, pd[i], z[i]);
+}
+
+// ax = D g - r (A x carried by the recurrence)
+__global__ void k_b_minus(int n, int nvr, const double* __restrict__ sc, const double* __restrict__ g,
+                          const double* __restrict__ r, double* __restrict__ ax) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) ax[i] = (i >= nvr ? g[i] * sc[0] : g[i]) - r[i];
+}
+
+static void launch_pcg_start(const SrPipeline& p, hipStream_t st) {
+  const int n = p.nv, nb = (n + 255) / 256;
+  launch_orow_op(p, p.x, p.ax, p.r, st);  // r = b - A x0
+  hipLaunchKernelGGL(k_copy, dim3(nb), dim3(256), 0, st, n, (const double*)p.r, p.z);
+  launch_chol_solve(n, p.L, p.dinv, p.z, p.tmp, st);
+  hipLaunchKernelGGL(k_copy, dim3(nb), dim3(256), 0, st, n, (const double*)p.z, p.pd);
+  hipLaunchKernelGGL(k_dot, dim3(1), dim3(1024), 0, st, n, (const double*)p.r, (const double*)p.z, p.sc + 9);
+}
+
+void launch_pcg_iters(const SrPipeline& p, int k0, int k1, hipStream_t st) {
+  const int n = p.nv, nb = (n + 255) / 256;
+  for (int k = k0; k < k1; k++) {
+    double* rz_old = p.sc + ((k & 1) ? 11 : 9);
+    double* rz_new = p.sc + ((k & 1) ? 9 : 11);
+    launch_orow_op(p, p.pd, p.ax, nullptr, st);  // q = A p
+    hipLaunchKernelGGL(k_dot, dim3(1), dim3(1024), 0, st, n, (const double*)p.pd, (const double*)p.ax, p.sc + 10);
+    hipLaunchKernelGGL(k_cg_step, dim3(nb), dim3(256), 0, st, n, (const double*)rz_old, (const double*)(p.sc + 10),
+                       (const double*)p.pd, (const double*)p.ax, p.x, p.r);
+    hipLaunchKernelGGL(k_copy, dim3(nb), dim3(256), 0, st, n, (const double*)p.r, p.z);
+    launch_chol_solve(n, p.L, p.dinv, p.z, p.tmp, st);
+    hipLaunchKernelGGL(k_dot, dim3(1), dim3(1024), 0, st, n, (const double*)p.r, (const double*)p.z, rz_new);
+    hipLaunchKernelGGL(k_cg_dir, dim3(nb), dim3(256), 0, st, n, (const double*)rz_new, (const double*)rz_old,
+                       (const double*)p.z, p.pd);
+  }
+}
+
+void launch_pcg_final(const SrPipeline& p, hipStream_t st) {
+  const int n = p.nv;
+  hipLaunchKernelGGL(k_b_minus, dim3((n + 255) / 256), dim3(256), 0, st, n, p.nvr, (const double*)p.sc, p.g,
+                     (const double*)p.r, p.ax);
+  hipLaunchKernelGGL(k_sr_final, dim3(1), dim3(1024), 0, st, n, p.nvr, p.sc, p.x, p.ax, p.g);
 }
 
 size_t orow_op_scratch(int rows, int nv) { return (size_t)rows + (size_t)OP_CHUNKS * (nv + 1); }
@@ -632,14 +718,12 @@ void launch_sr_pipeline(const SrPipeline& p, hipStream_t st) {
   launch_cholesky(n, p.L, p.dinv, p.fl + 4, st);
   launch_chol_solve(n, p.L, p.dinv, p.x, p.ax, st);
   if (p.orows) {
-    // refinement and lambda against the FP64 operator of the O rows
-    for (int it = 0; it < p.refine_iters; it++) {
-      launch_orow_op(p, p.ax, p.r, st);
-      launch_chol_solve(n, p.L, p.dinv, p.r, p.ax, st);
-      hipLaunchKernelGGL(k_axpy1, dim3((n + 255) / 256), dim3(256), 0, st, n, p.r, p.x);
-    }
-    launch_orow_op(p, p.ax, nullptr, st);
-    hipLaunchKernelGGL(k_sr_final, dim3(1), dim3(1024), 0, st, n, p.nvr, p.sc, p.x, p.ax, p.g);
+    // conjugate gradients on the FP64 operator of the O rows, preconditioned
+    // by the factor of the assembled matrix, from its solution; lambda from
+    // the recurrence's A x
+    launch_pcg_start(p, st);
+    launch_pcg_iters(p, 0, p.refine_iters, st);
+    launch_pcg_final(p, st);
     return;
   }
   for (int it = 0; it < p.refine_iters; it++) {

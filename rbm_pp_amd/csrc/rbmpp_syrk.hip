@@ -31,7 +31,10 @@ namespace {
 constexpr int TILE = 128;      // output tile edge
 constexpr int COLB = 128;      // bytes per column per K-block
 constexpr int LDP = COLB + 16; // LDS pitch of a column (bank spread)
-constexpr int FLUSH = 32;      // K-blocks between FP32 -> FP64 flushes
+#ifndef RBMPP_SYRK_FLUSH
+#define RBMPP_SYRK_FLUSH 32
+#endif
+constexpr int FLUSH = RBMPP_SYRK_FLUSH;  // K-blocks between FP32 -> FP64 flushes
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));

@@ -55,7 +55,8 @@ module rbmpp_mod
 
   type, bind(C), public :: rbmpp_sr_report
     real(c_double)     :: lambda, delw_max, scaling, sdiag_av, s_factor, residual, mean_abs_f
-    integer(c_int32_t) :: k_nonzero, reserved
+    integer(c_int32_t) :: k_nonzero, cg_iters
+    real(c_double) :: precond_boost
   end type
 
   public :: rbmpp_create, rbmpp_destroy, rbmpp_last_error, rbmpp_nv

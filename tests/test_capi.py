@@ -37,7 +37,7 @@ def test_config_struct_matches_header():
     # rbmpp_config: 14 int32 + 2 double + uint64 + 2 double, no padding
     assert ctypes.sizeof(api.Config) == 14 * 4 + 2 * 8 + 8 + 3 * 8
     assert ctypes.sizeof(api.SrConfig) == 4 * 8 + 2 * 4
-    assert ctypes.sizeof(api.SrReport) == 7 * 8 + 2 * 4
+    assert ctypes.sizeof(api.SrReport) == 8 * 8 + 2 * 4
 
 
 FMOD = os.path.join(ROOT, "rbm_pp_amd", "fortran", "rbmpp_mod.f90")
