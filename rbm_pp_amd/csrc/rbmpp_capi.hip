@@ -160,7 +160,7 @@ static DevModel devmodel(const rbmpp_handle* h) {
     m.sp_cs[s] = h->sp_cs[s]; m.sp_cc[s] = h->sp_cc[s]; m.sp_ss[s] = h->sp_ss[s]; m.sp_w[s] = h->sp_w[s];
   }
   m.wmap = h->d_wmap; m.invmap = h->d_invmap; m.orb = h->d_orb; m.subB = h->d_subB; m.chi = h->d_chi;
-  m.se = h->d_se; m.Wr = h->d_Wr; m.br = h->d_br; m.Wc = h->d_Wc; m.bc = h->d_bc;
+  m.se = h->d_se; m.f = h->d_f; m.Wr = h->d_Wr; m.br = h->d_br; m.Wc = h->d_Wc; m.bc = h->d_bc;
   return m;
 }
 

@@ -23,6 +23,7 @@ struct DevModel {
   const unsigned char* subB;  // [N]
   const double2* chi;         // [P]     para_mp_trans
   const double* se;           // [C][2N][2N]  se[q][c][r] = Slater_Elm(r,c,q)
+  const double* f;            // [nsl] pair-orbital parameters (Slater_Elm is built from them)
   const double* Wr;           // [ar][N]
   const double* br;           // [ar]
   const double2* Wc;          // [ac][N]

@@ -60,26 +60,31 @@ void launch_slater(const DevModel& m, const double* f, double* se, int* bad, hip
 // shared-memory layout of the sampler workgroup.  Each wave (one copy)
 // keeps a per-row quad {s_j|a_j, t_j|b_j, p_j, q_j} for broadcasts; the quads
 // share storage with the LDS recompute matrix, which is only live between
-// proposals.  RL > 0 (block form without the fused measurement): rows
-// [RR, RR + RL) of every copy's F^-1 stay in LDS between proposals; the
-// recompute's stages overlap them (they are reloaded after every recompute),
-// and the amplitude tables are sized for C * ar <= 256, C * ac <= 64 and
-// ar N + 2 ac N <= 512 (the RBM tables stay in global memory past that).
-template <int NMAX, bool LDSMAT, int RL = 0>
+// proposals.  RL >= 0 selects the slim layout of the block form without the
+// fused measurement: no measurement buffers, the recompute's stages in the
+// union, amplitude tables for C * ar <= 256, C * ac <= 64 and
+// ar N + 2 ac N <= 512 (the RBM tables stay in global memory past that), and
+// the orbital map and f for Slater_Elm lookups in LDS.  RL > 0 keeps rows
+// [RR, RR + RL) of every copy's F^-1 in the union between proposals; the
+// recompute's stages overlap them (they are reloaded after every recompute).
+template <int NMAX, bool LDSMAT, int RL = -1>
 struct SweepSmem {
+  static constexpr bool SLIM = RL >= 0;
   static constexpr int NW = RBMPP_CMAX;
   static constexpr int HS = NMAX / 2;  // row stride of the resident F^-1 rows
-  static constexpr int VEC = RL > 0 ? NW * RL * HS : NW * 4 * NMAX;
-  static constexpr int MRL = RL > 0 ? 256 : RBMPP_MRMAX;  // real angles held
-  static constexpr int MCL = RL > 0 ? 64 : RBMPP_MCMAX;   // complex angles held
+  static constexpr int VEC = (SLIM && NW * RL * HS > NW * 4 * NMAX) ? NW * RL * HS : NW * 4 * NMAX;
+  static constexpr int MRL = SLIM ? 256 : RBMPP_MRMAX;  // real angles held
+  static constexpr int MCL = SLIM ? 64 : RBMPP_MCMAX;   // complex angles held
   static constexpr int MAT = LDSMAT ? (NMAX < RBMPP_NLDS ? NMAX * NMAX : RBMPP_NLDS * RBMPP_NLDS) : 0;
   // fused measurement: one copy's X^-1 (at offset 0) + per-wave a, b vectors
   static constexpr int MWAVES = 8;
-  // (RL > 0: the block recompute's stages, or the N x N recompute matrix)
-  static constexpr int MEAS =
-      LDSMAT ? (RL > 0 ? (NW * 8 * HS > MAT ? NW * 8 * HS : MAT) : MAT + MWAVES * 2 * NMAX) : 0;
+  // (slim: the block recompute's write-back stages)
+  static constexpr int MEAS = LDSMAT ? (SLIM ? NW * 8 * HS : MAT + MWAVES * 2 * NMAX) : 0;
   static constexpr int UNION = VEC > MEAS ? VEC : MEAS;
-  static constexpr int MB = (LDSMAT && RL == 0) ? 4 * NMAX : 1;  // bonds / orbital bins of the fused measurement
+  static constexpr int MB = (LDSMAT && !SLIM) ? 4 * NMAX : 1;  // bonds / orbital bins of the fused measurement
+  // Slater_Elm lookups (slim): orbital index of every site pair, f
+  static constexpr int SO = SLIM ? NMAX * NMAX : 1;
+  static constexpr int SF = SLIM ? 4 * NMAX : 1;
   double u[UNION];
   double2 psib[MB];
   double opp[2][MB];
@@ -102,9 +107,11 @@ struct SweepSmem {
   double gselv[2];
   // RBM tables staged for the proposals' amplitudes (Wr, Wc as doubles;
   // W_map; sublattice flags), used when they fit
-  static constexpr int TW = LDSMAT ? (RL > 0 ? 512 : 1536) : 2;
+  static constexpr int TW = LDSMAT ? (SLIM ? 512 : 1536) : 2;
   static constexpr int TM = LDSMAT ? (RBMPP_CMAX / 2) * NMAX : 1;
   alignas(16) double tw[TW];
+  double sf[SF];
+  unsigned short sorb[SO];
   int tmap[TM];
   unsigned char tsub[LDSMAT ? NMAX : 1];
   unsigned long long ctr;
@@ -120,7 +127,8 @@ __device__ __attribute__((noinline)) void wg_recompute(const DevModel& m, SweepS
   // Pf and X^-1 of each copy: with 16 waves and N <= 112 the register-tile
   // Gauss-Jordan (rbmpp_skew.h), otherwise row-swapping Gauss-Jordan on a
   // matrix in LDS (or global scratch)
-  const bool regpath = LDSMAT && nth == 1024 && N <= 112;
+  constexpr bool SLIM = SweepSmem<NMAX, LDSMAT, RL>::SLIM;  // launched for block-form models only
+  const bool regpath = SLIM || (LDSMAT && nth == 1024 && N <= 112);
   double* A = LDSMAT ? s.u : gscratch;
   const int ld = N;
   double dpf = 0.0, dinv = 0.0;
@@ -128,7 +136,7 @@ __device__ __attribute__((noinline)) void wg_recompute(const DevModel& m, SweepS
     for (int k = tid; k < 4 * rbmpp_skew::MPAD; k += nth) s.u[k] = 0.0;  // pivot-row buffers
   }
   const int H = N >> 1;
-  if (regpath && m.block) {
+  if (SLIM || (regpath && m.block)) {
     // block form: every wave inverts its copy's F (rbmpp_block.h) into
     // Mq[d * np + u] = F^-1(d, u), all copies at once
     const int lane = tid & 63, wid = tid >> 6, np = m.npair;
@@ -156,7 +164,7 @@ __device__ __attribute__((noinline)) void wg_recompute(const DevModel& m, SweepS
     if (s.status) return;
     if (check)
       for (int q = 0; q < C; q++) dinv = fmax(dinv, s.red[q]);
-  } else
+  } else if constexpr (!SLIM)
   for (int q = 0; q < C; q++) {
     const double* S = m.se + (size_t)q * 4 * N * N;
     double* Mq = Mw + (size_t)q * fold_size(N);
@@ -588,7 +596,7 @@ __device__ __forceinline__ uint64_t uni64(uint64_t v) {
 // is streamed from global memory by each ratio pass.  Global memory is made
 // current before a recompute with drift check and at the end of the launch,
 // and the resident rows are reloaded after every recompute.
-template <int NMAX, bool LDSMAT, bool MEAS, bool BLK, int RR = 0, int RL = 0>
+template <int NMAX, bool LDSMAT, bool MEAS, bool BLK, int RR = 0, int RL = -1>
 __device__ __forceinline__ void sweep_walker(const SweepArgs& a, const int w, SweepSmem<NMAX, LDSMAT, RL>& s) {
   constexpr bool RES = RR + RL > 0;
   static_assert(!RES || (BLK && !MEAS && LDSMAT), "resident F^-1 rows: block form without the fused measurement");
@@ -616,6 +624,16 @@ __device__ __forceinline__ void sweep_walker(const SweepArgs& a, const int w, Sw
   for (int q = tid; q < C; q += nth) s.pf[q] = a.ws.pf[(size_t)w * C + q];
   for (int g = tid; g < P; g += nth) s.sgn[g] = a.ws.sgn[(size_t)w * P + g];
   RbmTables tb = model_tables(m);
+  // slim layout: Slater_Elm(a, b + N) of copy q from f and the orbital map in
+  // LDS instead of the global table (block form: one of cc, ss is zero)
+  constexpr bool SLIM = SweepSmem<NMAX, LDSMAT, RL>::SLIM;
+  const bool slt = SLIM && m.nsl <= SweepSmem<NMAX, LDSMAT, RL>::SF && N <= NMAX;
+  if constexpr (SLIM) {
+    if (slt) {
+      for (int i = tid; i < N * N; i += nth) s.sorb[i] = (unsigned short)m.orb[i];
+      for (int k = tid; k < m.nsl; k += nth) s.sf[k] = m.f[k];
+    }
+  }
   if constexpr (LDSMAT) {
     using SS = SweepSmem<NMAX, LDSMAT, RL>;
     const int nwr = m.ar * N, nwc = 2 * m.ac * N;
@@ -650,6 +668,23 @@ __device__ __forceinline__ void sweep_walker(const SweepArgs& a, const int w, Sw
     }
   }
   rng.ctr = uni64(rng.ctr);
+  // The sweep's draws come from a per-wave table of 64 consecutive counters
+  // of the walker's stream, one per lane, formed on the vector unit when the
+  // counter leaves it and read by readlane: the same numbers in the same
+  // order as Rng::u01, without every wave of the workgroup running the same
+  // Philox rounds on the scalar unit it shares with the others.
+  double rtab = 0.0;
+  uint64_t rbase = rng.ctr + RBMPP_WAVE;  // empty
+  auto draw = [&]() -> double {
+    if (rng.ctr - rbase >= (uint64_t)RBMPP_WAVE) {
+      rbase = rng.ctr;
+      rtab = philox_u01(rng.k0, rng.k1, rng.stream, rbase + (uint64_t)lane);
+    }
+    const double u = readlane_d(rtab, (int)(rng.ctr - rbase));
+    rng.ctr++;
+    return u;
+  };
+  auto nran = [&](int n) -> int { return ((int)((double)n * draw())) % n; };  // Rng::nran0
   int nacc_tmp = 0;
   int nfast = a.ws.nfastup[w];
   long long naccept = 0;
@@ -671,7 +706,7 @@ __device__ __forceinline__ void sweep_walker(const SweepArgs& a, const int w, Sw
   // resident rows: breg (registers), lres (LDS, row d - RR at (d - RR) * HS);
   // the host launches this form only with npair >= RR + RL
   double breg[RR > 0 ? RR : 1];
-  double* lres = s.u + (size_t)q * RL * HS;
+  double* lres = s.u + (size_t)q * (RL > 0 ? RL : 0) * HS;
   const int bnp = m.npair;
   const bool blive = lane < bnp;
   const int blo = blive ? lane : 0;  // an in-range column for the lanes past npair
@@ -740,7 +775,7 @@ __device__ __forceinline__ void sweep_walker(const SweepArgs& a, const int w, Sw
     for (int iu = 0; iu < a.nupdate; iu++) {
       rng.ctr = uni64(rng.ctr);
       // ---- global spin flip (make_sample.f90:135-158)
-      if (uni(rng.u01() < a.p_global)) {
+      if (uni(draw() < a.p_global)) {
         const double2 psi_old = s.psi;
         for (int i = tid; i < N; i += nth) s.x[i] = -s.x[i];
         __syncthreads();
@@ -749,7 +784,7 @@ __device__ __forceinline__ void sweep_walker(const SweepArgs& a, const int w, Sw
         pend = false;
         if (uni(s.status)) { status = uni(s.status); break; }
         const double r1 = (cmod(s.psi) + a.shift) / (cmod(psi_old) + a.shift);
-        if (!uni(rng.u01() < r1 * r1)) {
+        if (!uni(draw() < r1 * r1)) {
           for (int i = tid; i < N; i += nth) s.x[i] = -s.x[i];
           __syncthreads();
           wg_x_to_ele(m, s);
@@ -761,12 +796,12 @@ __device__ __forceinline__ void sweep_walker(const SweepArgs& a, const int w, Sw
       // ---- exchange candidate (make_sample.f90:244-294)
       int i1 = -1, i2 = -1;
       {
-        const int dist = rng.nran0(m.L - 1) + 2;
+        const int dist = nran(m.L - 1) + 2;
         for (int it = 0; it < 1000000; it++) {
-          const int c1 = rng.nran0(N);
+          const int c1 = nran(N);
           int ix = c1 / m.L, iy = c1 - (c1 / m.L) * m.L;
-          ix += rng.nran0(dist);
-          iy += rng.nran0(dist);
+          ix += nran(dist);
+          iy += nran(dist);
           if (ix >= m.L) ix -= m.L;
           if (iy >= m.L) iy -= m.L;
           const int c2 = ix * m.L + iy;
@@ -813,7 +848,23 @@ __device__ __forceinline__ void sweep_walker(const SweepArgs& a, const int w, Sw
           const double* T1 = Sq + (size_t)(s1 + N) * 2 * N;
           const double* T2 = Sq + (size_t)(s2 + N) * 2 * N;
           double rv = 0.0;
-          if (lane < np) {
+          if (slt) {
+            // Sl(a, b) = Slater_Elm(a, b + N) = cc f[orb(w a, w b)] + ss f[orb(w b, w a)]
+            // (slater.f90:7-64) with the zero weight's term left out (it
+            // adds an exact zero); Slater_Elm(a + N, b) = -Sl(b, a)
+            const int g = q >> 1;
+            const double wc = m.sp_cc[q & 1], ws = m.sp_ss[q & 1];
+            const int* wm = tb.wmap + g * N;
+            auto sl = [&](int wa, int wb) -> double {
+              return wc != 0.0 ? s.sf[s.sorb[wa * N + wb]] * wc : s.sf[s.sorb[wb * N + wa]] * ws;
+            };
+            const int w1 = wm[s1], w2 = wm[s2];
+            if (lane < np) {
+              const int wd = wm[s.eidx[np + lane]], wu = wm[s.eidx[lane]];
+              rv = lane == db ? sl(w2, w1) - sl(w1, w2) : sl(w2, wd) - sl(w1, wd);
+              bcv = lane == bua ? 0.0 : sl(wu, w1) - sl(wu, w2);
+            }
+          } else if (lane < np) {
             const int sd = s.eidx[np + lane], su = s.eidx[lane];
             rv = lane == db ? S2[s1 + N] - S1[s2 + N] : S2[sd + N] - S1[sd + N];
             bcv = lane == bua ? 0.0 : T2[su] - T1[su];
@@ -1114,7 +1165,7 @@ __device__ __forceinline__ void sweep_walker(const SweepArgs& a, const int w, Sw
       double2 psin = make_double2(0.0, 0.0);
       for (int qq = 0; qq < C; qq++) psin = cadd(psin, s.psiq[qq]);
       const double r1 = (cmod(psin) + a.shift) / (cmod(s.psi) + a.shift);
-      if (uni(rng.u01() < r1 * r1)) {
+      if (uni(draw() < r1 * r1)) {
         // ---- accept: rank-4 update of this copy's X^-1 (calculate_new_InvM_two_child)
         if (q < C) {
           if constexpr (BLK) {
@@ -1330,7 +1381,7 @@ __device__ __forceinline__ void sweep_walker(const SweepArgs& a, const int w, Sw
   }
 }
 
-template <int NMAX, bool LDSMAT, bool MEAS, bool BLK, int RR = 0, int RL = 0>
+template <int NMAX, bool LDSMAT, bool MEAS, bool BLK, int RR = 0, int RL = -1>
 __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
   __shared__ SweepSmem<NMAX, LDSMAT, RL> s;
   const int w = blockIdx.x;
@@ -1338,12 +1389,13 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
   sweep_walker<NMAX, LDSMAT, MEAS, BLK, RR, RL>(a, w, s);
 }
 
-// Resident form for block-form models with 16 copies, N <= 104 and N/2 >= 22:
-// rows [0, 22) of every copy's F^-1 in LDS, the rest streamed.  (Rows kept in
-// registers as well spill: the sweep already fills the 128 registers that a
-// 1024-thread workgroup allows.)
-constexpr int RES_NMAX = 104, RES_RR = 0, RES_RL = 22;
+// Block-form models with 16 copies and N <= 104 use the slim layout (Slater_Elm
+// lookups in LDS); the resident form also keeps rows [0, 16) of every copy's
+// F^-1 in LDS, the rest streamed.  (Rows kept in registers as well spill: the
+// sweep already fills the 128 registers that a 1024-thread workgroup allows.)
+constexpr int RES_NMAX = 104, RES_RR = 0, RES_RL = 16;
 static_assert(sizeof(SweepSmem<RES_NMAX, true, RES_RL>) <= 160 * 1024, "resident rows fit the LDS");
+static_assert(sizeof(SweepSmem<RES_NMAX, true, 0>) <= 160 * 1024, "the slim layout fits the LDS");
 
 // Walkers of several models (lattices of different sizes, couplings and
 // parameters) in one launch: workgroup b belongs to the entry whose block
@@ -1410,6 +1462,12 @@ static bool resident_enabled() {
   const char* e = getenv("RBMPP_SWEEP_RESIDENT");
   return e && e[0] == '1';
 }
+// RBMPP_SWEEP_SLIM=0 selects the full layout (Slater_Elm from the global
+// table) for comparisons
+static bool slim_enabled() {
+  const char* e = getenv("RBMPP_SWEEP_SLIM");
+  return !(e && e[0] == '0');
+}
 
 int launch_sweep(const DevModel& hm, const SweepArgs& a, hipStream_t st) {
   const int N = hm.N;
@@ -1418,9 +1476,13 @@ int launch_sweep(const DevModel& hm, const SweepArgs& a, hipStream_t st) {
   // block form needs the register recompute (16 waves, N <= RBMPP_NLDS)
   const bool blk = hm.block && hm.C == RBMPP_CMAX && N <= RBMPP_NLDS;
   using RS = SweepSmem<RES_NMAX, true, RES_RL>;
-  if (blk && !a.measure && N <= RES_NMAX && hm.npair >= RES_RR + RES_RL && hm.C * hm.ar <= RS::MRL &&
-      hm.C * hm.ac <= RS::MCL && resident_enabled()) {
+  const bool slim = blk && !a.measure && N <= RES_NMAX && hm.C * hm.ar <= RS::MRL && hm.C * hm.ac <= RS::MCL;
+  if (slim && hm.npair >= RES_RR + RES_RL && resident_enabled()) {
     hipLaunchKernelGGL((k_sweep<RES_NMAX, true, false, true, RES_RR, RES_RL>), dim3(a.nwalk), dim3(1024), 0, st, a);
+    return 0;
+  }
+  if (slim && slim_enabled()) {
+    hipLaunchKernelGGL((k_sweep<RES_NMAX, true, false, true, 0, 0>), dim3(a.nwalk), dim3(1024), 0, st, a);
     return 0;
   }
   if (N <= 64) launch_sweep_t<64, true>(hm.C, blk, a, st);

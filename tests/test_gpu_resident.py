@@ -1,9 +1,12 @@
-"""The sampler's resident form (rows [0, 22) of every copy's F^-1 kept in LDS
-for the whole launch, the rest streamed from global memory) against the
-streaming form on the same walkers and seeds: the same configurations and
-acceptance counts, amplitudes to rounding, and F^-1 as close to a recompute
-of the final configuration as the streaming form's (the two forms group the
-wave sums of the pending update differently).  The cases
+"""The sampler's layouts for block-form models on the same walkers and seeds.
+The slim layout (the default at N <= 104: Slater_Elm entries looked up from
+f and the orbital map in LDS) against the full layout (the global Slater_Elm
+table): bit-identical samples, amplitudes and F^-1.  The resident form (rows
+[0, 16) of every copy's F^-1 kept in LDS for the whole launch, the rest
+streamed) against the slim one: the same configurations and acceptance
+counts, amplitudes to rounding, and F^-1 as close to a recompute of the
+final configuration (the two group the wave sums of the pending update
+differently).  The cases
 cover the adaptive and a fixed recompute period (the flush before every
 checked recompute and the reload after it), a tight conditioning guard
 (recomputes after single accepts) and global spin flips."""
@@ -21,11 +24,12 @@ def _relerr(a, b):
     return np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-300)
 
 
-def _run(resident, spec, p, W, ns, K, guard, p_global):
+def _run(form, spec, p, W, ns, K, guard, p_global):
     """Two make_sample calls; the samples, the walkers and the drift of each
     walker's F^-1 from a recompute of its final configuration."""
     from rbm_pp_amd import api
-    os.environ["RBMPP_SWEEP_RESIDENT"] = "1" if resident else "0"
+    os.environ["RBMPP_SWEEP_RESIDENT"] = "1" if form == "resident" else "0"
+    os.environ["RBMPP_SWEEP_SLIM"] = "0" if form == "full" else "1"
     try:
         eng = api.Engine(spec, params=p, n_walkers=W, n_samples=ns, seed=77, n_warmup=20, recompute_k=K,
                          cond_guard=guard, p_global=p_global)
@@ -35,6 +39,7 @@ def _run(resident, spec, p, W, ns, K, guard, p_global):
         wk = eng.get_walkers(with_invm=True)
     finally:
         os.environ.pop("RBMPP_SWEEP_RESIDENT", None)
+        os.environ.pop("RBMPP_SWEEP_SLIM", None)
     # InvM_walker(i, j) = InvM_eval(rank[i], rank[j]): the walker keeps its
     # own electron labels, rbmpp_eval numbers them in site order per spin
     ele = eng.walker_electrons()
@@ -52,13 +57,18 @@ def _run(resident, spec, p, W, ns, K, guard, p_global):
 
 @pytest.mark.parametrize("L,ar,K,guard,p_global", [(10, 4, 0, 1e4, 3e-4), (10, 4, 12, 1e4, 0.02),
                                                    (10, 4, 0, 3.0, 3e-4), (8, 2, 0, 1e4, 3e-4)])
-def test_resident_matches_streaming(L, ar, K, guard, p_global):
+def test_sampler_layouts_agree(L, ar, K, guard, p_global):
     spec = lt.j1j2_model(L, alphar=ar, J2=0.5)
     p = lt.random_params(spec, seed_w=3, seed_f=4, sigma_w=0.1)
     W, ns = 192, 6
     print(f"resident vs streaming: L={L} ar={ar} K={K} guard={guard} p_global={p_global} W={W}", flush=True)
-    xs, ps, ws, ds = _run(False, spec, p, W, ns, K, guard, p_global)
-    xr, pr, wr, dr = _run(True, spec, p, W, ns, K, guard, p_global)
+    xs, ps, ws, ds = _run("slim", spec, p, W, ns, K, guard, p_global)
+    xf, pf, wf, _ = _run("full", spec, p, W, ns, K, guard, p_global)
+    assert np.array_equal(xf, xs)
+    assert np.array_equal(pf, ps)
+    assert np.array_equal(wf["invm"], ws["invm"])
+    assert np.array_equal(wf["naccept"], ws["naccept"])
+    xr, pr, wr, dr = _run("resident", spec, p, W, ns, K, guard, p_global)
     assert np.array_equal(xr, xs)
     assert np.array_equal(wr["naccept"], ws["naccept"])
     assert np.array_equal(wr["x"], ws["x"])
@@ -67,5 +77,5 @@ def test_resident_matches_streaming(L, ar, K, guard, p_global):
     # stays as close to a recompute as the other's
     assert _relerr(pr, ps) < 1e-7
     assert _relerr(wr["pf"], ws["pf"]) < 1e-7
-    print(f"F^-1 drift from a recompute: resident {dr.max():.2e}, streaming {ds.max():.2e}", flush=True)
+    print(f"F^-1 drift from a recompute: resident {dr.max():.2e}, slim {ds.max():.2e}", flush=True)
     assert dr.max() <= max(1e-7, 4 * ds.max())
