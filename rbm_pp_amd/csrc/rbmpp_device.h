@@ -143,6 +143,24 @@ __device__ __forceinline__ double dpp_add(double v) {
   const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWMASK, 0xf, false);
   return v + __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
 }
+// v * (v moved by the DPP pattern CTRL); lanes without a source multiply by 1
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_mul(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, ROWMASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0x3ff00000, (int)(b >> 32), CTRL, ROWMASK, 0xf, false);
+  return v * __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+// Product over the wave, valid in lane 63 (the stages of wave_sum_to63)
+__device__ __forceinline__ double wave_prod_to63(double v) {
+  v = dpp_mul<0x111, 0xf>(v);
+  v = dpp_mul<0x112, 0xf>(v);
+  v = dpp_mul<0x114, 0xf>(v);
+  v = dpp_mul<0x118, 0xf>(v);
+  v = dpp_mul<0x142, 0xa>(v);
+  v = dpp_mul<0x143, 0xc>(v);
+  return v;
+}
 // Sum over the wave, valid in lane 63 (row_shr 1, 2, 4, 8 within rows of 16
 // lanes, then row_bcast 15 and 31 across rows).
 __device__ __forceinline__ double wave_sum_to63(double v) {
@@ -314,11 +332,15 @@ __device__ __forceinline__ double2 psi_wo_pf_exchanged_wave(const DevModel& m, c
     t = t - 2.0 * tb.Wr[f * m.N + iw2] * (double)x2 * s;
     r1 = r1 * cosh(t);
   }
+  // the real factors by DPP stages into lane 63; the complex ones (models
+  // with alphac > 0) by a butterfly
+  r1 = readlane_d(wave_prod_to63(r1), 63);
+  if (m.ac > 0) {
 #pragma unroll
-  for (int o = 1; o < RBMPP_WAVE; o <<= 1) {
-    r1 = r1 * __shfl_xor(r1, o, RBMPP_WAVE);
-    const double2 oc = make_double2(__shfl_xor(c.x, o, RBMPP_WAVE), __shfl_xor(c.y, o, RBMPP_WAVE));
-    c = cmul(c, oc);
+    for (int o = 1; o < RBMPP_WAVE; o <<= 1) {
+      const double2 oc = make_double2(__shfl_xor(c.x, o, RBMPP_WAVE), __shfl_xor(c.y, o, RBMPP_WAVE));
+      c = cmul(c, oc);
+    }
   }
   c = cscale(c, r1);
   c = cscale(c, (double)sgn_new);

@@ -684,7 +684,14 @@ __device__ __forceinline__ void sweep_walker(const SweepArgs& a, const int w, Sw
     rng.ctr++;
     return u;
   };
-  auto nran = [&](int n) -> int { return ((int)((double)n * draw())) % n; };  // Rng::nran0
+  // Rng::nran0: (int)(n u) lies in [0, n], so the modulo is one subtraction
+  auto nran = [&](int n) -> int {
+    const int r = (int)((double)n * draw());
+    return r >= n ? r - n : r;
+  };
+  // c / L for the candidate's site arithmetic: the high word of c * ceil(2^32 / L)
+  // is exact for L <= 256 and c < L^2
+  const uint32_t mL = (uint32_t)(((1ull << 32) + (uint64_t)m.L - 1) / (uint64_t)m.L);
   int nacc_tmp = 0;
   int nfast = a.ws.nfastup[w];
   long long naccept = 0;
@@ -799,7 +806,7 @@ __device__ __forceinline__ void sweep_walker(const SweepArgs& a, const int w, Sw
         const int dist = nran(m.L - 1) + 2;
         for (int it = 0; it < 1000000; it++) {
           const int c1 = nran(N);
-          int ix = c1 / m.L, iy = c1 - (c1 / m.L) * m.L;
+          int ix = (int)__umulhi((uint32_t)c1, mL), iy = c1 - ix * m.L;
           ix += nran(dist);
           iy += nran(dist);
           if (ix >= m.L) ix -= m.L;

@@ -19,7 +19,7 @@
 // loaded.  Per lane the MFMA operands are contiguous along k (a double4, two
 // float4 or a bf16x8 pair), so FP64 and FP32 products use the lane's four k
 // values over four MFMAs (k slot s of lane l holds k = 4 (l/16) + s for A and
-// B alike).  FP32 accumulators are added into FP64 ones every 32 K-blocks;
+// B alike).  FP32 accumulators are added into FP64 ones every FLUSH (128) K-blocks;
 // split partials are summed in FP64, in a fixed order, by k_syrk_reduce.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -32,7 +32,7 @@ constexpr int TILE = 128;      // output tile edge
 constexpr int COLB = 128;      // bytes per column per K-block
 constexpr int LDP = COLB + 16; // LDS pitch of a column (bank spread)
 #ifndef RBMPP_SYRK_FLUSH
-#define RBMPP_SYRK_FLUSH 32
+#define RBMPP_SYRK_FLUSH 128
 #endif
 constexpr int FLUSH = RBMPP_SYRK_FLUSH;  // K-blocks between FP32 -> FP64 flushes
 
