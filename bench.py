@@ -171,11 +171,15 @@ def sr_shape_bench(api, lt, L, alphar, J2, ns, nsl=None, seed=5):
     return out
 
 
-def run_sr_shapes(api, lt):
+def run_sr_shapes(api, lt, sizes="both"):
     """The 16x16 workload's SR step (alphar = 16, Nv = 5136, 4x10^5
     configurations) and a synthetic Nv = 19984 (4x10^5 configurations)."""
-    return [sr_shape_bench(api, lt, 16, 16, 0.55, 400000),
-            sr_shape_bench(api, lt, 16, 16, 0.55, 400000, nsl=62 * 256)]
+    out = []
+    if sizes in ("both", "5136"):
+        out.append(sr_shape_bench(api, lt, 16, 16, 0.55, 400000))
+    if sizes in ("both", "19984"):
+        out.append(sr_shape_bench(api, lt, 16, 16, 0.55, 400000, nsl=62 * 256))
+    return out
 
 
 def energy_run(args):
@@ -325,6 +329,8 @@ def main():
                     help="skip the SR steps at the 16x16 sizes (Nv = 5136 and a synthetic ~2e4)")
     ap.add_argument("--sr-only", action="store_true",
                     help="run only the SR steps at the large sizes and print them")
+    ap.add_argument("--sr-sizes", choices=["both", "5136", "19984"], default="both",
+                    help="which of the large SR shapes --sr-only runs")
     ap.add_argument("--energy-run", choices=["fp64", "fp32", "bf16x3"],
                     help="one full optimisation through the Fortran driver with this S precision; prints "
                          "the final energy per site and its error bar")
@@ -368,7 +374,7 @@ def main():
     if args.sr_only:
         from rbm_pp_amd import api
         from rbm_pp_amd import lattice as lt
-        print(json.dumps({"sr_shapes": run_sr_shapes(api, lt)}))
+        print(json.dumps({"sr_shapes": run_sr_shapes(api, lt, args.sr_sizes)}))
         return
 
     # CPU baseline first: worker processes are forked before the GPU is touched

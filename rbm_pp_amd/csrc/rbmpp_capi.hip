@@ -535,6 +535,10 @@ static int ensure_part(rbmpp_handle* h, size_t n) {
 
 // workgroups of the block-form estimator (each holds C (N/2)^2 doubles of scratch)
 static const int BLK_GRID = 1024;
+// workgroups of the correlation run (each keeps its own partial sums); the
+// block form launches at most BLK_GRID, and every slice summed must be written
+static const int CORR_GRID = 1024;
+static_assert(CORR_GRID <= BLK_GRID, "the correlation run's workgroups all run in block form too");
 
 static int run_local(rbmpp_handle* h, int n, const signed char* d_x, int mode, double shift, const int* ex1,
                      const int* ex2, double* d_invm, double* d_pf, double2* d_psiq, double2* d_expsi,
@@ -566,7 +570,7 @@ static int run_local(rbmpp_handle* h, int n, const signed char* d_x, int mode, d
   int cap = grid_cap > 0 ? grid_cap : h->local_grid;
   if (grid_cap <= 0 && h->tune_meas_grid > 0) cap = std::min(cap, h->tune_meas_grid);
   const DevModel hm0 = devmodel(h);
-  if (!d_corr && hm0.block) {
+  if (hm0.block) {
     // the block-form estimator keeps F^-1 of every copy per workgroup
     cap = std::min(cap, BLK_GRID);
     if (!h->d_bscr && (rc = dalloc(h, &h->d_bscr, (size_t)BLK_GRID * h->C * (h->N / 2) * (h->N / 2)))) return rc;
@@ -670,9 +674,6 @@ int rbmpp_measure_energy(rbmpp_handle* h, double shift, double* etot, double* et
   if (wf_av) *wf_av = h->m_wfav;
   return RBMPP_OK;
 }
-
-// workgroups of the correlation run (each keeps its own partial sums)
-static const int CORR_GRID = 1024;
 
 int rbmpp_measure_correlation(rbmpp_handle* h, double shift, double* etot, double* etot2, double* norm,
                               double* wf_av, double* wf_sgn, double* chi_sp, double* chi_vbs) {

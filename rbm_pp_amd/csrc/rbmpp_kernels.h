@@ -114,5 +114,8 @@ void launch_meas(const LocalArgs& a, int grid, hipStream_t s);
 void launch_meas_blk(const LocalArgs& a, int grid, hipStream_t s);
 // the same estimator with the correlation measurement (N <= RBMPP_NLDS)
 void launch_meas_corr(const LocalArgs& a, int grid, hipStream_t s);
+// the block-form estimator with the correlation measurement; a.scratch as for
+// launch_meas_blk, a.corr_part [grid][corr_len(N)]
+void launch_meas_blk_corr(const LocalArgs& a, int grid, hipStream_t s);
 // out[k] = sum_c part[c * stride + k] for k < len, c < nchunk (fixed order)
 void launch_sum_strided(int len, int nchunk, int stride, const double* part, double* out, hipStream_t s);

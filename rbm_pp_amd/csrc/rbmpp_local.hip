@@ -265,9 +265,16 @@ int launch_local(const DevModel& hm, const LocalArgs& a, int grid, hipStream_t s
   if (hm.C > RBMPP_CMAX || hm.mr > RBMPP_MRMAX || hm.mc > RBMPP_MCMAX) return -1;
   if (hm.nsl > 4 * N || (N & 1)) return -1;
   if (a.corr_part) {
-    // the correlation run: pair pass of the register estimator only
-    if (N > RBMPP_NLDS || hm.L < 4 || (N & 3)) return -1;
-    launch_meas_corr(a, grid, st);
+    // the correlation run: the pair pass of the block-form or the register
+    // estimator
+    if (N > RBMPP_NLDS || hm.L < 4) return -1;
+    if (hm.block && hm.C == RBMPP_CMAX) {
+      if (!a.scratch) return -1;
+      launch_meas_blk_corr(a, grid, st);
+    } else {
+      if (N & 3) return -1;
+      launch_meas_corr(a, grid, st);
+    }
   } else if (hm.block && hm.C == RBMPP_CMAX && N <= RBMPP_NLDS) {
     if (!a.scratch) return -1;
     launch_meas_blk(a, grid, st);

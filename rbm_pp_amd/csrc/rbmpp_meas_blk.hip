@@ -1,274 +1,7 @@
-// Per-configuration estimators in block form (DevModel::block, normal
-// projection): every X is [[0, F], [-F^T, 0]] with F the np x np block of up
-// rows and down columns, so Pf X = (-1)^(np(np-1)/2) det F and
-// X^-1 = [[0, -F^-T], [F^-1, 0]] (calculate_MAll, matrix.f90:6-110, for this
-// structure).  One workgroup of 1024 threads per configuration (grid-stride):
-//  * each of the 16 waves inverts one copy's F by Gauss-Jordan in registers
-//    (rbmpp_block.h: lane i holds row i; partial pivoting over the rows not
-//    yet used, chosen logically), with no workgroup barrier, and writes F^-1
-//    to the workgroup's global scratch;
-//  * then copy by copy F^-1 is staged in LDS for the exchange ratio of every
-//    antiparallel bond (calculate_new_PfM_two_child, pfupdate_two.f90:27-88,
-//    as det F'/det F for F' = F + e_ua r^T + c e_db^T) and for the
-//    pair-product log-derivatives (slater.f90:136-167).
-// The energy and O rows follow calc_phys_quantity.f90:20-246 as in the skew
-// estimator (rbmpp_meas_impl.h).
-#include "rbmpp_block.h"
-#include "rbmpp_kernels.h"
-#include "rbmpp_measure.h"
-
-namespace {
-
-using rbmpp_block::copy_block_inverse;
-
-constexpr int BNT = 1024;                // threads per workgroup
-constexpr int BNW = BNT / RBMPP_WAVE;    // waves: one per projection copy
-constexpr int NMX = RBMPP_NLDS;          // largest N
-constexpr int HMX = rbmpp_block::HMX;    // largest np = N/2
-constexpr int BT = 4;                    // bonds per wave and pass
-
-struct BlkSmem {
-  double binv[HMX * HMX];        // F^-1 of the current copy: B[d][u] at d * np + u
-  int piv[BNW][HMX];             // pivot row (lane) of each elimination step
-  double stage[BNW][8 * HMX];    // write-back stage of each wave's elimination
-  double vr[BNW][BT][HMX];       // r vectors of the bonds in flight
-  double opp[2][4 * NMX];        // pair-product O, re / im
-  double2 psib[4 * NMX];         // per antiparallel bond: sum over copies of psi(x')
-  double rat[4 * NMX];           // per antiparallel bond: Pf ratio of the current copy
-  double thr[RBMPP_MRMAX];
-  double2 thc[RBMPP_MCMAX];
-  double pf[RBMPP_CMAX];
-  double2 psiq[RBMPP_CMAX], psiwo[RBMPP_CMAX];
-  double selv[2];
-  int x[NMX], eidx[NMX], ecfg[NMX], site[NMX];
-  short bi1[4 * NMX], bi2[4 * NMX];
-  int sgn[RBMPP_CMAX / 2];
-  unsigned sgnbits[RBMPP_CMAX / 2];
-  int nb, status;
-};
-
-// cold helpers kept out of line (their transcendental temporaries stay out of
-// the register allocation of the elimination and bond loops)
-__device__ __attribute__((noinline)) double2 exchanged_amp(const DevModel& m, int q, const BlkSmem& s, int sgn,
-                                                           int i1, int i2) {
-  return psi_wo_pf_exchanged(m, q, s.thc, s.thr, sgn, i1, i2, s.x[i1], s.x[i2]);
-}
-__device__ __attribute__((noinline)) void theta_sign(const DevModel& m, BlkSmem& s) { wg_theta_sign(m, s, 0); }
-__device__ __attribute__((noinline)) void o_row(const DevModel& m, const BlkSmem& s, double rw, double* ore,
-                                                double* oim, bool cplx_rows) {
-  wg_orow(m, s, rw, ore, oim, cplx_rows);
-}
-__device__ __attribute__((noinline)) int bonds_of(const DevModel& m, BlkSmem& s) { return bond_list(m, s); }
-__device__ __attribute__((noinline)) double2 eloc_of(const DevModel& m, const BlkSmem& s, double2 p) {
-  return eloc_from_bonds(m, s, p);
-}
-
-__global__ __launch_bounds__(BNT) void k_meas_blk(LocalArgs a) {
-  __shared__ BlkSmem s;
-  const DevModel& m = *a.m;
-  const int tid = threadIdx.x, nth = BNT;
-  const int lane = tid & (RBMPP_WAVE - 1), wid = tid >> 6;
-  const int N = m.N, C = m.C, np = m.npair;
-  const bool want_eloc = a.mode & 1, want_o = a.mode & 2, want_inv = a.mode & 4, custom = a.mode & 8;
-  const double pfsign = ((np * (np - 1) / 2) & 1) ? -1.0 : 1.0;  // Pf X = pfsign det F
-  double* Bw = a.scratch + (size_t)blockIdx.x * C * np * np;  // F^-1 of every copy
-
-  for (int cfg = blockIdx.x; cfg < a.nconf; cfg += gridDim.x) {
-    __syncthreads();
-    for (int i = tid; i < N; i += nth) s.x[i] = a.xs[(size_t)cfg * N + i];
-    if (tid == 0) s.status = WS_OK;
-    for (int k = tid; k < 4 * NMX; k += nth) {
-      s.opp[0][k] = 0.0;
-      s.opp[1][k] = 0.0;
-      s.psib[k] = make_double2(0.0, 0.0);
-    }
-    __syncthreads();
-    if (tid == 0) {
-      // x_to_Ele_Idx_Ele_Cfg (interface_for_PP.f90:5-33)
-      int u = 0, d = np;
-      for (int i = 0; i < N; i++) {
-        if (s.x[i] == 1) { s.eidx[u] = i; s.ecfg[i] = u; u++; }
-        else { s.eidx[d] = i; s.ecfg[i] = d; d++; }
-      }
-      if (u != np) s.status = WS_INIT;
-      int nb = 0;
-      if (custom) {
-        s.bi1[0] = (short)a.ex_i1[cfg];
-        s.bi2[0] = (short)a.ex_i2[cfg];
-        nb = 1;
-      } else if (want_eloc) {
-        nb = bonds_of(m, s);
-      }
-      s.nb = nb;
-    }
-    __syncthreads();
-    if (s.status != WS_OK) {
-      if (tid == 0 && a.status) a.status[cfg] = s.status;
-      continue;
-    }
-    theta_sign(m, s);
-    for (int q = tid; q < C; q += nth) s.psiwo[q] = psi_wo_pf(m, q, s.thc, s.thr, s.sgn[q >> 1]);
-    // ---- det and F^-1 of every copy, one wave each
-    if (wid < C) {
-      const int q = wid;
-      const double det = copy_block_inverse(m.se + (size_t)q * 4 * N * N, N, np, s.eidx, s.piv[wid],
-                                            s.stage[wid], Bw + (size_t)q * np * np, lane);
-      if (lane == 0) {
-        s.pf[q] = pfsign * det;
-        if (det == 0.0) s.status = WS_SINGULAR;
-      }
-    }
-    __syncthreads();
-    if (s.status != WS_OK) {
-      if (tid == 0 && a.status) a.status[cfg] = s.status;
-      continue;
-    }
-    const int nb = s.nb;
-    for (int q = 0; q < C; q++) {
-      const int g = q >> 1, sidx = q & 1;
-      const double* S = m.se + (size_t)q * 4 * N * N;
-      const double pfq = s.pf[q];
-      __syncthreads();  // the previous copy's readers of binv and site are done
-      for (int e = tid; e < np * np; e += nth) s.binv[e] = Bw[(size_t)q * np * np + e];
-      for (int e = tid; e < N; e += nth) s.site[e] = m.wmap[g * N + s.eidx[e]];
-      if (tid == 0) s.psiq[q] = cscale(s.psiwo[q], pfq);
-      __syncthreads();
-      if (want_inv) {
-        // reference layout out[i*N + j] = X^-1(i, j): X^-1(u, np+d) = -F^-1(d, u),
-        // X^-1(np+d, u) = F^-1(d, u), zero same-spin blocks
-        double* out = a.invm + ((size_t)cfg * C + q) * N * N;
-        for (int e = tid; e < N * N; e += nth) {
-          const int i = e / N, j = e - i * N;
-          double v = 0.0;
-          if (i < np && j >= np) v = -s.binv[(j - np) * np + i];
-          else if (i >= np && j < np) v = s.binv[(i - np) * np + j];
-          out[e] = v;
-        }
-      }
-      if (want_o) {
-        // calculate_derivative_Pfaffian_child (slater.f90:136-167) weighted by
-        // psi_wo_pf(q): X^-1(u, np+d) = -F^-1(d,u) lands in the orbital of
-        // (site(np+d), site(u)) with weight -ss, X^-1(np+d, u) = F^-1(d,u) in
-        // that of (site(u), site(np+d)) with weight cc
-        const double cc = pfq * m.sp_cc[sidx], ss = pfq * m.sp_ss[sidx];
-        const double2 wq = s.psiwo[q];
-        for (int e = tid; e < np * np; e += nth) {
-          const int d = e / np, u = e - d * np;
-          const double bv = s.binv[e];
-          const int su = s.site[u], sd = s.site[np + d];
-          const int o1 = m.orb[sd * N + su], o2 = m.orb[su * N + sd];
-          const double d1 = ss * bv, d2 = cc * bv;
-          atomicAdd(&s.opp[0][o1], d1 * wq.x);
-          atomicAdd(&s.opp[0][o2], d2 * wq.x);
-          if (!m.real_only) {
-            atomicAdd(&s.opp[1][o1], d1 * wq.y);
-            atomicAdd(&s.opp[1][o2], d2 * wq.y);
-          }
-        }
-      }
-      // exchange ratios, BT bonds per wave and pass: the up electron ua moves
-      // from s1 to s2, the down electron db from s2 to s1; with B = F^-1,
-      // y = r^T B: det F'/det F = (1 + y_ua)(1 + B[db].c) - (y.c) B[db][ua]
-      for (int p = wid; BT * p < nb; p += BNW) {
-        int bb[BT], ua[BT], db[BT];
-        double cv[BT];
-#pragma unroll
-        for (int t = 0; t < BT; t++) {
-          bb[t] = min(BT * p + t, nb - 1);  // a missing bond repeats the last one (not written)
-          const int i1 = s.bi1[bb[t]], i2 = s.bi2[bb[t]];
-          const int msa = s.ecfg[i1], msb = s.ecfg[i2];
-          const bool aup = msa < np;
-          ua[t] = __builtin_amdgcn_readfirstlane(aup ? msa : msb);
-          db[t] = __builtin_amdgcn_readfirstlane((aup ? msb : msa) - np);
-          const int s1 = aup ? i1 : i2, s2 = aup ? i2 : i1;
-          const double* S1 = S + (size_t)s1 * 2 * N;
-          const double* S2 = S + (size_t)s2 * 2 * N;
-          // c(u) from rows s1+N and s2+N: the off-diagonal blocks of the
-          // block form are exact negations of each other (one weight is zero)
-          const double* T1 = S + (size_t)(s1 + N) * 2 * N;
-          const double* T2 = S + (size_t)(s2 + N) * 2 * N;
-          double rv = 0.0;
-          cv[t] = 0.0;
-          if (lane < np) {
-            const int sd = s.eidx[np + lane], su = s.eidx[lane];
-            rv = lane == db[t] ? S2[s1 + N] - S1[s2 + N] : S2[sd + N] - S1[sd + N];
-            cv[t] = lane == ua[t] ? 0.0 : T2[su] - T1[su];
-          }
-          if (lane < np) s.vr[wid][t][lane] = rv;
-        }
-        wave_sync();
-        double y[BT];
-#pragma unroll
-        for (int t = 0; t < BT; t++) y[t] = 0.0;
-        if (lane < np) {
-#pragma unroll 4
-          for (int d = 0; d < np; d++) {
-            const double bv = s.binv[d * np + lane];
-#pragma unroll
-            for (int t = 0; t < BT; t++) y[t] = fma(s.vr[wid][t][d], bv, y[t]);
-          }
-        }
-        // B[db].c and y.c of the four bonds by two four-way wave sums
-        // (lanes 15, 47, 31, 63 hold bonds 0-3)
-        static_assert(BT == 4, "the four-way wave sums take four bonds");
-        double brow[BT];
-#pragma unroll
-        for (int t = 0; t < BT; t++) brow[t] = lane < np ? s.binv[db[t] * np + lane] : 0.0;
-        const double dv = wave_sum4(brow[0] * cv[0], brow[1] * cv[1], brow[2] * cv[2], brow[3] * cv[3]);
-        const double ev = wave_sum4(y[0] * cv[0], y[1] * cv[1], y[2] * cv[2], y[3] * cv[3]);
-        constexpr int SUMLANE[BT] = {15, 47, 31, 63};
-#pragma unroll
-        for (int t = 0; t < BT; t++) {
-          if (BT * p + t >= nb) break;  // wave-uniform
-          const double gam = readlane_d(y[t], ua[t]);
-          const double bet = readlane_d(brow[t], ua[t]);
-          const double del = readlane_d(dv, SUMLANE[t]);
-          const double eps = readlane_d(ev, SUMLANE[t]);
-          if (lane == 0) s.rat[bb[t]] = (1.0 + gam) * (1.0 + del) - eps * bet;
-        }
-        wave_sync();  // vr is rewritten by the next pass
-      }
-      __syncthreads();
-      // psi(x') of this copy for every bond, one thread per bond
-      for (int b = tid; b < nb; b += nth) {
-        const int j1 = s.bi1[b], j2 = s.bi2[b];
-        int sfac = -1;
-        if (m.subB[j1]) sfac = -sfac;
-        if (m.subB[j2]) sfac = -sfac;
-        const double pfn = s.rat[b] * pfq;
-        const double2 c = exchanged_amp(m, q, s, sfac * s.sgn[g], j1, j2);
-        s.psib[b] = cadd(s.psib[b], cscale(c, pfn));
-        if (custom && a.ex_pf) a.ex_pf[(size_t)cfg * C + q] = pfn;
-      }
-    }
-    __syncthreads();
-    // amplitude, local energy, reweighting
-    if (tid == 0) {
-      double2 p = make_double2(0.0, 0.0);
-      for (int q = 0; q < C; q++) p = cadd(p, s.psiq[q]);
-      const double rw = a.shift >= 0.0 ? 1.0 / (cmod(p) + a.shift) : 1.0;
-      a.psi[cfg] = p;
-      if (a.rw) a.rw[cfg] = rw;
-      if (want_eloc && !custom) a.eloc[cfg] = cscale(eloc_of(m, s, p), rw);
-      if (custom && a.ex_psi) a.ex_psi[cfg] = s.psib[0];
-      s.selv[0] = rw;
-    }
-    for (int q = tid; q < C; q += nth) {
-      if (a.pf) a.pf[(size_t)cfg * C + q] = s.pf[q];
-      if (a.psi_q) a.psi_q[(size_t)cfg * C + q] = s.psiq[q];
-    }
-    __syncthreads();
-    if (want_o) {
-      const bool cplx_rows = !m.real_only;
-      double* ore = a.orows + (size_t)(cplx_rows ? 2 * cfg : cfg) * m.nv;
-      o_row(m, s, s.selv[0], ore, ore + m.nv, cplx_rows);
-    }
-    if (tid == 0 && a.status) a.status[cfg] = WS_OK;
-  }
-}
-
-}  // namespace
+// The block-form estimator (k_meas_blk<false>); the kernel itself is in
+// rbmpp_meas_blk_impl.h.
+#include "rbmpp_meas_blk_impl.h"
 
 void launch_meas_blk(const LocalArgs& a, int grid, hipStream_t st) {
-  hipLaunchKernelGGL(k_meas_blk, dim3(grid), dim3(BNT), 0, st, a);
+  hipLaunchKernelGGL(k_meas_blk<false>, dim3(grid), dim3(BNT), 0, st, a);
 }

@@ -466,42 +466,10 @@ __global__ __launch_bounds__(MNT) void k_meas(LocalArgs a) {
       o_row(m, s, s.selv[0], ore, ore + m.nv, cplx_rows);
     }
     if constexpr (CORR) {
-      // Calc_Correlation/calc_phys_quantity.f90:318-338: SdotS per bond
-      // (calc_local_energy_correlation, :348-413) reweighted, then
-      // Chi_vbs += Re(conj SdotS(i1,t) SdotS(i2,t)) and
-      // Chi_sp += rw^2 Re(conj(psi) Chi_loc) (calc_local_correlation, :469-550)
       const double2 p = s.psi_cur;
-      const double rw = s.selv[0], w2 = rw * rw, ap2 = p.x * p.x + p.y * p.y;
-      double2* sdb = (double2*)&s.opp[0][0];  // SdotS(i1, t) at sdb[t * N + i1]
-      for (int e = tid; e < 4 * N; e += nth) {
-        const int t = e / N, i1 = e - t * N;
-        sdb[e] = cscale(p, (double)(s.x[i1] * s.x[bond_partner(m.L, t, i1)]));
-      }
-      __syncthreads();
-      for (int b = tid; b < nb; b += nth) {
-        const int i1 = s.bi1[b], i2 = s.bi2[b];
-        int t = 0;
-        while (t < 3 && bond_partner(m.L, t, i1) != i2) t++;  // partners are distinct for L >= 3
-        sdb[t * N + i1] = cadd(sdb[t * N + i1], cscale(s.psib[b], t <= 1 ? -2.0 : 2.0));
-      }
-      __syncthreads();
-      for (int e = tid; e < 4 * N; e += nth) sdb[e] = cscale(sdb[e], rw);
-      __syncthreads();
-      double* sp1 = cpart;
+      const double rw = s.selv[0], w2 = rw * rw;
+      corr_accumulate(m, s, p, rw, nb, cpart, nchi);
       double* sp2 = cpart + nchi;
-      double* vbs = cpart + 2 * nchi;
-      for (int e = tid; e < N * N; e += nth) {
-        const int i2 = e / N, i1 = e - i2 * N;
-        if (i1 > i2) continue;
-        const int i0 = i2 * (i2 + 1) / 2 + i1;
-        sp1[i0] += w2 * ((double)(s.x[i1] * s.x[i2]) * ap2);
-        if (i1 == i2) sp2[i0] += w2 * ap2;
-#pragma unroll
-        for (int t = 0; t < 4; t++) {
-          const double2 u = sdb[t * N + i1], v = sdb[t * N + i2];
-          vbs[t * nchi + i0] += u.x * v.x + u.y * v.y;
-        }
-      }
       // exchange part of Chi_loc(2, .) for the antiparallel pairs, by the
       // thread that accumulated the pair: s_i1 s_i2 psi(x')
       if (lane < np) {
@@ -515,11 +483,6 @@ __global__ __launch_bounds__(MNT) void k_meas(LocalArgs a) {
           const double2 c = psip[u * np + lane];
           sp2[i2 * (i2 + 1) / 2 + i1] += w2 * (sg * (p.x * c.x + p.y * c.y));
         }
-      }
-      if (tid == 0) {
-        const double ap = cmod(p);
-        cpart[6 * nchi] += p.x / ap;
-        cpart[6 * nchi + 1] += p.y / ap;
       }
     }
     if (tid == 0 && a.status) a.status[cfg] = WS_OK;

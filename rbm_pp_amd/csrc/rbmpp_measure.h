@@ -101,3 +101,53 @@ __device__ void wg_orow(const DevModel& m, const S& s, double rw, double* ore, d
     if (cplx_rows) oim[k] = o.y * rw;
   }
 }
+
+// One configuration's share of the correlation run's sums
+// (Calc_Correlation/calc_phys_quantity.f90:318-338), by the whole workgroup:
+// SdotS per bond (calc_local_energy_correlation, :348-413) reweighted by rw,
+// Chi_vbs += Re(conj SdotS(i1,t) SdotS(i2,t)), Chi_sp(1, .) and the diagonal
+// of Chi_sp(2, .) (calc_local_correlation, :469-550) times rw^2, and wf_sgn.
+// cpart is the workgroup's slice (corr_len layout); the SdotS table takes
+// s.opp.  The exchange part of Chi_sp(2, .) (off-diagonal) is left to the
+// caller, which holds the pair amplitudes; every entry this adds to is owned by
+// one thread of the workgroup in every configuration.
+template <class S>
+__device__ void corr_accumulate(const DevModel& m, S& s, double2 p, double rw, int nb, double* cpart, int nchi) {
+  const int N = m.N, tid = threadIdx.x, nth = blockDim.x;
+  const double w2 = rw * rw, ap2 = p.x * p.x + p.y * p.y;
+  double2* sdb = (double2*)&s.opp[0][0];  // SdotS(i1, t) at sdb[t * N + i1]
+  for (int e = tid; e < 4 * N; e += nth) {
+    const int t = e / N, i1 = e - t * N;
+    sdb[e] = cscale(p, (double)(s.x[i1] * s.x[bond_partner(m.L, t, i1)]));
+  }
+  __syncthreads();
+  for (int b = tid; b < nb; b += nth) {
+    const int i1 = s.bi1[b], i2 = s.bi2[b];
+    int t = 0;
+    while (t < 3 && bond_partner(m.L, t, i1) != i2) t++;  // partners are distinct for L >= 3
+    sdb[t * N + i1] = cadd(sdb[t * N + i1], cscale(s.psib[b], t <= 1 ? -2.0 : 2.0));
+  }
+  __syncthreads();
+  for (int e = tid; e < 4 * N; e += nth) sdb[e] = cscale(sdb[e], rw);
+  __syncthreads();
+  double* sp1 = cpart;
+  double* sp2 = cpart + nchi;
+  double* vbs = cpart + 2 * nchi;
+  for (int e = tid; e < N * N; e += nth) {
+    const int i2 = e / N, i1 = e - i2 * N;
+    if (i1 > i2) continue;
+    const int i0 = i2 * (i2 + 1) / 2 + i1;
+    sp1[i0] += w2 * ((double)(s.x[i1] * s.x[i2]) * ap2);
+    if (i1 == i2) sp2[i0] += w2 * ap2;
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      const double2 u = sdb[t * N + i1], v = sdb[t * N + i2];
+      vbs[t * nchi + i0] += u.x * v.x + u.y * v.y;
+    }
+  }
+  if (tid == 0) {
+    const double ap = cmod(p);
+    cpart[6 * nchi] += p.x / ap;
+    cpart[6 * nchi + 1] += p.y / ap;
+  }
+}

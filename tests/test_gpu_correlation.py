@@ -1,10 +1,12 @@
 """The correlation run on the GPU (rbmpp_measure_correlation: Chi_sp, Chi_vbs,
 wf_sgn of Calc_Correlation/calc_phys_quantity.f90:264-550, normalised as
 Calc_Correlation/main.f90:543-581) against the oracle on the same samples.
-The GPU forms the pair ratios from Y = G InvM^T and Z = G Y^T, the oracle
-(pinned to the reference's calc_local_correlation and to exact enumeration)
-from the rank-2 update of pfupdate_two.f90 per pair, so the bound is
-relative 1e-9 normwise, like the S comparisons."""
+The GPU forms the pair ratios in block form (default normal projection) from
+Yh = Gdd F^-1, Xh = F^-1 Guu and W = Yh Guu, in skew form (the reference's
+exact weights, transverse weights) from Y = G InvM^T and Z = G Y^T; the
+oracle (pinned to the reference's calc_local_correlation and to exact
+enumeration) from the rank-2 update of pfupdate_two.f90 per pair, so the bound
+is relative 1e-9 normwise, like the S comparisons."""
 import numpy as np
 import pytest
 
@@ -45,12 +47,18 @@ def _check(got, ref, n):
 
 
 @pytest.mark.parametrize("L,ar,ac,n", [(4, 2, 1, 24), (6, 2, 0, 16), (6, 1, 1, 12), (10, 4, 0, 6)])
-def test_correlation_matches_oracle(oracle_lib, L, ar, ac, n):
+@pytest.mark.parametrize("proj", ["block", "exact"])
+def test_correlation_matches_oracle(oracle_lib, L, ar, ac, n, proj):
+    """proj "block": the default normal projection (cos(pi/2) = 0, block-form
+    kernel); "exact": the reference's weights bit for bit (skew-form kernel).
+    The oracle keeps the reference's weights in both cases."""
     api = _api()
     spec = lt.j1j2_model(L, alphar=ar, alphac=ac, J2=0.5)
     p = lt.random_params(spec, seed_w=L, seed_f=L + 100, sigma_w=0.05)
     xs = _configs(spec.N, n, seed=500 + L)
     eng = api.Engine(spec, n_walkers=1, n_samples=n, params=p)
+    if proj == "exact":
+        eng.set_projection(api.PROJ_NORMAL_EXACT)
     eng.set_samples(xs)
     shift = 1e-3
     got = eng.measure_correlation(shift=shift)
