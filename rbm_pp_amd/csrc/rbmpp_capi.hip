@@ -80,7 +80,7 @@ struct rbmpp_handle {
   // and the O-row operator it refines against (null: the assembled matrix)
   const double *sr_graph_orows = nullptr, *sr_graph_kappa = nullptr;
   int sr_graph_rows = 0;
-  double* sr_graph_t = nullptr;
+  double* sr_graph_part = nullptr;
   double* d_op = nullptr;  // scratch of the O-row operator
   int sr_pcg_iters = 0;    // conjugate-gradient iterations of the last SR step (0: FP64 S)
   double sr_boost = 0;     // extra diagonal its preconditioner needed
@@ -804,10 +804,11 @@ static int build_s(rbmpp_handle* h) {
   HIPCHK(h, hipMemcpyAsync(d_obar, h->m_o.data(), sizeof(double) * nv, hipMemcpyHostToDevice, h->st));
   launch_row_kappa(n, h->real_only ? 0 : 1, h->d_mpsi, h->d_rw, h->d_kappa, h->st);
   tic(h);
-  launch_center_pack(pl, rows, nv, h->d_orows, h->d_kappa, d_obar, h->d_packed, h->st);
+  // a mixed-precision S gets its FP64 diagonal from the packing pass's column sums
+  double* dpart = prec != RBMPP_S_FP64 ? h->d_op : nullptr;
+  launch_center_pack(pl, rows, nv, h->d_orows, h->d_kappa, d_obar, h->d_packed, dpart, h->st);
   launch_syrk_packed(pl, nv, h->d_packed, h->d_part, 1.0 / h->m_norm, h->d_S, h->st);
-  if (prec != RBMPP_S_FP64)
-    launch_s_diag_fp64(rows, nv, h->d_orows, h->d_kappa, d_obar, 1.0 / h->m_norm, h->d_op, h->d_S, h->st);
+  if (dpart) launch_s_diag_finish(nv, 1.0 / h->m_norm, dpart, h->d_S, h->st);
   HIPCHK(h, hipGetLastError());
   h->t[3] = toc(h);
   h->s_valid = 1;
@@ -855,7 +856,7 @@ int rbmpp_sr_step(rbmpp_handle* h, const rbmpp_sr_config* c, double* gamma, rbmp
     const int rows = h->real_only ? (int)h->n_meas : 2 * (int)h->n_meas;
     if ((rc = ensure_op(h, rows))) return rc;
     p.orows = h->d_orows; p.kappa = h->d_kappa; p.obar = h->d_vec + 4 * nv;
-    p.rows = rows; p.t = h->d_op; p.part = h->d_op + rows;
+    p.rows = rows; p.part = h->d_op;
   }
   const double inv_norm = 1.0 / h->m_norm;
   HIPCHK(h, hipMemcpyAsync(d_g, g.data(), sizeof(double) * nv, hipMemcpyHostToDevice, h->st));
@@ -871,7 +872,7 @@ int rbmpp_sr_step(rbmpp_handle* h, const rbmpp_sr_config* c, double* gamma, rbmp
       return 0;
     }
     if (!h->sr_exec || h->sr_graph_iters != c->refine_iters || h->sr_graph_orows != p.orows ||
-        h->sr_graph_kappa != p.kappa || h->sr_graph_rows != p.rows || h->sr_graph_t != p.t) {
+        h->sr_graph_kappa != p.kappa || h->sr_graph_rows != p.rows || h->sr_graph_part != p.part) {
       if (h->sr_exec) { hipGraphExecDestroy(h->sr_exec); h->sr_exec = nullptr; }
       if (h->sr_graph) { hipGraphDestroy(h->sr_graph); h->sr_graph = nullptr; }
       HIPCHK(h, hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
@@ -884,7 +885,7 @@ int rbmpp_sr_step(rbmpp_handle* h, const rbmpp_sr_config* c, double* gamma, rbmp
       h->sr_graph_orows = p.orows;
       h->sr_graph_kappa = p.kappa;
       h->sr_graph_rows = p.rows;
-      h->sr_graph_t = p.t;
+      h->sr_graph_part = p.part;
     }
     HIPCHK(h, hipGraphLaunch(h->sr_exec, h->st));
     return 0;

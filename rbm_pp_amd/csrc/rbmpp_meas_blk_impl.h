@@ -15,15 +15,16 @@
 //    to the workgroup's global scratch;
 //  * then copy by copy F^-1 is staged in LDS for the exchange ratio of every
 //    antiparallel bond (calculate_new_PfM_two_child, pfupdate_two.f90:27-88,
-//    as det F'/det F for F' = F + e_ua r^T + c e_db^T) and for the
+//    as det F'/det F for F' = F + e_u r^T + c e_d^T) and for the
 //    pair-product log-derivatives (slater.f90:136-167).
 // The energy and O rows follow calc_phys_quantity.f90:20-246 as in the skew
 // estimator (rbmpp_meas_impl.h).
 //
-// The correlation run (k_meas_blk<true>, Calc_Correlation/calc_phys_quantity.f90:
-// 264-550) also needs the exchange ratio of every antiparallel pair, (N/2)^2
-// of them: per copy three (N/2)^3 products (corr_pairs_blk) give each ratio
-// from a few entries.  The sums of Chi_sp, Chi_vbs and wf_sgn are kept per
+// The exchange ratios come from Yh = Gdd F^-1, one (N/2)^3 product per copy,
+// and two (N/2)-long dot products per exchange (blk_ratio).  The correlation
+// run (k_meas_blk<true>, Calc_Correlation/calc_phys_quantity.f90:264-550)
+// needs the ratio of every antiparallel pair, (N/2)^2 of them, from the same
+// Yh (corr_pairs_blk).  The sums of Chi_sp, Chi_vbs and wf_sgn are kept per
 // workgroup as in the skew kernel.
 #include "rbmpp_block.h"
 #include "rbmpp_kernels.h"
@@ -37,7 +38,6 @@ constexpr int BNT = 1024;                // threads per workgroup
 constexpr int BNW = BNT / RBMPP_WAVE;    // waves: one per projection copy
 constexpr int NMX = RBMPP_NLDS;          // largest N
 constexpr int HMX = rbmpp_block::HMX;    // largest np = N/2
-constexpr int BT = 4;                    // bonds per wave and pass
 
 struct BlkSmem {
   double binv[HMX * HMX];        // F^-1 of the current copy: B[d][u] at d * np + u
@@ -45,10 +45,9 @@ struct BlkSmem {
   union {
     struct {
       double stage[BNW][8 * HMX];  // write-back stage of each wave's elimination
-      double vr[BNW][BT][HMX];     // r vectors of the bonds in flight
     } e;
-    // correlation run, after a copy's bond pass: Gdd, Guu and Yh = Gdd F^-1
-    // of the copy (corr_pairs_blk)
+    // per copy, after the elimination: Gdd and Yh = Gdd F^-1 (copy_yh), and
+    // Guu for the correlation run's pairs (corr_pairs_blk)
     struct {
       double gdd[HMX * HMX], guu[HMX * HMX], yh[HMX * HMX];
     } c;
@@ -86,31 +85,40 @@ __device__ __attribute__((noinline)) double2 eloc_of(const DevModel& m, const Bl
   return eloc_from_bonds(m, s, p);
 }
 
-// Exchange ratio of every antiparallel pair of copy q (up electron u at site
-// su, down electron d at site sd) and psi(x') of the copy, added to
-// psip[d * np + u]; called by the whole workgroup with F^-1 in s.binv.  The
-// exchange moves u to sd and d to su: F' = F + e_u r^T + c e_d^T with
+// Exchange ratios in block form.  An exchange of the up electron u (site su)
+// and the down electron d (site sd) moves u to sd and d to su:
+// F' = F + e_u r^T + c e_d^T with
 //   r = R_sd - R_su + delta e_d,  R_s(d') = Slater[s][s_d' + N] (the row of F
 //       an up electron at s has), delta = Slater[sd][su+N] - Slater[sd][sd+N],
 //   c = C_su - C_sd - eps e_u,    C_s(u') = Slater[s_u'][s + N] (the column
 //       a down electron at s has), eps = Slater[su][su+N] - Slater[su][sd+N].
 // With B = F^-1, Gdd(d, d') = R_sd(d'), Guu(u', u) = C_su(u'), Yh = Gdd B,
-// Xh = B Guu, W = Yh Guu, and R_su B = e_u, B C_sd = e_d, Yh C_sd = Gdd(d, d):
+// Xh(d, u) = B[d] . Guu[., u], W(d, u) = Yh[d] . Guu[., u], and R_su B = e_u,
+// B C_sd = e_d, Yh C_sd = Gdd(d, d):
 //   1 + y_u    = Yh(d, u) + delta B(d, u)              (y = r^T B)
 //   1 + B[d].c = Xh(d, u) - eps B(d, u)
 //   y.c        = W(d, u) - Gdd(d, d) - delta + delta Xh(d, u) - eps (1 + y_u)
 // and det F'/det F = (1 + y_u)(1 + B[d].c) - (y.c) B(d, u), the ratio of
-// pfupdate_two.f90:27-88 for the block structure: three (N/2)^3 products per
-// copy instead of a pass over F^-1 per pair.  Thread e owns pairs
-// e, e + 1024, ... in every copy, so psip needs no synchronisation.
-__device__ __attribute__((noinline)) void corr_pairs_blk(const DevModel& m, BlkSmem& s, const double* S, int q,
-                                                         double pfq, double2* psip) {
+// pfupdate_two.f90:27-88 for the block structure: one (N/2)^3 product per
+// copy and two (N/2)-long dot products per exchange instead of a pass over
+// F^-1 per exchange.
+__device__ __forceinline__ double blk_ratio(double yh, double xh, double w, double b, double delta, double eps,
+                                            double gdd) {
+  const double a1 = yh + delta * b;
+  const double a2 = xh - eps * b;
+  const double yc = w - gdd - delta + delta * xh - eps * a1;
+  return a1 * a2 - yc * b;
+}
+
+// Gdd and Yh = Gdd F^-1 of copy S (F^-1 in s.binv), and with guu also Guu,
+// by the whole workgroup; ends in step.
+__device__ __attribute__((noinline)) void copy_yh(const DevModel& m, BlkSmem& s, const double* S, bool guu) {
   const int N = m.N, np = m.npair, two = 2 * N, tid = threadIdx.x;
   const int npp = np * np;
   for (int e = tid; e < npp; e += BNT) {
     const int r = e / np, c = e - r * np;
     s.u.c.gdd[e] = S[(size_t)s.eidx[np + r] * two + s.eidx[np + c] + N];
-    s.u.c.guu[e] = S[(size_t)s.eidx[r] * two + s.eidx[c] + N];
+    if (guu) s.u.c.guu[e] = S[(size_t)s.eidx[r] * two + s.eidx[c] + N];
   }
   __syncthreads();
   for (int e = tid; e < npp; e += BNT) {
@@ -121,6 +129,63 @@ __device__ __attribute__((noinline)) void corr_pairs_blk(const DevModel& m, BlkS
     s.u.c.yh[e] = t;
   }
   __syncthreads();
+}
+
+// Exchange ratios of the antiparallel bonds b < nb of the copy into s.rat,
+// four bonds per wave and pass (lanes over the up electrons u'): Xh and W of
+// the four by two four-way wave sums, so that the four bonds' gathers and
+// reductions overlap; Guu[., u] is read from row su + N of Slater_Elm (with
+// the block weights Slater[su+N][su'] = -Slater[su'][su+N] exactly).  Needs Yh.
+__device__ __attribute__((noinline)) void bond_ratios(const DevModel& m, BlkSmem& s, const double* S, int nb) {
+  constexpr int BT = 4;
+  const int N = m.N, np = m.npair, two = 2 * N;
+  const int lane = threadIdx.x & (RBMPP_WAVE - 1), wid = threadIdx.x >> 6;
+  const bool live = lane < np;
+  const int su_l = live ? s.eidx[lane] : 0;  // site of up electron `lane`
+  for (int p = wid; BT * p < nb; p += BNW) {
+    int bb[BT], u[BT], d[BT], su[BT], sd[BT];
+    double g[BT], bv[BT], yv[BT];
+#pragma unroll
+    for (int t = 0; t < BT; t++) {
+      bb[t] = min(BT * p + t, nb - 1);  // a missing bond repeats the last one (not written)
+      const int i1 = s.bi1[bb[t]], i2 = s.bi2[bb[t]];
+      const int msa = s.ecfg[i1], msb = s.ecfg[i2];
+      const bool aup = msa < np;
+      u[t] = __builtin_amdgcn_readfirstlane(aup ? msa : msb);
+      d[t] = __builtin_amdgcn_readfirstlane((aup ? msb : msa) - np);
+      su[t] = __builtin_amdgcn_readfirstlane(aup ? i1 : i2);
+      sd[t] = __builtin_amdgcn_readfirstlane(aup ? i2 : i1);
+      g[t] = live ? -S[(size_t)(su[t] + N) * two + su_l] : 0.0;
+      bv[t] = live ? s.binv[d[t] * np + lane] : 0.0;
+      yv[t] = live ? s.u.c.yh[d[t] * np + lane] : 0.0;
+    }
+    // lanes 15, 47, 31, 63 hold bonds 0-3
+    const double xv = wave_sum4(bv[0] * g[0], bv[1] * g[1], bv[2] * g[2], bv[3] * g[3]);
+    const double wv = wave_sum4(yv[0] * g[0], yv[1] * g[1], yv[2] * g[2], yv[3] * g[3]);
+    constexpr int SUMLANE[BT] = {15, 47, 31, 63};
+#pragma unroll
+    for (int t = 0; t < BT; t++) {
+      if (BT * p + t >= nb) break;  // wave-uniform
+      const double xh = readlane_d(xv, SUMLANE[t]), w = readlane_d(wv, SUMLANE[t]);
+      const double bdu = readlane_d(bv[t], u[t]), ydu = readlane_d(yv[t], u[t]);
+      if (lane == t) {
+        const double* Su = S + (size_t)su[t] * two;
+        const double* Sd = S + (size_t)sd[t] * two;
+        s.rat[bb[t]] = blk_ratio(ydu, xh, w, bdu, Sd[su[t] + N] - Sd[sd[t] + N], Su[su[t] + N] - Su[sd[t] + N],
+                                 Sd[sd[t] + N]);
+      }
+    }
+  }
+}
+
+// Exchange ratio of every antiparallel pair of the copy and psi(x') of the
+// copy, added to psip[d * np + u]; called by the whole workgroup with Gdd,
+// Guu and Yh of the copy in place (copy_yh).  Thread e owns pairs e,
+// e + 1024, ... in every copy, so psip needs no synchronisation.
+__device__ __attribute__((noinline)) void corr_pairs_blk(const DevModel& m, BlkSmem& s, const double* S, int q,
+                                                         double pfq, double2* psip) {
+  const int N = m.N, np = m.npair, two = 2 * N, tid = threadIdx.x;
+  const int npp = np * np;
   const int g = q >> 1;
   for (int e = tid; e < npp; e += BNT) {
     const int d = e / np, u = e - d * np;
@@ -135,13 +200,7 @@ __device__ __attribute__((noinline)) void corr_pairs_blk(const DevModel& m, BlkS
     const int su = s.eidx[u], sd = s.eidx[np + d];
     const double* Su = S + (size_t)su * two;
     const double* Sd = S + (size_t)sd * two;
-    const double delta = Sd[su + N] - Sd[sd + N];
-    const double eps = Su[su + N] - Su[sd + N];
-    const double b = bd[u];
-    const double a1 = yd[u] + delta * b;
-    const double a2 = xh - eps * b;
-    const double yc = w - Sd[sd + N] - delta + delta * xh - eps * a1;
-    const double rat = a1 * a2 - yc * b;
+    const double rat = blk_ratio(yd[u], xh, w, bd[u], Sd[su + N] - Sd[sd + N], Su[su + N] - Su[sd + N], Sd[sd + N]);
     const int i1 = min(su, sd), i2 = max(su, sd);
     int sfac = -1;  // make_sample.f90:195-197
     if (m.subB[i1]) sfac = -sfac;
@@ -268,67 +327,10 @@ __global__ __launch_bounds__(BNT) void k_meas_blk(LocalArgs a) {
           }
         }
       }
-      // exchange ratios, BT bonds per wave and pass: the up electron ua moves
-      // from s1 to s2, the down electron db from s2 to s1; with B = F^-1,
-      // y = r^T B: det F'/det F = (1 + y_ua)(1 + B[db].c) - (y.c) B[db][ua]
-      for (int p = wid; BT * p < nb; p += BNW) {
-        int bb[BT], ua[BT], db[BT];
-        double cv[BT];
-#pragma unroll
-        for (int t = 0; t < BT; t++) {
-          bb[t] = min(BT * p + t, nb - 1);  // a missing bond repeats the last one (not written)
-          const int i1 = s.bi1[bb[t]], i2 = s.bi2[bb[t]];
-          const int msa = s.ecfg[i1], msb = s.ecfg[i2];
-          const bool aup = msa < np;
-          ua[t] = __builtin_amdgcn_readfirstlane(aup ? msa : msb);
-          db[t] = __builtin_amdgcn_readfirstlane((aup ? msb : msa) - np);
-          const int s1 = aup ? i1 : i2, s2 = aup ? i2 : i1;
-          const double* S1 = S + (size_t)s1 * 2 * N;
-          const double* S2 = S + (size_t)s2 * 2 * N;
-          // c(u) from rows s1+N and s2+N: the off-diagonal blocks of the
-          // block form are exact negations of each other (one weight is zero)
-          const double* T1 = S + (size_t)(s1 + N) * 2 * N;
-          const double* T2 = S + (size_t)(s2 + N) * 2 * N;
-          double rv = 0.0;
-          cv[t] = 0.0;
-          if (lane < np) {
-            const int sd = s.eidx[np + lane], su = s.eidx[lane];
-            rv = lane == db[t] ? S2[s1 + N] - S1[s2 + N] : S2[sd + N] - S1[sd + N];
-            cv[t] = lane == ua[t] ? 0.0 : T2[su] - T1[su];
-          }
-          if (lane < np) s.u.e.vr[wid][t][lane] = rv;
-        }
-        wave_sync();
-        double y[BT];
-#pragma unroll
-        for (int t = 0; t < BT; t++) y[t] = 0.0;
-        if (lane < np) {
-#pragma unroll 4
-          for (int d = 0; d < np; d++) {
-            const double bv = s.binv[d * np + lane];
-#pragma unroll
-            for (int t = 0; t < BT; t++) y[t] = fma(s.u.e.vr[wid][t][d], bv, y[t]);
-          }
-        }
-        // B[db].c and y.c of the four bonds by two four-way wave sums
-        // (lanes 15, 47, 31, 63 hold bonds 0-3)
-        static_assert(BT == 4, "the four-way wave sums take four bonds");
-        double brow[BT];
-#pragma unroll
-        for (int t = 0; t < BT; t++) brow[t] = lane < np ? s.binv[db[t] * np + lane] : 0.0;
-        const double dv = wave_sum4(brow[0] * cv[0], brow[1] * cv[1], brow[2] * cv[2], brow[3] * cv[3]);
-        const double ev = wave_sum4(y[0] * cv[0], y[1] * cv[1], y[2] * cv[2], y[3] * cv[3]);
-        constexpr int SUMLANE[BT] = {15, 47, 31, 63};
-#pragma unroll
-        for (int t = 0; t < BT; t++) {
-          if (BT * p + t >= nb) break;  // wave-uniform
-          const double gam = readlane_d(y[t], ua[t]);
-          const double bet = readlane_d(brow[t], ua[t]);
-          const double del = readlane_d(dv, SUMLANE[t]);
-          const double eps = readlane_d(ev, SUMLANE[t]);
-          if (lane == 0) s.rat[bb[t]] = (1.0 + gam) * (1.0 + del) - eps * bet;
-        }
-        wave_sync();  // vr is rewritten by the next pass
+      // exchange ratios of the antiparallel bonds (blk_ratio)
+      if (nb > 0 || CORR) {
+        copy_yh(m, s, S, CORR);
+        bond_ratios(m, s, S, nb);
       }
       __syncthreads();
       // psi(x') of this copy for every bond, one thread per bond
@@ -342,10 +344,7 @@ __global__ __launch_bounds__(BNT) void k_meas_blk(LocalArgs a) {
         s.psib[b] = cadd(s.psib[b], cscale(c, pfn));
         if (custom && a.ex_pf) a.ex_pf[(size_t)cfg * C + q] = pfn;
       }
-      if constexpr (CORR) {
-        __syncthreads();  // the bond pass is done with the r vectors (s.u.e)
-        corr_pairs_blk(m, s, S, q, pfq, psip);
-      }
+      if constexpr (CORR) corr_pairs_blk(m, s, S, q, pfq, psip);
     }
     __syncthreads();
     // amplitude, local energy, reweighting

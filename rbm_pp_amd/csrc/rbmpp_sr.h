@@ -28,8 +28,14 @@ struct SyrkPlan {
 // tile <= 0: chosen from precision and nv; nsplit <= 0: about two resident
 // workgroups per CU
 SyrkPlan syrk_plan(int precision, int nrows, int nv, int cus, int tile = 0, int nsplit = 0);
+// row chunks of the passes over the O rows whose column sums are kept per
+// chunk and added in a fixed order: [SR_ROW_CHUNKS][nv + 1] partials
+constexpr int SR_ROW_CHUNKS = 256;
+// centre and pack the O rows for the products; for FP32 and split BF16 x 3
+// also the FP64 column sums of squares per chunk into dpart
+// ([SR_ROW_CHUNKS][nv + 1], finished by launch_s_diag_finish)
 void launch_center_pack(const SyrkPlan& pl, int nrows, int nv, const double* orows, const double* kappa,
-                        const double* obar, unsigned char* packed, hipStream_t st);
+                        const double* obar, unsigned char* packed, double* dpart, hipStream_t st);
 void launch_syrk_packed(const SyrkPlan& pl, int nv, const unsigned char* packed, double* part, double inv_norm,
                         double* S, hipStream_t st);
 void launch_gemv(int n, const double* A, const double* x, double* y, hipStream_t st);
@@ -64,8 +70,7 @@ struct SrPipeline {
   const double* kappa = nullptr;  // [rows]
   const double* obar = nullptr;   // [nv]
   int rows = 0;
-  double* t = nullptr;            // orow_op_scratch(rows, nv) doubles
-  double* part = nullptr;         // (inside t's allocation)
+  double* part = nullptr;         // orow_op_scratch(rows, nv) doubles: per-chunk column sums
   double *z = nullptr, *pd = nullptr, *tmp = nullptr;  // [nv] conjugate-gradient vectors, solve scratch
 };
 // With orows set, refine_iters counts conjugate-gradient iterations on the
@@ -78,7 +83,6 @@ void launch_pcg_iters(const SrPipeline& p, int k0, int k1, hipStream_t st);
 void launch_pcg_final(const SrPipeline& p, hipStream_t st);
 // doubles of scratch of the O-row operator
 size_t orow_op_scratch(int rows, int nv);
-// S_jj = inv_norm sum_k (O_kj - kappa_k obar_j)^2 in FP64 over the diagonal
-// of S (part: orow_op_scratch(rows, nv) doubles of scratch)
-void launch_s_diag_fp64(int rows, int nv, const double* orows, const double* kappa, const double* obar,
-                        double inv_norm, double* part, double* S, hipStream_t st);
+// S_jj = inv_norm sum_c part[c][j] over the diagonal of S (the FP64 column
+// sums of squares of launch_center_pack, added in chunk order)
+void launch_s_diag_finish(int nv, double inv_norm, const double* part, double* S, hipStream_t st);

@@ -483,11 +483,9 @@ void launch_chol_solve(int n, const double* L, const double* dinv, double* x, do
 
 // ------------------------------------------- the FP64 operator of the O rows
 // y = (D S D + shift I) x with S = inv_norm C^T C and C = O - kappa obar^T,
-// the centred rows taken from the FP64 O rows themselves (two passes over O),
-// so that a refinement against it converges to the FP64 system's solution
-// whatever the precision of the assembled S.  sc[7] = inv_norm, sc[8] holds
-// obar . D x.
-#define OP_CHUNKS 64
+// the centred rows taken from the FP64 O rows themselves, so that a
+// refinement against it converges to the FP64 system's solution whatever the
+// precision of the assembled S.  sc[7] = inv_norm, sc[8] holds obar . D x.
 
 __global__ __launch_bounds__(1024) void k_op_obar(int nv, int nvr, double* __restrict__ sc,
                                                   const double* __restrict__ obar, const double* __restrict__ x) {
@@ -505,55 +503,103 @@ __global__ __launch_bounds__(1024) void k_op_obar(int nv, int nvr, double* __res
   if (tid == 0) sc[8] = rs[0];
 }
 
-// t_k = C_k . D x, one wave per row
-__global__ void k_op_rows(int rows, int nv, int nvr, const double* __restrict__ sc, const double* __restrict__ O,
-                          const double* __restrict__ kappa, const double* __restrict__ x, double* __restrict__ t) {
-  const int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (row >= rows) return;
-  const double scal = sc[0];
-  const double* Or = O + (size_t)row * nv;
-  double a = 0.0;
-  for (int j = lane; j < nv; j += 64) a = fma(Or[j], j >= nvr ? x[j] * scal : x[j], a);
-  a = wave_sum(a);
-  if (lane == 0) t[row] = a - kappa[row] * sc[8];
-}
-
-// column partial sums over a chunk of rows: part[c][j] = sum_k O[k][j] t_k
-// (j < nv) and part[c][nv] = sum_k kappa_k t_k, each in a fixed order
-__global__ __launch_bounds__(256) void k_op_cols(int rows, int nv, const double* __restrict__ O,
-                                                 const double* __restrict__ kappa, const double* __restrict__ t,
-                                                 double* __restrict__ part) {
-  __shared__ double rs[256];
-  const int c = blockIdx.y, j = blockIdx.x * 256 + threadIdx.x;
-  const int per = (rows + OP_CHUNKS - 1) / OP_CHUNKS;
-  const int r0 = c * per, r1 = min(rows, r0 + per);
-  double* pc = part + (size_t)c * (nv + 1);
-  if (j < nv) {
-    double a = 0.0;
-    int k = r0;
-    for (; k + 4 <= r1; k += 4) {
-      const double o0 = O[(size_t)k * nv + j], o1 = O[(size_t)(k + 1) * nv + j];
-      const double o2 = O[(size_t)(k + 2) * nv + j], o3 = O[(size_t)(k + 3) * nv + j];
-      a = fma(o0, t[k], a);
-      a = fma(o1, t[k + 1], a);
-      a = fma(o2, t[k + 2], a);
-      a = fma(o3, t[k + 3], a);
-    }
-    for (; k < r1; k++) a = fma(O[(size_t)k * nv + j], t[k], a);
-    pc[j] = a;
+// One pass over the O rows: workgroup c takes rows [c per, (c + 1) per) in
+// batches of RB and forms t_k = O_k . D x - kappa_k (obar . D x) from the rows
+// held in registers (thread tid holds columns tid + 1024 i, i < NPT), then
+// accumulates part[c][j] = sum_k O_kj t_k and part[c][nv] = sum_k kappa_k t_k
+// from the same registers, each in a fixed order.  With PF the next batch's
+// rows are loaded while the current batch's dot products are reduced (where
+// the registers allow it).
+template <int NPT, int RB, bool PF>
+__global__ __launch_bounds__(1024) void k_op_pass(int rows, int nv, int nvr, const double* __restrict__ sc,
+                                                  const double* __restrict__ O, const double* __restrict__ kappa,
+                                                  const double* __restrict__ x, double* __restrict__ part) {
+  static_assert(RB == 1 || RB == 2 || RB == 4, "rows per batch: one wave sum, or a two- or four-way one");
+  __shared__ double red[2][RB][16];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const double scal = sc[0], od = sc[8];
+  double dv[NPT], acc[NPT];
+#pragma unroll
+  for (int i = 0; i < NPT; i++) {
+    const int j = tid + 1024 * i;
+    dv[i] = j < nv ? (j >= nvr ? x[j] * scal : x[j]) : 0.0;
+    acc[i] = 0.0;
   }
-  if (blockIdx.x == 0) {
-    double a = 0.0;
-    for (int k = r0 + (int)threadIdx.x; k < r1; k += 256) a = fma(kappa[k], t[k], a);
-    rs[threadIdx.x] = a;
+  const int per = (rows + gridDim.x - 1) / gridDim.x;
+  const int r0 = blockIdx.x * per, r1 = min(rows, r0 + per);
+  double o[RB][NPT];
+  double on[PF ? RB : 1][PF ? NPT : 1];
+  auto load = [&](int k0, double (&dst)[RB][NPT]) {
+#pragma unroll
+    for (int r = 0; r < RB; r++) {
+      const bool live = k0 + r < r1;
+      const double* Or = O + (size_t)(live ? k0 + r : 0) * nv;
+#pragma unroll
+      for (int i = 0; i < NPT; i++) {
+        const int j = tid + 1024 * i;
+        dst[r][i] = (live && j < nv) ? Or[j] : 0.0;
+      }
+    }
+  };
+  double kt = 0.0;
+  int buf = 0;
+  if constexpr (PF) {
+    if (r0 < r1) load(r0, on);
+  }
+  for (int k0 = r0; k0 < r1; k0 += RB) {
+    if constexpr (PF) {
+#pragma unroll
+      for (int r = 0; r < RB; r++)
+#pragma unroll
+        for (int i = 0; i < NPT; i++) o[r][i] = on[r][i];
+      if (k0 + RB < r1) load(k0 + RB, on);
+    } else {
+      load(k0, o);
+    }
+    double d[RB];
+#pragma unroll
+    for (int r = 0; r < RB; r++) {
+      d[r] = 0.0;
+#pragma unroll
+      for (int i = 0; i < NPT; i++) d[r] = fma(o[r][i], dv[i], d[r]);
+    }
+    if constexpr (RB == 4) {
+      const double v = wave_sum4(d[0], d[1], d[2], d[3]);  // lanes 15, 47, 31, 63
+      if (lane == 15) red[buf][0][wid] = v;
+      if (lane == 47) red[buf][1][wid] = v;
+      if (lane == 31) red[buf][2][wid] = v;
+      if (lane == 63) red[buf][3][wid] = v;
+    } else if constexpr (RB == 2) {
+      const double v = wave_sum2(d[0], d[1]);  // lanes 31, 63
+      if (lane == 31) red[buf][0][wid] = v;
+      if (lane == 63) red[buf][1][wid] = v;
+    } else {
+      const double v = wave_sum_to63(d[0]);
+      if (lane == 63) red[buf][0][wid] = v;
+    }
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-      if ((int)threadIdx.x < o) rs[threadIdx.x] += rs[threadIdx.x + o];
-      __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RB; r++) {
+      double s = 0.0;
+#pragma unroll
+      for (int w = 0; w < 16; w++) s += red[buf][r][w];
+      const bool live = k0 + r < r1;
+      const double tk = live ? s - kappa[k0 + r] * od : 0.0;
+#pragma unroll
+      for (int i = 0; i < NPT; i++) acc[i] = fma(o[r][i], tk, acc[i]);
+      if (tid == 0 && live) kt = fma(kappa[k0 + r], tk, kt);
     }
-    if (threadIdx.x == 0) pc[nv] = rs[0];
+    // red[buf] is written again two batches on, after every thread has
+    // passed the next batch's barrier (and so has read it)
+    buf ^= 1;
   }
+  double* pc = part + (size_t)blockIdx.x * (nv + 1);
+#pragma unroll
+  for (int i = 0; i < NPT; i++) {
+    const int j = tid + 1024 * i;
+    if (j < nv) pc[j] = acc[i];
+  }
+  if (tid == 0) pc[nv] = kt;
 }
 
 // y = D inv_norm (sum_c part_c - obar sum_c part_c[nv]) + shift x, and with
@@ -564,7 +610,7 @@ __global__ void k_op_finish(int nv, int nvr, const double* __restrict__ sc, cons
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nv) return;
   double w = 0.0, kt = 0.0;
-  for (int c = 0; c < OP_CHUNKS; c++) {
+  for (int c = 0; c < SR_ROW_CHUNKS; c++) {
     w += part[(size_t)c * (nv + 1) + j];
     kt += part[(size_t)c * (nv + 1) + nv];
   }
@@ -575,14 +621,25 @@ __global__ void k_op_finish(int nv, int nvr, const double* __restrict__ sc, cons
   if (r) r[j] = (j >= nvr ? g[j] * scal : g[j]) - yj;
 }
 
+template <int NPT, int RB, bool PF>
+static void launch_op_pass(const SrPipeline& p, const double* v, hipStream_t st) {
+  hipLaunchKernelGGL((k_op_pass<NPT, RB, PF>), dim3(SR_ROW_CHUNKS), dim3(1024), 0, st, p.rows, p.nv, p.nvr,
+                     (const double*)p.sc, p.orows, p.kappa, v, p.part);
+}
+
 // y = A v for the operator of the O rows; with r: r = D g - y
 static void launch_orow_op(const SrPipeline& p, const double* v, double* y, double* r, hipStream_t st) {
   const int n = p.nv;
   hipLaunchKernelGGL(k_op_obar, dim3(1), dim3(1024), 0, st, n, p.nvr, p.sc, p.obar, v);
-  hipLaunchKernelGGL(k_op_rows, dim3((p.rows + 3) / 4), dim3(256), 0, st, p.rows, n, p.nvr, (const double*)p.sc,
-                     p.orows, p.kappa, v, p.t);
-  hipLaunchKernelGGL(k_op_cols, dim3((n + 255) / 256, OP_CHUNKS), dim3(256), 0, st, p.rows, n, p.orows, p.kappa,
-                     (const double*)p.t, p.part);
+  const int npt = (n + 1023) / 1024;
+  if (npt <= 1) launch_op_pass<1, 4, true>(p, v, st);
+  else if (npt <= 2) launch_op_pass<2, 4, true>(p, v, st);
+  else if (npt <= 4) launch_op_pass<4, 2, true>(p, v, st);
+  else if (npt <= 6) launch_op_pass<6, 2, false>(p, v, st);
+  else if (npt <= 8) launch_op_pass<8, 2, false>(p, v, st);
+  else if (npt <= 12) launch_op_pass<12, 1, false>(p, v, st);
+  else if (npt <= 16) launch_op_pass<16, 1, false>(p, v, st);
+  else launch_op_pass<20, 1, false>(p, v, st);
   hipLaunchKernelGGL(k_op_finish, dim3((n + 255) / 256), dim3(256), 0, st, n, p.nvr, (const double*)p.sc, p.obar,
                      (const double*)p.part, v, p.g, y, r);
 }
@@ -672,42 +729,23 @@ void launch_pcg_final(const SrPipeline& p, hipStream_t st) {
   hipLaunchKernelGGL(k_sr_final, dim3(1), dim3(1024), 0, st, n, p.nvr, p.sc, p.x, p.ax, p.g);
 }
 
-size_t orow_op_scratch(int rows, int nv) { return (size_t)rows + (size_t)OP_CHUNKS * (nv + 1); }
+size_t orow_op_scratch(int rows, int nv) { return (size_t)SR_ROW_CHUNKS * (nv + 1); }
 
-// The diagonal of S in FP64 from the O rows, S_jj = inv_norm sum_k C_kj^2
-// (column chunks as k_op_cols, then a fixed-order sum), written over the
-// diagonal of an S assembled from FP32 or split-BF16 products: the PP
-// scaling, the shift and the diagonal checks then see FP64 values, and the
-// operator of the O rows and the assembled matrix share D and the shift.
-__global__ __launch_bounds__(256) void k_diag_cols(int rows, int nv, const double* __restrict__ O,
-                                                   const double* __restrict__ kappa, const double* __restrict__ obar,
-                                                   double* __restrict__ part) {
-  const int c = blockIdx.y, j = blockIdx.x * 256 + threadIdx.x;
-  if (j >= nv) return;
-  const int per = (rows + OP_CHUNKS - 1) / OP_CHUNKS;
-  const int r0 = c * per, r1 = min(rows, r0 + per);
-  const double ob = obar[j];
-  double a = 0.0;
-  for (int k = r0; k < r1; k++) {
-    const double v = fma(-kappa[k], ob, O[(size_t)k * nv + j]);
-    a = fma(v, v, a);
-  }
-  part[(size_t)c * (nv + 1) + j] = a;
-}
-
+// The diagonal of S in FP64, S_jj = inv_norm sum_k C_kj^2, from the per-chunk
+// sums of launch_center_pack added in chunk order, written over the diagonal
+// of an S assembled from FP32 or split-BF16 products: the PP scaling, the
+// shift and the diagonal checks then see FP64 values, and the operator of the
+// O rows and the assembled matrix share D and the shift.
 __global__ void k_diag_finish(int nv, double inv_norm, const double* __restrict__ part, double* __restrict__ S) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nv) return;
   double a = 0.0;
-  for (int c = 0; c < OP_CHUNKS; c++) a += part[(size_t)c * (nv + 1) + j];
+  for (int c = 0; c < SR_ROW_CHUNKS; c++) a += part[(size_t)c * (nv + 1) + j];
   S[(size_t)j * nv + j] = a * inv_norm;
 }
 
-void launch_s_diag_fp64(int rows, int nv, const double* orows, const double* kappa, const double* obar,
-                        double inv_norm, double* part, double* S, hipStream_t st) {
-  hipLaunchKernelGGL(k_diag_cols, dim3((nv + 255) / 256, OP_CHUNKS), dim3(256), 0, st, rows, nv, orows, kappa, obar,
-                     part);
-  hipLaunchKernelGGL(k_diag_finish, dim3((nv + 255) / 256), dim3(256), 0, st, nv, inv_norm, (const double*)part, S);
+void launch_s_diag_finish(int nv, double inv_norm, const double* part, double* S, hipStream_t st) {
+  hipLaunchKernelGGL(k_diag_finish, dim3((nv + 255) / 256), dim3(256), 0, st, nv, inv_norm, part, S);
 }
 
 void launch_sr_pipeline(const SrPipeline& p, hipStream_t st) {

@@ -44,52 +44,84 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 __host__ __device__ constexpr int kblock_rows(int mode) { return mode == 0 ? 16 : 32; }
 
 // ------------------------------------------------------------ centre + pack
-// One workgroup per (K-block, 128-column chunk): rows are read coalesced
-// along the columns, centred in FP64 into LDS, then written out column by
-// column (128 contiguous bytes each).
-template <int MODE>
-__global__ __launch_bounds__(256) void k_center_pack(int nrows, int nv, int nvp, const double* __restrict__ orows,
+// Workgroup (chunk, column block) takes 128 columns of the K-blocks
+// [chunk * per, (chunk + 1) * per): rows are read coalesced along the
+// columns (the next block's into registers while the current one is
+// packed), centred in FP64 into LDS, then written out column by column (128
+// contiguous bytes each).  With DIAG (the mixed precisions) it also sums the
+// squares of the FP64 centred values per column over its K-blocks, in a
+// fixed order, into dpart[chunk * (nv + 1) + col]: the FP64 diagonal of S
+// without another pass over the O rows.
+template <int MODE, bool DIAG>
+__global__ __launch_bounds__(256) void k_center_pack(int nrows, int nv, int nvp, int nkb, int per,
+                                                     const double* __restrict__ orows,
                                                      const double* __restrict__ kappa,
-                                                     const double* __restrict__ obar, unsigned char* __restrict__ P) {
+                                                     const double* __restrict__ obar, unsigned char* __restrict__ P,
+                                                     double* __restrict__ dpart) {
   constexpr int KB = kblock_rows(MODE);
+  constexpr int NL = KB * TILE / 256;  // tile elements loaded per thread
   __shared__ double t[KB][TILE + 1];
-  const int kb = blockIdx.y, c0 = blockIdx.x * TILE;
+  __shared__ double dsum[2][TILE];
+  const int c0 = blockIdx.x * TILE, chunk = blockIdx.y;
   const int tid = threadIdx.x;
-  for (int e = tid; e < KB * TILE; e += 256) {
-    const int r = e / TILE, c = e - r * TILE;
-    const int row = kb * KB + r, col = c0 + c;
-    double v = 0.0;
-    if (row < nrows && col < nv) v = orows[(size_t)row * nv + col] - kappa[row] * obar[col];
-    t[r][c] = v;
-  }
-  __syncthreads();
-  unsigned char* base = P + ((size_t)kb * nvp + c0) * COLB;
-  // 128 columns x 8 chunks of 16 bytes
-  for (int e = tid; e < TILE * 8; e += 256) {
-    const int c = e >> 3, q = e & 7;
-    u32x4 w;
-    if constexpr (MODE == 0) {
-      const double d0 = t[2 * q][c], d1 = t[2 * q + 1][c];
-      w = u32x4{(unsigned)__double_as_longlong(d0), (unsigned)(__double_as_longlong(d0) >> 32),
-                (unsigned)__double_as_longlong(d1), (unsigned)(__double_as_longlong(d1) >> 32)};
-    } else if constexpr (MODE == 1) {
-      w = u32x4{__float_as_uint((float)t[4 * q][c]), __float_as_uint((float)t[4 * q + 1][c]),
-                __float_as_uint((float)t[4 * q + 2][c]), __float_as_uint((float)t[4 * q + 3][c])};
-    } else {
-      // chunks 0..3: hi of k = 8q .. 8q+7; chunks 4..7: lo of k = 8(q-4) ..
-      const int k0 = 8 * (q & 3);
-      unsigned short h[8];
+  const int kb0 = chunk * per, kb1 = min(nkb, kb0 + per);
+  // element e = tid + 256 i of a tile: row e / TILE, column e % TILE
+  const int lc = tid & (TILE - 1), lr = tid >> 7;
+  const int col = c0 + lc;
+  const double ob = col < nv ? obar[col] : 0.0;
+  double nxt[NL];
+  auto load = [&](int kb) {
 #pragma unroll
-      for (int j = 0; j < 8; j++) {
-        const double v = t[k0 + j][c];
-        const __bf16 hi = (__bf16)(float)v;
-        const __bf16 sel = (q < 4) ? hi : (__bf16)(float)(v - (double)(float)hi);
-        h[j] = __builtin_bit_cast(unsigned short, sel);
-      }
-      w = u32x4{(unsigned)h[0] | ((unsigned)h[1] << 16), (unsigned)h[2] | ((unsigned)h[3] << 16),
-                (unsigned)h[4] | ((unsigned)h[5] << 16), (unsigned)h[6] | ((unsigned)h[7] << 16)};
+    for (int i = 0; i < NL; i++) {
+      const int row = kb * KB + lr + 2 * i;
+      nxt[i] = (row < nrows && col < nv) ? orows[(size_t)row * nv + col] - kappa[row] * ob : 0.0;
     }
-    *(u32x4*)(base + (size_t)c * COLB + 16 * q) = w;
+  };
+  double dacc = 0.0;  // column lc, rows lr, lr + 2, ... of every block
+  if (kb0 < kb1) load(kb0);
+  for (int kb = kb0; kb < kb1; kb++) {
+#pragma unroll
+    for (int i = 0; i < NL; i++) t[lr + 2 * i][lc] = nxt[i];
+    if constexpr (DIAG) {
+#pragma unroll
+      for (int i = 0; i < NL; i++) dacc = fma(nxt[i], nxt[i], dacc);
+    }
+    __syncthreads();
+    if (kb + 1 < kb1) load(kb + 1);
+    unsigned char* base = P + ((size_t)kb * nvp + c0) * COLB;
+    // 128 columns x 8 chunks of 16 bytes
+    for (int e = tid; e < TILE * 8; e += 256) {
+      const int c = e >> 3, q = e & 7;
+      u32x4 w;
+      if constexpr (MODE == 0) {
+        const double d0 = t[2 * q][c], d1 = t[2 * q + 1][c];
+        w = u32x4{(unsigned)__double_as_longlong(d0), (unsigned)(__double_as_longlong(d0) >> 32),
+                  (unsigned)__double_as_longlong(d1), (unsigned)(__double_as_longlong(d1) >> 32)};
+      } else if constexpr (MODE == 1) {
+        w = u32x4{__float_as_uint((float)t[4 * q][c]), __float_as_uint((float)t[4 * q + 1][c]),
+                  __float_as_uint((float)t[4 * q + 2][c]), __float_as_uint((float)t[4 * q + 3][c])};
+      } else {
+        // chunks 0..3: hi of k = 8q .. 8q+7; chunks 4..7: lo of k = 8(q-4) ..
+        const int k0 = 8 * (q & 3);
+        unsigned short h[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          const double v = t[k0 + j][c];
+          const __bf16 hi = (__bf16)(float)v;
+          const __bf16 sel = (q < 4) ? hi : (__bf16)(float)(v - (double)(float)hi);
+          h[j] = __builtin_bit_cast(unsigned short, sel);
+        }
+        w = u32x4{(unsigned)h[0] | ((unsigned)h[1] << 16), (unsigned)h[2] | ((unsigned)h[3] << 16),
+                  (unsigned)h[4] | ((unsigned)h[5] << 16), (unsigned)h[6] | ((unsigned)h[7] << 16)};
+      }
+      *(u32x4*)(base + (size_t)c * COLB + 16 * q) = w;
+    }
+    __syncthreads();  // t is rewritten by the next block
+  }
+  if constexpr (DIAG) {
+    dsum[lr][lc] = dacc;
+    __syncthreads();
+    if (tid < TILE && col < nv) dpart[(size_t)chunk * (nv + 1) + col] = dsum[0][tid] + dsum[1][tid];
   }
 }
 
@@ -442,14 +474,20 @@ SyrkPlan syrk_plan(int precision, int nrows, int nv, int cus, int tile, int nspl
 }
 
 void launch_center_pack(const SyrkPlan& pl, int nrows, int nv, const double* orows, const double* kappa,
-                        const double* obar, unsigned char* packed, hipStream_t st) {
-  const dim3 grid(pl.nvp / TILE, pl.nkb);
+                        const double* obar, unsigned char* packed, double* dpart, hipStream_t st) {
+  // SR_ROW_CHUNKS chunks of K-blocks (every chunk writes its diagonal
+  // partials, zero for an empty one)
+  const int per = (pl.nkb + SR_ROW_CHUNKS - 1) / SR_ROW_CHUNKS;
+  const dim3 grid(pl.nvp / TILE, SR_ROW_CHUNKS);
   if (pl.mode == 2)
-    hipLaunchKernelGGL(k_center_pack<2>, grid, dim3(256), 0, st, nrows, nv, pl.nvp, orows, kappa, obar, packed);
+    hipLaunchKernelGGL((k_center_pack<2, true>), grid, dim3(256), 0, st, nrows, nv, pl.nvp, pl.nkb, per, orows,
+                       kappa, obar, packed, dpart);
   else if (pl.mode == 1)
-    hipLaunchKernelGGL(k_center_pack<1>, grid, dim3(256), 0, st, nrows, nv, pl.nvp, orows, kappa, obar, packed);
+    hipLaunchKernelGGL((k_center_pack<1, true>), grid, dim3(256), 0, st, nrows, nv, pl.nvp, pl.nkb, per, orows,
+                       kappa, obar, packed, dpart);
   else
-    hipLaunchKernelGGL(k_center_pack<0>, grid, dim3(256), 0, st, nrows, nv, pl.nvp, orows, kappa, obar, packed);
+    hipLaunchKernelGGL((k_center_pack<0, false>), grid, dim3(256), 0, st, nrows, nv, pl.nvp, pl.nkb, per, orows,
+                       kappa, obar, packed, dpart);
 }
 
 void launch_syrk_packed(const SyrkPlan& pl, int nv, const unsigned char* packed, double* part, double inv_norm,
