@@ -206,7 +206,8 @@ def energy_run(args):
     log(f"energy run: {args.L}x{args.L}, S {args.energy_run}, {args.energy_iters} SR iterations, "
         f"Nav {args.energy_nav}, {args.walkers} walkers x {nspw} samples, run dir {rundir}")
     t0 = time.perf_counter()
-    proc = subprocess.Popen([driver, str(args.walkers), str(prec), str(args.recompute_k)], cwd=rundir,
+    proc = subprocess.Popen([driver, str(args.walkers), str(prec), str(args.recompute_k), "0", "0",
+                             str(args.energy_seed)], cwd=rundir,
                             stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     it = 0
     for line in proc.stdout:
@@ -222,7 +223,8 @@ def energy_run(args):
         sys.exit(rc)
     hist = np.loadtxt(os.path.join(rundir, "Energy_vs_Iteration.txt"))
     e_av, e_err = [float(t) for t in open(os.path.join(rundir, "Average_Energy.txt")).readline().split()[:2]]
-    print(json.dumps({"energy_run": args.energy_run, "workload": f"j1j2_{args.L}x{args.L}_J2_{args.J2}_alphar{args.alphar}",
+    print(json.dumps({"energy_run": args.energy_run, "seed": args.energy_seed,
+                      "workload": f"j1j2_{args.L}x{args.L}_J2_{args.J2}_alphar{args.alphar}",
                       "iterations": args.energy_iters, "nav": args.energy_nav,
                       "samples_per_iteration": args.walkers * nspw,
                       "energy_per_site": e_av / 4 / N, "error_per_site": e_err / 4 / N,
@@ -337,6 +339,7 @@ def main():
     ap.add_argument("--energy-iters", type=int, default=250)
     ap.add_argument("--energy-nav", type=int, default=20)
     ap.add_argument("--energy-dir", default=None, help="run directory of --energy-run")
+    ap.add_argument("--energy-seed", type=int, default=1, help="key of the walkers' Philox streams in --energy-run")
     ap.add_argument("--batch-run", type=int, default=0, metavar="ITERS",
                     help="the uneven multi-lattice workload (15 models in one sampler launch) for ITERS iterations")
     ap.add_argument("--batch-walkers", type=int, default=16384)
@@ -558,7 +561,9 @@ def main():
     # sampler keeps F^-1, (N/2)^2 entries per copy, and applies an accepted
     # move's update inside the next proposal's read pass: each proposal reads
     # them once and an accepted one writes them once, 2 C N^3 (1 + A).
-    block = args.projection == "normal"
+    # the block form needs the register recompute, N <= 104; larger lattices
+    # keep the folded skew storage in every projection
+    block = args.projection == "normal" and N <= 104
     b_unit = 2.0 * Cc * N ** 3 * (1.0 + acc) if block else 4.0 * Cc * N ** 3 * (1.0 + 2.0 * acc)
     achieved = b_unit * sweeps_per_launch / t_sweep / 1e9
     # measured HBM bytes per walker sweep of this kernel and workload

@@ -40,6 +40,8 @@ struct rbmpp_handle {
   std::vector<void*> allocs;
   // tables and parameters
   int *d_wmap = nullptr, *d_invmap = nullptr, *d_orb = nullptr;
+  int* d_pboff = nullptr;        // DevModel::pbin_off
+  unsigned* d_pbpairs = nullptr; // DevModel::pbin_pairs
   unsigned char* d_subB = nullptr;
   double2* d_chi = nullptr;
   double* d_se = nullptr;
@@ -161,6 +163,7 @@ static DevModel devmodel(const rbmpp_handle* h) {
   }
   m.wmap = h->d_wmap; m.invmap = h->d_invmap; m.orb = h->d_orb; m.subB = h->d_subB; m.chi = h->d_chi;
   m.se = h->d_se; m.f = h->d_f; m.Wr = h->d_Wr; m.br = h->d_br; m.Wc = h->d_Wc; m.bc = h->d_bc;
+  m.pbin_off = h->d_pboff; m.pbin_pairs = h->d_pbpairs;
   return m;
 }
 
@@ -303,7 +306,9 @@ int rbmpp_create(const rbmpp_config* cfg, const int32_t* w_map, const double* pa
       (rc = dalloc(h, &h->d_orb, NN)) || (rc = dalloc(h, &h->d_subB, N)) || (rc = dalloc(h, &h->d_chi, P)) ||
       (rc = dalloc(h, &h->d_se, C * 4 * NN)) || (rc = dalloc(h, &h->d_Wr, (size_t)h->ar * N)) ||
       (rc = dalloc(h, &h->d_br, h->ar)) || (rc = dalloc(h, &h->d_Wc, (size_t)h->ac * N)) ||
-      (rc = dalloc(h, &h->d_bc, h->ac)) || (rc = dalloc(h, &h->d_f, h->nsl)))
+      (rc = dalloc(h, &h->d_bc, h->ac)) || (rc = dalloc(h, &h->d_f, h->nsl)) ||
+      (rc = dalloc(h, &h->d_pboff, (size_t)P * (h->nsl + 1))) ||
+      (rc = dalloc(h, &h->d_pbpairs, (size_t)P * N * (N - 1))))
     return rc;
   WalkerState& ws = h->ws;
   if ((rc = dalloc(h, &ws.x, W * N)) || (rc = dalloc(h, &ws.eidx, W * N)) || (rc = dalloc(h, &ws.ecfg, W * N)) ||
@@ -325,6 +330,26 @@ int rbmpp_create(const rbmpp_config* cfg, const int32_t* w_map, const double* pa
   HIPCHK(h, hipMemcpyAsync(h->d_wmap, h->wmap.data(), sizeof(int) * P * N, hipMemcpyHostToDevice, h->st));
   HIPCHK(h, hipMemcpyAsync(h->d_invmap, h->invmap.data(), sizeof(int) * P * N, hipMemcpyHostToDevice, h->st));
   HIPCHK(h, hipMemcpyAsync(h->d_orb, h->orb.data(), sizeof(int) * NN, hipMemcpyHostToDevice, h->st));
+  // the site pairs of every orbital bin under each operation, for the
+  // fixed-order sums of the pair-product log-derivatives
+  std::vector<int> pboff((size_t)P * (h->nsl + 1), 0);
+  std::vector<unsigned> pbpairs((size_t)P * N * (N - 1));
+  for (int g = 0; g < P; g++) {
+    int* off = pboff.data() + (size_t)g * (h->nsl + 1);
+    const int* wm = h->wmap.data() + (size_t)g * N;
+    for (int A = 0; A < N; A++)
+      for (int B = 0; B < N; B++)
+        if (A != B) off[h->orb[(size_t)wm[A] * N + wm[B]] + 1]++;
+    for (int o = 0; o < h->nsl; o++) off[o + 1] += off[o];
+    std::vector<int> fill(off, off + h->nsl);
+    unsigned* pr = pbpairs.data() + (size_t)g * N * (N - 1);
+    for (int A = 0; A < N; A++)
+      for (int B = 0; B < N; B++)
+        if (A != B) pr[fill[h->orb[(size_t)wm[A] * N + wm[B]]]++] = (unsigned)A | ((unsigned)B << 16);
+  }
+  HIPCHK(h, hipMemcpyAsync(h->d_pboff, pboff.data(), sizeof(int) * pboff.size(), hipMemcpyHostToDevice, h->st));
+  HIPCHK(h, hipMemcpyAsync(h->d_pbpairs, pbpairs.data(), sizeof(unsigned) * pbpairs.size(), hipMemcpyHostToDevice,
+                           h->st));
   HIPCHK(h, hipMemcpyAsync(h->d_subB, h->subB.data(), N, hipMemcpyHostToDevice, h->st));
   HIPCHK(h, hipMemcpyAsync(h->d_chi, h->chi.data(), sizeof(double) * 2 * P, hipMemcpyHostToDevice, h->st));
   // walker bookkeeping: Nfastup = min(100, 2N) (main.f90:309), counters 0

@@ -28,6 +28,11 @@ struct DevModel {
   const double* br;           // [ar]
   const double2* Wc;          // [ac][N]
   const double2* bc;          // [ac]
+  // ordered site pairs (A, B), A != B, of every orbital bin o under operation
+  // g, orb[wmap_g(A)][wmap_g(B)] = o, in a fixed order: pairs
+  // pbin_pairs[g][pbin_off[g][o] .. pbin_off[g][o + 1]), packed A | B << 16
+  const int* pbin_off;        // [P][nsl + 1]
+  const unsigned* pbin_pairs; // [P][N (N - 1)]
 };
 
 // ---------------------------------------------------------------- complex

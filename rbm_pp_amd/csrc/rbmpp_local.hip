@@ -116,34 +116,7 @@ __global__ __launch_bounds__(512) void k_local(LocalArgs a) {
           out[e] = XINV(j, i);
         }
       }
-      if (want_o) {
-        // calculate_derivative_Pfaffian_child (slater.f90:136-167) weighted by
-        // psi_wo_pf(q) (calc_phys_quantity.f90:209-216)
-        const double cs = pfq * m.sp_cs[sidx], cc = pfq * m.sp_cc[sidx], ss = pfq * m.sp_ss[sidx];
-        const double2 wq = s.psiwo[q];
-        for (int e = tid; e < np * np; e += nth) {
-          const int msj = e / np, msi = e - msj * np;
-          const int ti = m.wmap[g * N + s.eidx[msi]], tip = m.wmap[g * N + s.eidx[msi + np]];
-          const int tj = m.wmap[g * N + s.eidx[msj]], tjp = m.wmap[g * N + s.eidx[msj + np]];
-          // InvM(a,b) = Xinv(b,a)
-          const double d1 = -XINV(msj, msi) * cs;
-          const double d2 = XINV(msj + np, msi) * cc;
-          const double d3 = -XINV(msj, msi + np) * ss;
-          const double d4 = XINV(msj + np, msi + np) * cs;
-          const int o1 = m.orb[ti * N + tj], o2 = m.orb[ti * N + tjp];
-          const int o3 = m.orb[tip * N + tj], o4 = m.orb[tip * N + tjp];
-          atomicAdd(&s.opp[0][o1], d1 * wq.x);
-          atomicAdd(&s.opp[0][o2], d2 * wq.x);
-          atomicAdd(&s.opp[0][o3], d3 * wq.x);
-          atomicAdd(&s.opp[0][o4], d4 * wq.x);
-          if (!m.real_only) {
-            atomicAdd(&s.opp[1][o1], d1 * wq.y);
-            atomicAdd(&s.opp[1][o2], d2 * wq.y);
-            atomicAdd(&s.opp[1][o3], d3 * wq.y);
-            atomicAdd(&s.opp[1][o4], d4 * wq.y);
-          }
-        }
-      }
+      if (want_o) wg_pp_oloc(m, s, q, pfq, s.psiwo[q], tid, nth, [&](int u, int v) -> double { return XINV(u, v); });
       // exchange ratios (calculate_new_PfM_two_child, pfupdate_two.f90:27-88)
       for (int b0 = 0; b0 < nb; b0 += nwv) {
         const int b = b0 + wid;

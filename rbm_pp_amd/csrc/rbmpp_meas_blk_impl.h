@@ -66,6 +66,9 @@ struct BlkSmem {
   unsigned sgnbits[RBMPP_CMAX / 2];
   int nb, status;
   double2 psi_cur;  // psi(x) of the configuration, for the correlation sums
+  int wc[3 * BNW];        // per-wave counts of the electron and bond lists
+  short bpos[4 * NMX];    // list position of bond (t, i1), -1 when parallel
+  double red[2 * BNW];    // per-wave partial sums of E_loc
 };
 static_assert(sizeof(BlkSmem) <= 160 * 1024, "the block estimator's LDS fits a gfx950 workgroup");
 
@@ -79,10 +82,6 @@ __device__ __attribute__((noinline)) void theta_sign(const DevModel& m, BlkSmem&
 __device__ __attribute__((noinline)) void o_row(const DevModel& m, const BlkSmem& s, double rw, double* ore,
                                                 double* oim, bool cplx_rows) {
   wg_orow(m, s, rw, ore, oim, cplx_rows);
-}
-__device__ __attribute__((noinline)) int bonds_of(const DevModel& m, BlkSmem& s) { return bond_list(m, s); }
-__device__ __attribute__((noinline)) double2 eloc_of(const DevModel& m, const BlkSmem& s, double2 p) {
-  return eloc_from_bonds(m, s, p);
 }
 
 // Exchange ratios in block form.  An exchange of the up electron u (site su)
@@ -125,6 +124,7 @@ __device__ __attribute__((noinline)) void copy_yh(const DevModel& m, BlkSmem& s,
     const int d = e / np, u = e - d * np;
     const double* gd = s.u.c.gdd + d * np;
     double t = 0.0;
+#pragma unroll 10
     for (int k = 0; k < np; k++) t = fma(gd[k], s.binv[k * np + u], t);
     s.u.c.yh[e] = t;
   }
@@ -143,8 +143,8 @@ __device__ __attribute__((noinline)) void bond_ratios(const DevModel& m, BlkSmem
   const bool live = lane < np;
   const int su_l = live ? s.eidx[lane] : 0;  // site of up electron `lane`
   for (int p = wid; BT * p < nb; p += BNW) {
-    int bb[BT], u[BT], d[BT], su[BT], sd[BT];
-    double g[BT], bv[BT], yv[BT];
+    int bb[BT], u[BT], d[BT];
+    double g[BT], bv[BT], yv[BT], dl[BT], ep[BT], gd[BT];
 #pragma unroll
     for (int t = 0; t < BT; t++) {
       bb[t] = min(BT * p + t, nb - 1);  // a missing bond repeats the last one (not written)
@@ -153,11 +153,18 @@ __device__ __attribute__((noinline)) void bond_ratios(const DevModel& m, BlkSmem
       const bool aup = msa < np;
       u[t] = __builtin_amdgcn_readfirstlane(aup ? msa : msb);
       d[t] = __builtin_amdgcn_readfirstlane((aup ? msb : msa) - np);
-      su[t] = __builtin_amdgcn_readfirstlane(aup ? i1 : i2);
-      sd[t] = __builtin_amdgcn_readfirstlane(aup ? i2 : i1);
-      g[t] = live ? -S[(size_t)(su[t] + N) * two + su_l] : 0.0;
+      const int su = __builtin_amdgcn_readfirstlane(aup ? i1 : i2);
+      const int sd = __builtin_amdgcn_readfirstlane(aup ? i2 : i1);
+      g[t] = live ? -S[(size_t)(su + N) * two + su_l] : 0.0;
       bv[t] = live ? s.binv[d[t] * np + lane] : 0.0;
       yv[t] = live ? s.u.c.yh[d[t] * np + lane] : 0.0;
+      // the scalars of blk_ratio, loaded here so that their latency overlaps
+      // the gathers and the wave sums
+      const double* Su = S + (size_t)su * two;
+      const double* Sd = S + (size_t)sd * two;
+      gd[t] = Sd[sd + N];
+      dl[t] = Sd[su + N] - gd[t];
+      ep[t] = Su[su + N] - Su[sd + N];
     }
     // lanes 15, 47, 31, 63 hold bonds 0-3
     const double xv = wave_sum4(bv[0] * g[0], bv[1] * g[1], bv[2] * g[2], bv[3] * g[3]);
@@ -168,12 +175,7 @@ __device__ __attribute__((noinline)) void bond_ratios(const DevModel& m, BlkSmem
       if (BT * p + t >= nb) break;  // wave-uniform
       const double xh = readlane_d(xv, SUMLANE[t]), w = readlane_d(wv, SUMLANE[t]);
       const double bdu = readlane_d(bv[t], u[t]), ydu = readlane_d(yv[t], u[t]);
-      if (lane == t) {
-        const double* Su = S + (size_t)su[t] * two;
-        const double* Sd = S + (size_t)sd[t] * two;
-        s.rat[bb[t]] = blk_ratio(ydu, xh, w, bdu, Sd[su[t] + N] - Sd[sd[t] + N], Su[su[t] + N] - Su[sd[t] + N],
-                                 Sd[sd[t] + N]);
-      }
+      if (lane == t) s.rat[bb[t]] = blk_ratio(ydu, xh, w, bdu, dl[t], ep[t], gd[t]);
     }
   }
 }
@@ -244,23 +246,16 @@ __global__ __launch_bounds__(BNT) void k_meas_blk(LocalArgs a) {
       s.psib[k] = make_double2(0.0, 0.0);
     }
     __syncthreads();
+    // electron lists and the antiparallel bonds (or the one custom exchange)
+    wg_ele_bonds(m, s, want_eloc && !custom, s.wc, s.bpos);
     if (tid == 0) {
-      // x_to_Ele_Idx_Ele_Cfg (interface_for_PP.f90:5-33)
-      int u = 0, d = np;
-      for (int i = 0; i < N; i++) {
-        if (s.x[i] == 1) { s.eidx[u] = i; s.ecfg[i] = u; u++; }
-        else { s.eidx[d] = i; s.ecfg[i] = d; d++; }
-      }
-      if (u != np) s.status = WS_INIT;
-      int nb = 0;
       if (custom) {
         s.bi1[0] = (short)a.ex_i1[cfg];
         s.bi2[0] = (short)a.ex_i2[cfg];
-        nb = 1;
-      } else if (want_eloc) {
-        nb = bonds_of(m, s);
+        s.nb = 1;
+      } else if (!want_eloc) {
+        s.nb = 0;
       }
-      s.nb = nb;
     }
     __syncthreads();
     if (s.status != WS_OK) {
@@ -307,25 +302,12 @@ __global__ __launch_bounds__(BNT) void k_meas_blk(LocalArgs a) {
         }
       }
       if (want_o) {
-        // calculate_derivative_Pfaffian_child (slater.f90:136-167) weighted by
-        // psi_wo_pf(q): X^-1(u, np+d) = -F^-1(d,u) lands in the orbital of
-        // (site(np+d), site(u)) with weight -ss, X^-1(np+d, u) = F^-1(d,u) in
-        // that of (site(u), site(np+d)) with weight cc
-        const double cc = pfq * m.sp_cc[sidx], ss = pfq * m.sp_ss[sidx];
-        const double2 wq = s.psiwo[q];
-        for (int e = tid; e < np * np; e += nth) {
-          const int d = e / np, u = e - d * np;
-          const double bv = s.binv[e];
-          const int su = s.site[u], sd = s.site[np + d];
-          const int o1 = m.orb[sd * N + su], o2 = m.orb[su * N + sd];
-          const double d1 = ss * bv, d2 = cc * bv;
-          atomicAdd(&s.opp[0][o1], d1 * wq.x);
-          atomicAdd(&s.opp[0][o2], d2 * wq.x);
-          if (!m.real_only) {
-            atomicAdd(&s.opp[1][o1], d1 * wq.y);
-            atomicAdd(&s.opp[1][o2], d2 * wq.y);
-          }
-        }
+        // X^-1(u, np+d) = -F^-1(d, u), X^-1(np+d, u) = F^-1(d, u); the
+        // same-spin entries have weight cs = 0
+        const double* Bq = s.binv;
+        wg_pp_oloc(m, s, q, pfq, s.psiwo[q], tid, nth, [&](int u, int v) -> double {
+          return u < np ? -Bq[(v - np) * np + u] : Bq[(u - np) * np + v];
+        });
       }
       // exchange ratios of the antiparallel bonds (blk_ratio)
       if (nb > 0 || CORR) {
@@ -348,16 +330,21 @@ __global__ __launch_bounds__(BNT) void k_meas_blk(LocalArgs a) {
     }
     __syncthreads();
     // amplitude, local energy, reweighting
-    if (tid == 0) {
+    {
+      // every thread forms p the same way (the 16 terms in copy order)
       double2 p = make_double2(0.0, 0.0);
       for (int q = 0; q < C; q++) p = cadd(p, s.psiq[q]);
       const double rw = a.shift >= 0.0 ? 1.0 / (cmod(p) + a.shift) : 1.0;
-      a.psi[cfg] = p;
-      if constexpr (CORR) s.psi_cur = p;
-      if (a.rw) a.rw[cfg] = rw;
-      if (want_eloc && !custom) a.eloc[cfg] = cscale(eloc_of(m, s, p), rw);
-      if (custom && a.ex_psi) a.ex_psi[cfg] = s.psib[0];
-      s.selv[0] = rw;
+      double2 el = make_double2(0.0, 0.0);
+      if (want_eloc && !custom) el = wg_eloc(m, s, p, s.bpos, s.red);
+      if (tid == 0) {
+        a.psi[cfg] = p;
+        if constexpr (CORR) s.psi_cur = p;
+        if (a.rw) a.rw[cfg] = rw;
+        if (want_eloc && !custom) a.eloc[cfg] = cscale(el, rw);
+        if (custom && a.ex_psi) a.ex_psi[cfg] = s.psib[0];
+        s.selv[0] = rw;
+      }
     }
     for (int q = tid; q < C; q += nth) {
       if (a.pf) a.pf[(size_t)cfg * C + q] = s.pf[q];

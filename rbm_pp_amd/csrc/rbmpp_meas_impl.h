@@ -268,30 +268,11 @@ __global__ __launch_bounds__(MNT) void k_meas(LocalArgs a) {
         s.pf[q] = pfq;
         s.psiq[q] = cscale(s.psiwo[q], pfq);
       }
-      if (want_o) {
-        // calculate_derivative_Pfaffian_child (slater.f90:136-167) weighted by
-        // psi_wo_pf(q) (calc_phys_quantity.f90:209-216): entry (u, v) of X^-1
-        // lands in the orbital of (site(v), site(u)) with weight -cs, cc, -ss
-        // or cs by the spins of electrons u and v
-        const double cs = pfq * m.sp_cs[sidx], cc = pfq * m.sp_cc[sidx], ss = pfq * m.sp_ss[sidx];
-        const double2 wq = s.psiwo[q];
-#pragma unroll
-        for (int aa = 0; aa < RA; aa++) {
-          const int u = tr + 16 * aa;
-#pragma unroll
-          for (int b = 0; b < RB; b++) {
-            const int v = tc + 64 * b;
-            if (u < N && v < N) {
-              const double w = u < np ? (v < np ? -cs : -ss) : (v < np ? cc : cs);
-              const double d = w * TILE(x, aa, b);
-              const int o = m.orb[s.site[v] * N + s.site[u]];
-              atomicAdd(&s.opp[0][o], d * wq.x);
-              if (!m.real_only) atomicAdd(&s.opp[1][o], d * wq.y);
-            }
-          }
-        }
-      }
       __syncthreads();  // inv complete
+      if (want_o) {
+        // X^-1(u, v) = -InvM(u, v) = -inv[u * ld + v]
+        wg_pp_oloc(m, s, q, pfq, s.psiwo[q], tid, nth, [&](int u, int v) -> double { return -s.inv[u * ld + v]; });
+      }
       // exchange ratios (calculate_new_PfM_two_child, pfupdate_two.f90:27-88):
       // waves 0..BW-1 take BT bonds per pass (bonds BT*p .. BT*p+BT-1 for
       // p = w, w + BW, ...); every row of X^-1 read from LDS feeds BT dot

@@ -28,6 +28,128 @@ __device__ int bond_list(const DevModel& m, S& s) {
   return nb;
 }
 
+// Electron lists (x_to_Ele_Idx_Ele_Cfg, interface_for_PP.f90:5-33) and, with
+// bonds, the antiparallel bonds in the order of calc_phys_quantity.f90:332-339,
+// by a workgroup of at least 4N threads: thread i takes site i and bond
+// (t, i1) = (i / N, i % N); ranks come from wave ballots and the per-wave
+// counts in wc (3 x waves ints of LDS).  bpos[e] receives the list position
+// of bond e (-1 when parallel); s.nb the count.  Sets s.status on a
+// configuration outside the Sz = 0 sector.  Ends in step.
+template <class S>
+__device__ void wg_ele_bonds(const DevModel& m, S& s, bool bonds, int* wc, short* bpos) {
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, nw = blockDim.x >> 6;
+  const int N = m.N, np = m.npair;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const bool site = tid < N;
+  const bool up = site && s.x[tid] == 1;
+  const unsigned long long bu = __ballot(up), bd = __ballot(site && !up);
+  bool anti = false;
+  int i1 = 0, i2 = 0;
+  const bool inb = bonds && tid < 4 * N;
+  if (inb) {
+    const int t = tid / N;
+    i1 = tid - t * N;
+    i2 = bond_partner(m.L, t, i1);
+    anti = s.x[i1] != s.x[i2];
+  }
+  const unsigned long long ba = __ballot(anti);
+  if (lane == 0) {
+    wc[wid] = __popcll(bu);
+    wc[nw + wid] = __popcll(bd);
+    wc[2 * nw + wid] = __popcll(ba);
+  }
+  __syncthreads();
+  if (site) {
+    int ou = 0, od = 0;
+    for (int w = 0; w < wid; w++) { ou += wc[w]; od += wc[nw + w]; }
+    if (up) {
+      const int u = ou + __popcll(bu & below);
+      if (u < np) { s.eidx[u] = tid; s.ecfg[tid] = u; }
+    } else {
+      const int d = np + od + __popcll(bd & below);
+      if (d < N) { s.eidx[d] = tid; s.ecfg[tid] = d; }
+    }
+  }
+  if (inb) {
+    int b = -1;
+    if (anti) {
+      int o = 0;
+      for (int w = 0; w < wid; w++) o += wc[2 * nw + w];
+      b = o + __popcll(ba & below);
+      s.bi1[b] = (short)i1;
+      s.bi2[b] = (short)i2;
+    }
+    bpos[tid] = (short)b;
+  }
+  if (tid == 0) {
+    int nu = 0, nb = 0;
+    for (int w = 0; w < nw; w++) { nu += wc[w]; nb += wc[2 * nw + w]; }
+    if (nu != np) s.status = WS_INIT;
+    if (bonds) s.nb = nb;
+  }
+  __syncthreads();
+}
+
+// E_loc (calc_phys_quantity.f90:332-375) by the whole workgroup: thread e <
+// 4N takes bond (t, i1) = (e / N, e % N), J s_i1 s_i2 psi plus, for the
+// antiparallel bond at bpos[e] >= 0, 2 J sgn psi(x'); the terms are summed in
+// a fixed order (wave sums, then the waves in order; red: 2 x waves doubles
+// of LDS).  Every thread returns the sum.
+template <class S>
+__device__ double2 wg_eloc(const DevModel& m, const S& s, double2 p, const short* bpos, double* red) {
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, nw = blockDim.x >> 6;
+  const int N = m.N;
+  double2 e = make_double2(0.0, 0.0);
+  if (tid < 4 * N) {
+    const int t = tid / N, i1 = tid - t * N, i2 = bond_partner(m.L, t, i1);
+    const double J = t <= 1 ? m.J1 : m.J2;
+    const double sg = t <= 1 ? -1.0 : 1.0;
+    e = cscale(p, J * (double)s.x[i1] * (double)s.x[i2]);
+    const int b = bpos[tid];
+    if (b >= 0) e = cadd(e, cscale(s.psib[b], 2.0 * J * sg));
+  }
+  const double ex = wave_sum_to63(e.x), ey = wave_sum_to63(e.y);
+  if (lane == 63) {
+    red[2 * wid] = ex;
+    red[2 * wid + 1] = ey;
+  }
+  __syncthreads();
+  double2 r = make_double2(0.0, 0.0);
+  for (int w = 0; w < nw; w++) r = cadd(r, make_double2(red[2 * w], red[2 * w + 1]));
+  return r;
+}
+
+// Pair-product log-derivatives of copy q (calculate_derivative_Pfaffian_child,
+// slater.f90:136-167, weighted by psi_wo_pf(q) = wq as calc_phys_quantity.f90:
+// 209-216), summed in a fixed order: entry (u, v) of X^-1 lands in the
+// orbital of (site(v), site(u)) with weight -cs, -ss, cc or cs by the spins
+// of electrons u and v, so thread t0 + k tn owns orbital o and adds, over the
+// site pairs (A, B) of o under the copy's operation (DevModel::pbin_*, in
+// their fixed order), w(u, v) xinv(u, v) for v = Ele_Cfg(A), u = Ele_Cfg(B);
+// opp[.][o] += sum wq.  xinv(u, v) returns X^-1(u, v) and is called for
+// nonzero weights only.  Every orbital keeps its owner in every copy, so the
+// sums need no atomics and are the same from run to run.
+template <class S, class XI>
+__device__ void wg_pp_oloc(const DevModel& m, S& s, int q, double pfq, double2 wq, int t0, int tn, XI xinv) {
+  const int g = q >> 1, sidx = q & 1, N = m.N, np = m.npair;
+  const double cs = pfq * m.sp_cs[sidx], cc = pfq * m.sp_cc[sidx], ss = pfq * m.sp_ss[sidx];
+  const int* off = m.pbin_off + (size_t)g * (m.nsl + 1);
+  const unsigned* pr = m.pbin_pairs + (size_t)g * N * (N - 1);
+  for (int o = t0; o < m.nsl; o += tn) {
+    double acc = 0.0;
+    const int k1 = off[o + 1];
+#pragma unroll 4
+    for (int k = off[o]; k < k1; k++) {
+      const unsigned ab = pr[k];
+      const int v = s.ecfg[ab & 0xffffu], u = s.ecfg[ab >> 16];
+      const double w = u < np ? (v < np ? -cs : -ss) : (v < np ? cc : cs);
+      if (w != 0.0) acc = fma(w, xinv(u, v), acc);
+    }
+    s.opp[0][o] += acc * wq.x;
+    if (!m.real_only) s.opp[1][o] += acc * wq.y;
+  }
+}
+
 // E_loc = sum_bonds J s_i s_j psi + sum_antiparallel 2 J sgn psi(x')
 // (calc_phys_quantity.f90:332-375), accumulated in the reference's order
 template <class S>

@@ -409,32 +409,12 @@ __device__ __attribute__((noinline)) void wg_measure_sample(const DevModel& m, S
     const bool pp_wave = (nwv >= 2) ? (wid >= nbw) : true;
     const bool bond_wave = (nwv >= 2) ? (wid < nbw) : true;
     if (pp_wave) {
-      // InvM(a,b) = Ms[a*N + b]; weights cs, cc, ss times Pf (slater.f90:136-167)
-      const double cs = pfq * m.sp_cs[sidx], cc = pfq * m.sp_cc[sidx], ss = pfq * m.sp_ss[sidx];
+      // pair-product O_k of the copy (slater.f90:136-167) from InvM(a, b) =
+      // Ms[a * N + b] = X^-1(b, a)
       const double2 wq = psi_wo_pf(m, q, s.thc, s.thr, s.sgn[g]);
       const int t0 = (nwv >= 2) ? (tid - nbw * RBMPP_WAVE) : tid;
       const int tn = (nwv >= 2) ? (nth - nbw * RBMPP_WAVE) : nth;
-      for (int e = t0; e < np * np; e += tn) {
-        const int msj = e / np, msi = e - msj * np;
-        const int ti = m.wmap[g * N + s.eidx[msi]], tip = m.wmap[g * N + s.eidx[msi + np]];
-        const int tj = m.wmap[g * N + s.eidx[msj]], tjp = m.wmap[g * N + s.eidx[msj + np]];
-        const double d1 = -Ms[(size_t)msi * N + msj] * cs;
-        const double d2 = Ms[(size_t)msi * N + msj + np] * cc;
-        const double d3 = -Ms[(size_t)(msi + np) * N + msj] * ss;
-        const double d4 = Ms[(size_t)(msi + np) * N + msj + np] * cs;
-        const int o1 = m.orb[ti * N + tj], o2 = m.orb[ti * N + tjp];
-        const int o3 = m.orb[tip * N + tj], o4 = m.orb[tip * N + tjp];
-        atomicAdd(&s.opp[0][o1], d1 * wq.x);
-        atomicAdd(&s.opp[0][o2], d2 * wq.x);
-        atomicAdd(&s.opp[0][o3], d3 * wq.x);
-        atomicAdd(&s.opp[0][o4], d4 * wq.x);
-        if (!m.real_only) {
-          atomicAdd(&s.opp[1][o1], d1 * wq.y);
-          atomicAdd(&s.opp[1][o2], d2 * wq.y);
-          atomicAdd(&s.opp[1][o3], d3 * wq.y);
-          atomicAdd(&s.opp[1][o4], d4 * wq.y);
-        }
-      }
+      wg_pp_oloc(m, s, q, pfq, wq, t0, tn, [&](int u, int v) -> double { return Ms[(size_t)v * N + u]; });
     }
     if (bond_wave && wid < nbw) {
       double* va = s.u + SweepSmem<NMAX, LDSMAT, RL>::MAT + (size_t)wid * 2 * NMAX;

@@ -5,7 +5,9 @@
 ! W.input (the reference draws W from a clock-seeded stream when W.input is
 ! absent, parameter.f90:21-41; this driver requires the file).
 !
-!   rbmpp_vmc [walkers [s_precision [recompute_k [projection [mode]]]]]
+!   rbmpp_vmc [walkers [s_precision [recompute_k [projection [mode [seed]]]]]]
+!
+! seed is the key of the walkers' Philox streams (default 1).
 !
 ! projection 1 runs the transverse_code variant: its spin-projection weights
 ! (transverse_code/spin_projection.f90:30-38) and its reweighting-offset
@@ -44,11 +46,12 @@ program rbmpp_vmc
   integer(c_int) :: rc
   integer(c_int64_t) :: naccept
   integer :: walkers, nspw, prec, kre, proj, mode, it, i, nv, nav_chk
+  integer(c_int64_t) :: seed
   real(8) :: shift_factor
   real(8) :: etot, etot2, norm, wf_av, shift, shift_av, wf_av_list(10), acc, e_av, e_er, v_av, v_er
   character(len=64) :: arg
 
-  walkers = 4096; prec = RBMPP_S_FP64; kre = 0; proj = RBMPP_PROJ_NORMAL; mode = 0
+  walkers = 4096; prec = RBMPP_S_FP64; kre = 0; proj = RBMPP_PROJ_NORMAL; mode = 0; seed = 1_c_int64_t
   if (command_argument_count() >= 1) then
     call get_command_argument(1, arg); read(arg, *) walkers
   end if
@@ -63,6 +66,9 @@ program rbmpp_vmc
   end if
   if (command_argument_count() >= 5) then
     call get_command_argument(5, arg); read(arg, *) mode
+  end if
+  if (command_argument_count() >= 6) then
+    call get_command_argument(6, arg); read(arg, *) seed
   end if
   shift_factor = merge(0.003d0, 0.001d0, proj == RBMPP_PROJ_TRANSVERSE)
 
@@ -79,7 +85,7 @@ program rbmpp_vmc
   cfg%n_slater = nsl; cfg%spin_parity = spin_parity; cfg%J1 = J1; cfg%J2 = J2
   cfg%n_walkers = walkers; cfg%n_samples = nspw; cfg%n_warmup = -1; cfg%n_warmup_first = -1
   cfg%n_update = 0; cfg%recompute_k = kre; cfg%s_precision = prec; cfg%fused_measure = 0
-  cfg%seed = 1_c_int64_t; cfg%p_restart = 0.05d0; cfg%p_global = 3d-4; cfg%cond_guard = 1d4
+  cfg%seed = seed; cfg%p_restart = 0.05d0; cfg%p_global = 3d-4; cfg%cond_guard = 1d4
   rc = rbmpp_create(cfg, w_map, chi, orb, h)
   call check(rc, "rbmpp_create")
   nv = rbmpp_nv(h)

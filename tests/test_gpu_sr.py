@@ -145,3 +145,36 @@ def test_mixed_step_solved_on_fp64_operator_10x10(prec):
     print(f"{prec}: refined against the assembled S: step error {err_a:.2e}", flush=True)
     assert ra["cg_iters"] == 0
     assert err_a > 100 * err
+
+
+@pytest.mark.parametrize("L,ar,ac,proj", [(10, 4, 0, "block"), (10, 4, 0, "exact"), (6, 2, 1, "block"),
+                                          (12, 2, 0, "block")])
+def test_measurement_run_to_run_identical(L, ar, ac, proj):
+    """Two measurements of the same samples give bit-identical O rows, S and
+    SR step: the pair-product O_k are summed per orbital by one owning thread
+    in a fixed order (no atomics), every reduction has a fixed order.  Block
+    form, the skew form with the reference's exact weights, a complex model,
+    and N > 104 (the global-scratch estimator)."""
+    from rbm_pp_amd import api
+    spec = lt.j1j2_model(L, alphar=ar, alphac=ac, J2=0.5)
+    p = lt.random_params(spec, seed_w=4, seed_f=5, sigma_w=0.05)
+    n = 64
+    xs = _configs(spec.N, n, seed=900 + L)
+    eng = api.Engine(spec, params=p, n_walkers=1, n_samples=n)
+    if proj == "exact":
+        eng.set_projection(api.PROJ_NORMAL_EXACT)
+    out = []
+    for rep in range(2):
+        eng.set_gamma(p.gamma())
+        eng.set_samples(xs)
+        r = eng.calc_phys_quantity_opt(shift=1e-3)
+        S = eng.smat()
+        g, _ = eng.sr_step(p.gamma(), s_factor0=1e-3)
+        loc = eng.local(xs[:16], want_o=True)
+        out.append((r, S, g, loc))
+    (r0, S0, g0, l0), (r1, S1, g1, l1) = out
+    assert r0["Etot"] == r1["Etot"] and r0["norm"] == r1["norm"]
+    for k in ("psi", "eloc", "oloc"):
+        np.testing.assert_array_equal(l0[k], l1[k])
+    np.testing.assert_array_equal(S0, S1)
+    np.testing.assert_array_equal(g0, g1)
