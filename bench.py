@@ -210,11 +210,17 @@ def energy_run(args):
                              str(args.energy_seed)], cwd=rundir,
                             stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     it = 0
+    sr_res, sr_cg, sr_boost = [], [], []  # per iteration: the SR report of the step
     for line in proc.stdout:
         if line.startswith("Iteration"):
             it = line.split(":")[-1].strip()
         elif "Total Energy" in line or "total energy" in line:
             log(f"iteration {it}: E/N = {float(line.split(':')[-1]) / 4 / N:.6f}")
+        elif "SR residual, CG, boost" in line:
+            f = line.split(":")[-1].split()
+            sr_res.append(float(f[0])); sr_cg.append(int(f[1])); sr_boost.append(float(f[2]))
+            if sr_boost[-1] > 0 or sr_res[-1] > 1e-11:
+                log(f"iteration {it}: SR residual {sr_res[-1]:.2e}, {sr_cg[-1]} CG iterations, boost {sr_boost[-1]:.2e}")
         elif "failed" in line or "error" in line.lower():
             log(f"driver: {line.rstrip()}")
     rc = proc.wait()
@@ -229,6 +235,10 @@ def energy_run(args):
                       "samples_per_iteration": args.walkers * nspw,
                       "energy_per_site": e_av / 4 / N, "error_per_site": e_err / 4 / N,
                       "history_per_site": [float(v) / 4 / N for v in hist[:, 1]],
+                      "sr_residual_max": max(sr_res) if sr_res else None,
+                      "sr_cg_iters_max": max(sr_cg) if sr_cg else None,
+                      "sr_boosted_steps": sum(1 for b in sr_boost if b > 0),
+                      "sr_cg_iters": sr_cg, "sr_residual": sr_res,
                       "wall_s": time.perf_counter() - t0}))
 
 

@@ -156,7 +156,11 @@ static DevModel devmodel(const rbmpp_handle* h) {
   m.L = h->L; m.N = h->N; m.P = h->P; m.C = h->C; m.ac = h->ac; m.ar = h->ar;
   m.npair = h->npair; m.nsl = h->nsl; m.nv = h->nv; m.nv_rbm = h->nv_rbm; m.mc = h->mc; m.mr = h->mr;
   m.real_only = h->real_only;
-  m.block = (h->sp_cs[0] == 0.0 && h->sp_cs[1] == 0.0 && h->N <= RBMPP_NLDS && h->C == RBMPP_CMAX) ? 1 : 0;
+  // block form: the sampler keeps F^-1 per copy (N <= 104 with every
+  // sampler variant; up to N = 256 without the fused measurement, which
+  // exists in skew form only there)
+  m.block = (h->sp_cs[0] == 0.0 && h->sp_cs[1] == 0.0 && h->C == RBMPP_CMAX &&
+             (h->N <= RBMPP_NLDS || (h->N <= 256 && !h->cfg.fused_measure))) ? 1 : 0;
   m.J1 = h->cfg.J1; m.J2 = h->cfg.J2;
   for (int s = 0; s < 2; s++) {
     m.sp_cs[s] = h->sp_cs[s]; m.sp_cc[s] = h->sp_cc[s]; m.sp_ss[s] = h->sp_ss[s]; m.sp_w[s] = h->sp_w[s];
@@ -595,7 +599,7 @@ static int run_local(rbmpp_handle* h, int n, const signed char* d_x, int mode, d
   int cap = grid_cap > 0 ? grid_cap : h->local_grid;
   if (grid_cap <= 0 && h->tune_meas_grid > 0) cap = std::min(cap, h->tune_meas_grid);
   const DevModel hm0 = devmodel(h);
-  if (hm0.block) {
+  if (hm0.block && hm0.N <= RBMPP_NLDS) {
     // the block-form estimator keeps F^-1 of every copy per workgroup
     cap = std::min(cap, BLK_GRID);
     if (!h->d_bscr && (rc = dalloc(h, &h->d_bscr, (size_t)BLK_GRID * h->C * (h->N / 2) * (h->N / 2)))) return rc;
@@ -1294,8 +1298,8 @@ int rbmpp_local_noalloc(rbmpp_handle* h, int32_t n, const int32_t* x, double* el
   // per-workgroup scratch of the estimator that will run: F^-1 of every copy
   // (block form) or the N x N recompute matrix (N > RBMPP_NLDS)
   const size_t np = (size_t)h->npair;
-  const size_t wg = hm.block ? (size_t)h->C * np * np * sizeof(double)
-                             : (N > RBMPP_NLDS ? (size_t)N * N * sizeof(double) : 0);
+  const size_t wg = (hm.block && N <= RBMPP_NLDS) ? (size_t)h->C * np * np * sizeof(double)
+                                                  : (N > RBMPP_NLDS ? (size_t)N * N * sizeof(double) : 0);
   const int gcap = wg ? (int)std::min<size_t>(1024, NOALLOC_WG_BYTES / wg) : 1024;
   // per configuration: x as int32 and int8, psi and eloc (complex), rw, status
   const size_t per_cfg = (size_t)N * 5 + 2 * sizeof(double2) + sizeof(double) + sizeof(int);

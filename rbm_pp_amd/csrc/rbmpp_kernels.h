@@ -96,7 +96,7 @@ int launch_sweep(const DevModel& hm, const SweepArgs& a, hipStream_t s);
 
 // one entry of a multi-model sampler launch: the model's sweep arguments, the
 // first workgroup of its walkers and the kernel specialisation it needs
-enum { SWEEP_BLK64 = 0, SWEEP_BLK128 = 1, SWEEP_SKEW256 = 2 };
+enum { SWEEP_BLK64 = 0, SWEEP_BLK128 = 1, SWEEP_SKEW256 = 2, SWEEP_BLK256 = 3 };
 struct SweepBatchEntry {
   SweepArgs a;
   int first_block;

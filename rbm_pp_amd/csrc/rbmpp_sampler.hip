@@ -118,6 +118,118 @@ struct SweepSmem {
 };
 
 
+// det F and F^-1 of one copy in block form for N > 104 (np up to 128, where
+// a row per lane does not fit the registers), by the whole workgroup with F
+// in LDS: T(u, d) = F(u, d) = Slater[site_u][site_(np+d) + N], then
+// Gauss-Jordan exchange steps on the pivots (p_k, k), p_k the unused row with
+// the largest |T(., k)| (smallest index on ties), as the per-wave elimination
+// of rbmpp_block.h: pivot search by one wave, then the pivot row and column
+// staged and the whole matrix updated, three barriers a step.  Afterwards
+// F^-1(k, u) = T(perm[k], pinv[u]), written to Bout row-major.  det F is the
+// product of the pivots times the sign of the row order.  Returns 0 for an
+// exactly singular pivot (every thread).  With drift, drift[0] and drift[1]
+// get the largest |new - old| and |new| over Bout (every thread).
+// LDS: T np^2 doubles, rowb and colb np doubles each, used, perm, pinv np ints
+// each, red 32 doubles, s_p / s_pv / s_cnt one value each.
+__device__ __attribute__((noinline)) double wg_block_inverse_lds(const double* __restrict__ S, int N, int np,
+                                                                 const int* eidx, double* T, double* rowb,
+                                                                 double* colb, int* used, int* perm, int* pinv,
+                                                                 double* red, int* s_p, double* s_pv, int* s_cnt,
+                                                                 double* Bout, double* drift) {
+  const int tid = threadIdx.x, nth = blockDim.x, lane = tid & (RBMPP_WAVE - 1), wid = tid >> 6;
+  const int npp = np * np;
+  for (int e = tid; e < npp; e += nth) {
+    const int u = e / np, d = e - u * np;
+    T[e] = S[(size_t)eidx[u] * 2 * N + eidx[np + d] + N];
+  }
+  for (int r = tid; r < np; r += nth) used[r] = 0;
+  __syncthreads();
+  double det = 1.0;
+  int neg = 0;
+  for (int k = 0; k < np; k++) {
+    if (wid == 0) {
+      // rows lane and lane + 64: the unused one with the largest |T(., k)|
+      double bv = -1.0;
+      int bi = np;
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const int r = lane + RBMPP_WAVE * h;
+        if (r < np && !used[r]) {
+          const double v = fabs(T[r * np + k]);
+          if (v > bv) { bv = v; bi = r; }
+        }
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(bv, o, RBMPP_WAVE);
+        const int oi = __shfl_xor(bi, o, RBMPP_WAVE);
+        if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+      }
+      // unused rows below the pivot: the sign of the row order
+      const int r0 = lane, r1 = lane + RBMPP_WAVE;
+      const int below = __popcll(__ballot(r0 < np && r0 < bi && !used[r0])) +
+                        __popcll(__ballot(r1 < np && r1 < bi && !used[r1]));
+      if (lane == 0) {
+        if (bi < np) {
+          *s_p = bi;
+          *s_pv = T[bi * np + k];
+          *s_cnt = below;
+          used[bi] = 1;
+          perm[k] = bi;
+        } else {  // no finite candidate (NaN entries): reported as singular
+          *s_p = 0;
+          *s_pv = 0.0;
+          *s_cnt = 0;
+        }
+      }
+    }
+    __syncthreads();
+    const int p = *s_p;
+    const double pv = *s_pv;
+    if (pv == 0.0) return 0.0;  // the same value in every thread
+    det *= pv;
+    neg ^= *s_cnt & 1;
+    for (int c = tid; c < np; c += nth) rowb[c] = T[p * np + c];
+    for (int r = tid; r < np; r += nth) colb[r] = T[r * np + k];
+    __syncthreads();
+    const double inv = 1.0 / pv;
+    // other rows: T - (T(., k) / pv) * pivot row, column k -> -T(., k) / pv;
+    // the pivot row: / pv, column k -> 1 / pv
+    for (int e = tid; e < npp; e += nth) {
+      const int r = e / np, c = e - r * np;
+      double v;
+      if (r == p) v = c == k ? inv : rowb[c] * inv;
+      else v = c == k ? -(colb[r] * inv) : fma(-(colb[r] * inv), rowb[c], T[e]);
+      T[e] = v;
+    }
+    __syncthreads();
+  }
+  for (int k = tid; k < np; k += nth) pinv[perm[k]] = k;
+  __syncthreads();
+  double ldiff = 0.0, lmax = 0.0;
+  for (int e = tid; e < npp; e += nth) {
+    const int k = e / np, u = e - k * np;
+    const double v = T[perm[k] * np + pinv[u]];
+    if (drift) {
+      ldiff = fmax(ldiff, fabs(Bout[e] - v));
+      lmax = fmax(lmax, fabs(v));
+    }
+    Bout[e] = v;
+  }
+  if (drift) {
+    ldiff = wave_max(ldiff);
+    lmax = wave_max(lmax);
+    if (lane == 0) { red[2 * wid] = ldiff; red[2 * wid + 1] = lmax; }
+    __syncthreads();
+    double a = 0.0, b = 0.0;
+    for (int w = 0; w < (nth >> 6); w++) { a = fmax(a, red[2 * w]); b = fmax(b, red[2 * w + 1]); }
+    drift[0] = a;
+    drift[1] = b;
+    __syncthreads();  // red is reused by the caller
+  }
+  return neg ? -det : det;
+}
+
 template <int NMAX, bool LDSMAT, int RL>
 __device__ __attribute__((noinline)) void wg_recompute(const DevModel& m, SweepSmem<NMAX, LDSMAT, RL>& s, double* Mw,
                              double* gscratch, int check, double* drift_pf, double* drift_inv) {
@@ -164,6 +276,31 @@ __device__ __attribute__((noinline)) void wg_recompute(const DevModel& m, SweepS
     if (s.status) return;
     if (check)
       for (int q = 0; q < C; q++) dinv = fmax(dinv, s.red[q]);
+  } else if (NMAX > 128 && m.block) {
+    // block form at N > 104: one copy at a time with F in LDS (s.u holds
+    // (NMAX / 2)^2 doubles); pivot row and column in gjbuf, then the index
+    // arrays
+    constexpr int HM = NMAX / 2;
+    static_assert(SweepSmem<NMAX, LDSMAT, RL>::UNION >= HM * HM && 4 * NMAX >= 2 * HM + (3 * HM + 1) / 2,
+                  "F of the largest copy fits u, its pivot buffers gjbuf");
+    int* ib = (int*)(s.gjbuf + 2 * HM);
+    const int np = m.npair;
+    const double pfsign = ((np * (np - 1) / 2) & 1) ? -1.0 : 1.0;
+    for (int q = 0; q < C; q++) {
+      double dr[2] = {0.0, 0.0};
+      const double det = wg_block_inverse_lds(m.se + (size_t)q * 4 * N * N, N, np, s.eidx, s.u, s.gjbuf,
+                                              s.gjbuf + HM, ib, ib + HM, ib + 2 * HM, s.red, &s.gsel[0],
+                                              &s.gselv[0], &s.gsel[1], Mw + (size_t)q * fold_size(N),
+                                              check ? dr : nullptr);
+      if (tid == 0) {
+        s.pfn[q] = s.pf[q];  // old value, for the drift measure
+        s.pf[q] = pfsign * det;
+        if (det == 0.0) s.status = WS_SINGULAR;
+      }
+      if (check) dinv = fmax(dinv, dr[1] > 0.0 ? dr[0] / dr[1] : 0.0);
+      __syncthreads();
+      if (s.status) return;
+    }
   } else if constexpr (!SLIM)
   for (int q = 0; q < C; q++) {
     const double* S = m.se + (size_t)q * 4 * N * N;
@@ -540,6 +677,25 @@ __device__ __attribute__((noinline)) void blk_update_pass(double* Mq, int np, in
   }
 }
 
+// The same for np up to 128 (N > 104): lane l holds columns 2l and 2l + 1
+// of every row (one 16-byte access); cv, al, be per column likewise.
+__device__ __attribute__((noinline)) void blk_update_pass2(double* Mq, int np, int lane, double2 cv, int ua,
+                                                           double2 al, double2 be) {
+  ua = __builtin_amdgcn_readfirstlane(ua);
+  const __amdgpu_buffer_rsrc_t rs = block_rsrc(Mq, np * np * 8);
+  const int vo = 2 * lane < np ? 16 * lane : RBMPP_OOB;
+  const int rowb = 8 * np, ul = ua >> 1;
+  const bool uy = ua & 1;
+  for (int d = 0; d < np; d++) {
+    double2 g = bload2(rs, vo, d * rowb);
+    const double xd = readlane_d(wave_sum_to63(fma(g.y, cv.y, g.x * cv.x)), 63);
+    const double cd = readlane_d(uy ? g.y : g.x, ul);
+    g.x = fma(cd, al.x, fma(xd, be.x, g.x));
+    g.y = fma(cd, al.y, fma(xd, be.y, g.y));
+    bstore2(g, rs, vo, d * rowb);
+  }
+}
+
 // The pending update of one row g = B[d][lane] (x_d by a wave sum), and of
 // four rows with their four wave sums taken together.
 __device__ __forceinline__ double blk_upd1(double g, double cv, int ua, double al, double be) {
@@ -690,6 +846,13 @@ __device__ __forceinline__ void sweep_walker(const SweepArgs& a, const int w, Sw
   bool pend = false;
   int pua = 0;
   double pal = 0.0, pbe = 0.0, pcv = 0.0;
+  // N > 104 (NMAX = 256): two columns per lane
+  double2 pal2 = make_double2(0.0, 0.0), pbe2 = make_double2(0.0, 0.0), pcv2 = make_double2(0.0, 0.0);
+  // the pending update applied by a pass of its own
+  auto flush_pending = [&]() {
+    if constexpr (NMAX > 128) blk_update_pass2(Mq, m.npair, lane, pcv2, pua, pal2, pbe2);
+    else blk_update_pass(Mq, m.npair, lane, pcv, pua, pal, pbe);
+  };
   // resident rows: breg (registers), lres (LDS, row d - RR at (d - RR) * HS);
   // the host launches this form only with npair >= RR + RL
   double breg[RR > 0 ? RR : 1];
@@ -744,7 +907,7 @@ __device__ __forceinline__ void sweep_walker(const SweepArgs& a, const int w, Sw
       if constexpr (BLK) {
         // the drift check compares with the fast-updated F^-1
         if constexpr (RES) res_flush();
-        else if (pend && q < C) blk_update_pass(Mq, m.npair, lane, pcv, pua, pal, pbe);
+        else if (pend && q < C) flush_pending();
         pend = false;
       }
       double dpf = 0.0, dinv = 0.0;
@@ -810,6 +973,7 @@ __device__ __forceinline__ void sweep_walker(const SweepArgs& a, const int w, Sw
       double rq = 1.0;  // this copy's Pf ratio
       // block form: y = r^T F^-1 (lane u), row db of F^-1, c(u), and the scalars
       double by = 0.0, brow = 0.0, bcv = 0.0, bgam = 0.0, bdel = 0.0, beps = 0.0, bbet = 0.0;
+      double2 by2 = make_double2(0.0, 0.0), brow2 = make_double2(0.0, 0.0), bcv2 = make_double2(0.0, 0.0);
       int bua = 0;
       if constexpr (BLK) {
         if (q < C) {
@@ -820,6 +984,7 @@ __device__ __forceinline__ void sweep_walker(const SweepArgs& a, const int w, Sw
           // c(u) = F'(u, db) - F(u, db) (c(ua) = 0), and with B = F^-1 and
           // y = r^T B the ratio of pfupdate_two.f90:27-88 becomes
           // det F'/det F = (1 + y_ua)(1 + B[db].c) - (y.c) B[db][ua].
+          if constexpr (NMAX <= 128) {
           const int np = m.npair;
           const bool aup = msa < np;
           bua = __builtin_amdgcn_readfirstlane(aup ? msa : msb);
@@ -995,6 +1160,95 @@ __device__ __forceinline__ void sweep_walker(const SweepArgs& a, const int w, Sw
             bdel = readlane_d(v, 31);
             beps = readlane_d(v, 63);
           }
+          } else {
+          // N > 104 (np up to 128): lane l holds columns 2l and 2l + 1 of F^-1
+          // (one 16-byte load per row), r(d) and c(u) likewise; the same
+          // algebra, with rows read from the global Slater table
+          const int np = m.npair;
+          const bool aup = msa < np;
+          bua = __builtin_amdgcn_readfirstlane(aup ? msa : msb);
+          const int db = __builtin_amdgcn_readfirstlane((aup ? msb : msa) - np);
+          const int s1 = aup ? i1 : i2, s2 = aup ? i2 : i1;
+          const double* S1 = Sq + (size_t)s1 * 2 * N;
+          const double* S2 = Sq + (size_t)s2 * 2 * N;
+          const double* T1 = Sq + (size_t)(s1 + N) * 2 * N;
+          const double* T2 = Sq + (size_t)(s2 + N) * 2 * N;
+          const bool live = 2 * lane < np;  // np = N / 2 is even
+          double2 rv2 = make_double2(0.0, 0.0);
+          if (live) {
+            const int e0 = 2 * lane, e1 = e0 + 1;
+            const int sd0 = s.eidx[np + e0], sd1 = s.eidx[np + e1];
+            const int su0 = s.eidx[e0], su1 = s.eidx[e1];
+            const double rdb = S2[s1 + N] - S1[s2 + N];
+            rv2.x = e0 == db ? rdb : S2[sd0 + N] - S1[sd0 + N];
+            rv2.y = e1 == db ? rdb : S2[sd1 + N] - S1[sd1 + N];
+            bcv2.x = e0 == bua ? 0.0 : T2[su0] - T1[su0];
+            bcv2.y = e1 == bua ? 0.0 : T2[su1] - T1[su1];
+          }
+          double* rt = wv;  // r(d) for every d (zero past np)
+          *(double2*)&rt[2 * lane] = rv2;
+          wave_sync();
+          const __amdgpu_buffer_rsrc_t rs = block_rsrc(Mq, np * np * 8);
+          const int vo = live ? 16 * lane : RBMPP_OOB;
+          const int rowb = 8 * np, pl = pua >> 1;
+          const bool py = pua & 1;
+          double2 y = make_double2(0.0, 0.0);
+          constexpr int UB = 8;  // rows in flight
+          int d = 0;
+          for (; d + UB <= np; d += UB) {
+            double2 g[UB];
+#pragma unroll
+            for (int u = 0; u < UB; u++) g[u] = bload2(rs, vo, (d + u) * rowb);
+            if (pend) {
+#pragma unroll
+              for (int u = 0; u < UB; u += 4) {
+                const double v = wave_sum4(fma(g[u].y, pcv2.y, g[u].x * pcv2.x),
+                                           fma(g[u + 1].y, pcv2.y, g[u + 1].x * pcv2.x),
+                                           fma(g[u + 2].y, pcv2.y, g[u + 2].x * pcv2.x),
+                                           fma(g[u + 3].y, pcv2.y, g[u + 3].x * pcv2.x));
+                const double xd[4] = {readlane_d(v, 15), readlane_d(v, 47), readlane_d(v, 31), readlane_d(v, 63)};
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                  const double cd = readlane_d(py ? g[u + j].y : g[u + j].x, pl);
+                  g[u + j].x = fma(cd, pal2.x, fma(xd[j], pbe2.x, g[u + j].x));
+                  g[u + j].y = fma(cd, pal2.y, fma(xd[j], pbe2.y, g[u + j].y));
+                  bstore2(g[u + j], rs, vo, (d + u + j) * rowb);
+                }
+              }
+            }
+#pragma unroll
+            for (int u = 0; u < UB; u++) {
+              const double rd = rt[d + u];
+              y.x = fma(rd, g[u].x, y.x);
+              y.y = fma(rd, g[u].y, y.y);
+            }
+          }
+          for (; d < np; d++) {
+            double2 g = bload2(rs, vo, d * rowb);
+            if (pend) {
+              const double xd = readlane_d(wave_sum_to63(fma(g.y, pcv2.y, g.x * pcv2.x)), 63);
+              const double cd = readlane_d(py ? g.y : g.x, pl);
+              g.x = fma(cd, pal2.x, fma(xd, pbe2.x, g.x));
+              g.y = fma(cd, pal2.y, fma(xd, pbe2.y, g.y));
+              bstore2(g, rs, vo, d * rowb);
+            }
+            const double rd = rt[d];
+            y.x = fma(rd, g.x, y.x);
+            y.y = fma(rd, g.y, y.y);
+          }
+          // row db as updated above (each lane reads back its own store)
+          brow2 = bload2(rs, vo, db * rowb);
+          pend = false;
+          by2 = y;
+          const int bl = bua >> 1;
+          bgam = readlane_d((bua & 1) ? y.y : y.x, bl);
+          bbet = readlane_d((bua & 1) ? brow2.y : brow2.x, bl);
+          {
+            const double v = wave_sum2(fma(brow2.y, bcv2.y, brow2.x * bcv2.x), fma(y.y, bcv2.y, y.x * bcv2.x));
+            bdel = readlane_d(v, 31);
+            beps = readlane_d(v, 63);
+          }
+          }
           const double ratio = (1.0 + bgam) * (1.0 + bdel) - beps * bbet;
           rq = ratio;
           const double pfn = ratio * s.pf[q];
@@ -1162,9 +1416,17 @@ __device__ __forceinline__ void sweep_walker(const SweepArgs& a, const int w, Sw
             // be = -((1 + y_ua) B[db] - B[db][ua] y) / R,
             // left pending for the next pass over F^-1
             const double ia = -1.0 / rq;
-            pal = ia * ((1.0 + bdel) * by - beps * brow);
-            pbe = ia * ((1.0 + bgam) * brow - bbet * by);
-            pcv = bcv;
+            if constexpr (NMAX > 128) {
+              pal2 = make_double2(ia * ((1.0 + bdel) * by2.x - beps * brow2.x),
+                                  ia * ((1.0 + bdel) * by2.y - beps * brow2.y));
+              pbe2 = make_double2(ia * ((1.0 + bgam) * brow2.x - bbet * by2.x),
+                                  ia * ((1.0 + bgam) * brow2.y - bbet * by2.y));
+              pcv2 = bcv2;
+            } else {
+              pal = ia * ((1.0 + bdel) * by - beps * brow);
+              pbe = ia * ((1.0 + bgam) * brow - bbet * by);
+              pcv = bcv;
+            }
             pua = bua;
             pend = true;
           } else {
@@ -1334,7 +1596,7 @@ __device__ __forceinline__ void sweep_walker(const SweepArgs& a, const int w, Sw
       if (tid == 0) a.psis[row] = s.psi;
       if constexpr (LDSMAT && MEAS) {
         if constexpr (BLK) {
-          if (pend && q < C) blk_update_pass(Mq, m.npair, lane, pcv, pua, pal, pbe);
+          if (pend && q < C) flush_pending();
           pend = false;
           __syncthreads();  // every copy's F^-1 is current
         }
@@ -1344,7 +1606,7 @@ __device__ __forceinline__ void sweep_walker(const SweepArgs& a, const int w, Sw
   }
   if constexpr (RES) res_flush();
   else if constexpr (BLK) {
-    if (pend && q < C) blk_update_pass(Mq, m.npair, lane, pcv, pua, pal, pbe);
+    if (pend && q < C) flush_pending();
   }
   __syncthreads();
   // ---- write the walker state back
@@ -1403,6 +1665,7 @@ __global__ __launch_bounds__(1024) void k_sweep_batch(const SweepBatchEntry* __r
   switch (ent[k].kind) {
     case SWEEP_BLK64: sweep_walker<64, true, false, true>(a, w, *reinterpret_cast<SweepSmem<64, true>*>(smem)); break;
     case SWEEP_BLK128: sweep_walker<128, true, false, true>(a, w, *reinterpret_cast<SweepSmem<128, true>*>(smem)); break;
+    case SWEEP_BLK256: sweep_walker<256, false, false, true>(a, w, *reinterpret_cast<SweepSmem<256, false>*>(smem)); break;
     default: sweep_walker<256, false, false, false>(a, w, *reinterpret_cast<SweepSmem<256, false>*>(smem)); break;
   }
 }
@@ -1410,11 +1673,11 @@ __global__ __launch_bounds__(1024) void k_sweep_batch(const SweepBatchEntry* __r
 int sweep_kind(const DevModel& hm) {
   const int N = hm.N;
   if (hm.C > RBMPP_CMAX || hm.mr > RBMPP_MRMAX || hm.mc > RBMPP_MCMAX || (N & 1)) return -1;
-  const bool blk = hm.block && hm.C == RBMPP_CMAX && N <= RBMPP_NLDS;
+  const bool blk = hm.block && hm.C == RBMPP_CMAX;
   if (N <= RBMPP_NLDS && !blk) return -1;  // skew form at N <= 104: not in the batch kernel
   if (N <= 64) return SWEEP_BLK64;
   if (N <= RBMPP_NLDS) return SWEEP_BLK128;
-  if (N <= 256) return SWEEP_SKEW256;
+  if (N <= 256) return blk ? SWEEP_BLK256 : SWEEP_SKEW256;
   return -1;
 }
 
@@ -1460,8 +1723,8 @@ int launch_sweep(const DevModel& hm, const SweepArgs& a, hipStream_t st) {
   const int N = hm.N;
   if (hm.C > RBMPP_CMAX || hm.mr > RBMPP_MRMAX || hm.mc > RBMPP_MCMAX || (N & 1)) return -1;
   if (N > RBMPP_NLDS && !a.ws.scratch) return -1;
-  // block form needs the register recompute (16 waves, N <= RBMPP_NLDS)
-  const bool blk = hm.block && hm.C == RBMPP_CMAX && N <= RBMPP_NLDS;
+  // block form: per-wave register recompute at N <= RBMPP_NLDS, F in LDS above
+  const bool blk = hm.block && hm.C == RBMPP_CMAX;
   using RS = SweepSmem<RES_NMAX, true, RES_RL>;
   const bool slim = blk && !a.measure && N <= RES_NMAX && hm.C * hm.ar <= RS::MRL && hm.C * hm.ac <= RS::MCL;
   if (slim && hm.npair >= RES_RR + RES_RL && resident_enabled()) {
@@ -1474,7 +1737,13 @@ int launch_sweep(const DevModel& hm, const SweepArgs& a, hipStream_t st) {
   }
   if (N <= 64) launch_sweep_t<64, true>(hm.C, blk, a, st);
   else if (N <= RBMPP_NLDS) launch_sweep_t<128, true>(hm.C, blk, a, st);
-  else if (N <= 256) launch_sweep_t<256, false>(hm.C, false, a, st);
-  else return -1;
+  else if (N <= 256) {
+    if (blk) {
+      if (a.measure) return -1;  // the host keeps the skew form with the fused measurement
+      hipLaunchKernelGGL((k_sweep<256, false, false, true>), dim3(a.nwalk), dim3(64 * hm.C), 0, st, a);
+    } else {
+      launch_sweep_t<256, false>(hm.C, false, a, st);
+    }
+  } else return -1;
   return 0;
 }

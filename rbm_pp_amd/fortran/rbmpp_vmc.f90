@@ -124,6 +124,7 @@ program rbmpp_vmc
     write(6, '(a,F20.10)')  '   Energy Variance      :', (etot2 - etot*etot) / (etot*etot)
     write(6, '(a,F20.10)')  '   Acceptance Ratio     :', acc
     write(6, '(a,2F20.10)') '   Stabilization Factor :', rep%s_factor, s_cut
+    write(6, '(a,E12.3,I6,E12.3)') '   SR residual, CG, boost:', rep%residual, rep%cg_iters, rep%precond_boost
     write(6, '(a,2E20.5)')  '   wf_av, shift         :', wf_av, shift
     write(6, '(a,F20.10)')  '   norm                 :', norm / dble(walkers * nspw)
     write(10, *) it, etot, (etot2 - etot*etot) / (etot*etot)

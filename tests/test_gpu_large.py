@@ -98,16 +98,21 @@ def test_local_energy_and_ok_large(oracle_lib, L, ar, J2):
         assert _relerr(got["oloc"][s], o) < RTOL
 
 
-@pytest.mark.parametrize("L,ar,J2,K", [(12, 2, 0.5, 32), (14, 2, 0.4, 0), (16, 16, 0.55, 32),
-                                       (16, 16, 0.55, 0)])
-def test_sampler_trajectory_large(oracle_lib, L, ar, J2, K):
+@pytest.mark.parametrize("L,ar,J2,K,proj", [(12, 2, 0.5, 32, "block"), (14, 2, 0.4, 0, "block"),
+                                            (16, 16, 0.55, 32, "block"), (16, 16, 0.55, 0, "block"),
+                                            (12, 2, 0.5, 32, "exact"), (16, 16, 0.55, 0, "exact")])
+def test_sampler_trajectory_large(oracle_lib, L, ar, J2, K, proj):
     """Two walkers, two calls, against the oracle chain on the same Philox
     streams; configurations bit-exact, X^-1 after the launch entrywise
-    against a recompute in the walker's electron order."""
+    against a recompute in the walker's electron order.  proj "block": the
+    default normal projection (F^-1 per copy, two columns per lane, F
+    inverted in LDS); "exact": the reference's weights (folded skew X^-1)."""
     api = _api()
     spec, p = _model(L, ar, J2, sigma_w=0.1)
     W, ns, nwarm, seed = 2, 2, 2, 4321
     eng = api.Engine(spec, params=p, n_walkers=W, n_samples=ns, seed=seed, n_warmup=nwarm, recompute_k=K)
+    if proj == "exact":
+        eng.set_projection(api.PROJ_NORMAL_EXACT)
     eng.make_sample(True, shift=0.0)
     x_gpu, psi_gpu = eng.get_samples()
     wk = eng.get_walkers(with_invm=True)
