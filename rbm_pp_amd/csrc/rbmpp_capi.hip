@@ -939,16 +939,31 @@ int rbmpp_sr_step(rbmpp_handle* h, const rbmpp_sr_config* c, double* gamma, rbmp
     if (fl[4] == 0 || !p.orows || fl[3] != 0 || attempt == 4) break;
     boost = (boost == 0.0 ? 10.0 : 10.0 * boost / sc[1]) * sc[1];
   }
-  // more conjugate-gradient iterations (in fours, up to 40) until the
-  // residual against the operator of the O rows is below 1e-11
+  // more conjugate-gradient iterations (in fours, up to 40 per round) until
+  // the residual against the operator of the O rows is below 1e-13.  The
+  // recurrence's residual can drift from D g - A x over many iterations (a
+  // poor, boosted preconditioner of an ill-conditioned system), so each
+  // round ends by restarting from the true residual, which also gives lambda
+  // and the reported residual; up to four rounds.
   int pcg_iters = c->refine_iters;
+  const double tol = 1e-13;
+  auto rel_res = [&]() { return std::sqrt(sc[5] / std::max(sc[6], 1e-300)); };
   if (p.orows && fl[4] == 0 && fl[3] == 0)
-    while (std::sqrt(sc[5] / std::max(sc[6], 1e-300)) > 1e-11 && pcg_iters < c->refine_iters + 40) {
-      launch_pcg_iters(p, pcg_iters, pcg_iters + 4, h->st);
-      launch_pcg_final(p, h->st);
+    for (int round = 0; round < 4; round++) {
+      int k = round == 0 ? c->refine_iters : 0, k_end = k + 40;
+      while (rel_res() > tol && k < k_end) {
+        launch_pcg_iters(p, k, k + 4, h->st);
+        launch_pcg_final(p, h->st);
+        HIPCHK(h, hipGetLastError());
+        k += 4;
+        pcg_iters += 4;
+        if ((rc = read_scalars())) return rc;
+      }
+      launch_pcg_start(p, h->st);  // r = D g - A x
+      launch_pcg_final(p, h->st);  // A x = D g - r: lambda and the true residual
       HIPCHK(h, hipGetLastError());
-      pcg_iters += 4;
       if ((rc = read_scalars())) return rc;
+      if (rel_res() <= tol) break;
     }
   std::vector<double> x(nv);
   HIPCHK(h, hipMemcpyAsync(x.data(), p.x, sizeof(double) * nv, hipMemcpyDeviceToHost, h->st));

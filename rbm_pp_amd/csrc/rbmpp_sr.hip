@@ -696,7 +696,7 @@ __global__ void k_b_minus(int n, int nvr, const double* __restrict__ sc, const d
   if (i < n) ax[i] = (i >= nvr ? g[i] * sc[0] : g[i]) - r[i];
 }
 
-static void launch_pcg_start(const SrPipeline& p, hipStream_t st) {
+void launch_pcg_start(const SrPipeline& p, hipStream_t st) {
   const int n = p.nv, nb = (n + 255) / 256;
   launch_orow_op(p, p.x, p.ax, p.r, st);  // r = b - A x0
   hipLaunchKernelGGL(k_copy, dim3(nb), dim3(256), 0, st, n, (const double*)p.r, p.z);
