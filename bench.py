@@ -113,6 +113,26 @@ def synthetic_orbitals(L, nsl):
     return orb
 
 
+def sampler_form(projection, N, fused):
+    """True when the sampler keeps F^-1 per copy (block form): the normal
+    projection (every X = [[0, F], [-F^T, 0]]) up to N = 104 in every sampler
+    variant and up to N = 256 without the fused measurement; otherwise X^-1
+    in folded skew storage."""
+    return projection == "normal" and (N <= 104 or (N <= 256 and not fused))
+
+
+def sweep_bytes(block, C, N, acc):
+    """Algorithmic HBM bytes of one walker sweep (N proposals) at acceptance
+    rate acc.  Folded skew storage holds the N^2/2 independent entries of
+    each copy's X^-1: every proposal reads them once (8 C N^2/2 bytes) and an
+    accepted one reads and writes them once more, 4 C N^3 (1 + 2 acc).  In
+    block form the sampler keeps F^-1, (N/2)^2 entries per copy, and applies
+    an accepted move's update inside the next proposal's read pass: each
+    proposal reads them once and an accepted one writes them once,
+    2 C N^3 (1 + acc)."""
+    return 2.0 * C * N ** 3 * (1.0 + acc) if block else 4.0 * C * N ** 3 * (1.0 + 2.0 * acc)
+
+
 def refine_sweeps(name):
     """Refinement sweeps of the SR solve: one for FP64 S; two for a
     mixed-precision S, refined against the FP64 operator of the O rows."""
@@ -455,9 +475,18 @@ def main():
     if args.sampler_only:
         nacc = eng.make_sample(False, shift=0.0)
         t = eng.timings()
-        print(json.dumps({"kernel": "k_sweep", "walkers": args.walkers,
-                          "sweeps_per_walker": nwarm + 1 + nspw, "launch_s": t["sample"],
-                          "acceptance": nacc / (args.walkers * nspw * N)}))
+        acc = nacc / (args.walkers * nspw * N)
+        block = sampler_form(args.projection, N, args.fused_measure)
+        b_unit = sweep_bytes(block, Cc, N, acc)
+        gbs = b_unit * args.walkers * sweeps_per_launch_walker / t["sample"] / 1e9
+        print(json.dumps({"kernel": "k_sweep",
+                          "workload": f"j1j2_{args.L}x{args.L}_J2_{args.J2}_alphar{args.alphar}_c4v_parity",
+                          "walkers": args.walkers, "sweeps_per_walker": sweeps_per_launch_walker,
+                          "launch_s": t["sample"], "acceptance": acc,
+                          "samples_per_s": args.walkers * nspw / t["sample"],
+                          "layout": "F^-1 per copy (block form)" if block else "folded skew X^-1 per copy",
+                          "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                       "frac": gbs / HBM_PEAK_GBS, "bytes_per_walker_sweep": b_unit}}))
         return
 
     wf_list = [0.0] * 10
@@ -563,18 +592,9 @@ def main():
     acc = sum(r["nacc"] for r in recs) / samples / N
     sweeps_per_launch = args.walkers * sweeps_per_launch_walker
     t_sweep = sum(r["sweep_kernel"] for r in recs) / len(recs)
-    # algorithmic HBM bytes per sweep of one walker.  X^-1 of each copy is
-    # held as its N^2/2 independent entries (folded skew storage): each of
-    # the N proposals reads them once (8 C N^2/2 bytes) and an accepted one
-    # (fraction A) reads and writes them once more: 4 C N^3 (1 + 2A).
-    # In block form (normal projection: every X = [[0, F], [-F^T, 0]]) the
-    # sampler keeps F^-1, (N/2)^2 entries per copy, and applies an accepted
-    # move's update inside the next proposal's read pass: each proposal reads
-    # them once and an accepted one writes them once, 2 C N^3 (1 + A).
-    # the block form needs the register recompute, N <= 104; larger lattices
-    # keep the folded skew storage in every projection
-    block = args.projection == "normal" and N <= 104
-    b_unit = 2.0 * Cc * N ** 3 * (1.0 + acc) if block else 4.0 * Cc * N ** 3 * (1.0 + 2.0 * acc)
+    # algorithmic HBM bytes per sweep of one walker (sweep_bytes)
+    block = sampler_form(args.projection, N, args.fused_measure)
+    b_unit = sweep_bytes(block, Cc, N, acc)
     achieved = b_unit * sweeps_per_launch / t_sweep / 1e9
     # measured HBM bytes per walker sweep of this kernel and workload
     # (rocprofv3 FETCH_SIZE and WRITE_SIZE, each scaled by its reading for a

@@ -1,8 +1,10 @@
 """Parity of the N > 104 paths with the CPU oracle: lattices 12x12, 14x14
 and the 16x16 workload (J2/J1 = 0.55, alphar = 16, C4v + parity, N = 256).
-Above N = 104 the sampler keeps X^-1 of each copy in folded skew storage with
-its recompute matrix in global scratch (k_sweep<256, false>, wg_skew_inverse)
-and the estimator is the grid-strided k_local (rbmpp_local.hip).  Bounds as
+Above N = 104 the normal projection's sampler keeps F^-1 of each copy with
+two columns per lane and inverts F in LDS (k_sweep<256, true>); the
+reference's exact weights keep X^-1 in folded skew storage with the recompute
+matrix in global scratch (k_sweep<256, false>, wg_skew_inverse); the
+estimator is the grid-strided k_local (rbmpp_local.hip).  Bounds as
 in test_gpu_parity: FP64 values relative 1e-10 (normwise per matrix, scaled
 by the copy's condition where noted), trajectories bit-exact."""
 import numpy as np
