@@ -177,7 +177,8 @@ def sr_shape_bench(api, lt, L, alphar, J2, ns, nsl=None, seed=5):
         ach = f_s / r["t"]["s_build"] / 1e12
         d = dict(s_build_s=r["t"]["s_build"], solve_s=r["t"]["solve"], sr_step_s=r["t"]["sr_total"],
                  tflops=ach, peak_tflops=peaks[name], mfma_frac=ach / peaks[name],
-                 residual=r["rep"]["residual"],
+                 residual=r["rep"]["residual"], cg_iters=r["rep"]["cg_iters"],
+                 precond_boost=r["rep"]["precond_boost"],
                  step_rel_err=float(np.linalg.norm(r["step"] - base["step"]) / np.linalg.norm(base["step"])),
                  speedup_vs_fp64=base["t"]["sr_total"] / r["t"]["sr_total"],
                  refine_sweeps=refine_sweeps(name))
@@ -260,6 +261,71 @@ def energy_run(args):
                       "sr_boosted_steps": sum(1 for b in sr_boost if b > 0),
                       "sr_cg_iters": sr_cg, "sr_residual": sr_res,
                       "wall_s": time.perf_counter() - t0}))
+
+
+def step_compare(args):
+    """The mixed-precision SR steps against the FP64 step along one
+    optimisation of the bench workload, on the same samples: every iteration
+    samples and measures once (main.f90:323-370), forms the SR step from those
+    measurements with FP32 and split-BF16 x 3 S products and then with FP64 S
+    (SR_pdposv.f90), and continues with the FP64 step.  Reports per iteration
+    the mixed steps' relative error ||dg_mixed - dg_fp64|| / ||dg_fp64|| and
+    the energy change their difference makes to first order,
+    g . (gamma_mixed - gamma_fp64) with g the energy gradient of the same
+    samples (SR_pdposv.f90:4-6), per site in the units of E/N.  Prints one
+    JSON line."""
+    from rbm_pp_amd import api
+    from rbm_pp_amd import lattice as lt
+    spec = lt.j1j2_model(args.L, alphar=args.alphar, J2=args.J2)
+    p = lt.random_params(spec, seed_w=1, seed_f=2, sigma_w=0.01)
+    nspw = math.ceil(args.samples / args.walkers)
+    eng = api.Engine(spec, n_walkers=args.walkers, n_samples=nspw, seed=args.energy_seed,
+                     recompute_k=args.recompute_k, s_precision=api.S_FP64, params=p, cond_guard=args.cond_guard)
+    N = spec.N
+    log(f"step comparison: {args.L}x{args.L}, {args.step_compare} iterations, {args.walkers} walkers x {nspw} "
+        f"samples, nv={spec.nv}")
+    modes = (("fp32", api.S_FP32), ("bf16x3", api.S_BF16X3), ("fp64", api.S_FP64))
+    eng.make_sample(True, shift=0.0)
+    wf_list, shift, gam = [0.0] * 10, 0.0, eng.gamma()
+    hist = []
+    t0 = time.perf_counter()
+    for it in range(args.step_compare):
+        eng.make_sample(False, shift=shift)
+        r = eng.calc_phys_quantity_opt(shift=shift)
+        grad = r["gvec"] - 2.0 * r["Etot"] * r["ovec"]
+        res = {}
+        for name, prec in modes:  # FP64 last: its step is the one the engine keeps
+            eng.set_s_precision(prec)
+            g_new, rep = eng.sr_step(gam, s_factor0=1e-4, delta_tau=0.05, refine_iters=refine_sweeps(name))
+            res[name] = (g_new, rep, eng.timings()["sr_total"])
+        g64 = res["fp64"][0]
+        d64 = g64 - gam
+        row = {"E_per_site": r["Etot"] / 4 / N, "sr_fp64_s": res["fp64"][2]}
+        for name in ("fp32", "bf16x3"):
+            dm = res[name][0] - g64
+            row[name] = {"step_rel_err": float(np.linalg.norm(dm) / np.linalg.norm(d64)),
+                         "dE_per_site": float(grad @ dm) / 4 / N, "cg_iters": res[name][1]["cg_iters"],
+                         "boost": res[name][1]["precond_boost"], "sr_s": res[name][2]}
+        hist.append(row)
+        gam = g64
+        wf_list.pop(0)
+        wf_list.append(r["wf_av"])
+        shift = sum(wf_list) / 10.0 * 1e-3 / N / N
+        log(f"iteration {it}: E/N = {row['E_per_site']:.6f}, step error fp32 {row['fp32']['step_rel_err']:.1e} "
+            f"bf16x3 {row['bf16x3']['step_rel_err']:.1e}, dE/N bf16x3 {row['bf16x3']['dE_per_site']:.1e}")
+    out = {"step_compare": f"j1j2_{args.L}x{args.L}_J2_{args.J2}_alphar{args.alphar}", "nv": spec.nv,
+           "iterations": args.step_compare, "samples_per_iteration": args.walkers * nspw,
+           "E_per_site_first": hist[0]["E_per_site"], "E_per_site_last": hist[-1]["E_per_site"],
+           "wall_s": time.perf_counter() - t0}
+    for name in ("fp32", "bf16x3"):
+        out[name] = {"step_rel_err_max": max(h[name]["step_rel_err"] for h in hist),
+                     "abs_dE_per_site_max": max(abs(h[name]["dE_per_site"]) for h in hist),
+                     "cg_iters_max": max(h[name]["cg_iters"] for h in hist),
+                     "boosted_steps": sum(1 for h in hist if h[name]["boost"] > 0),
+                     "sr_s_mean": float(np.mean([h[name]["sr_s"] for h in hist]))}
+    out["fp64_sr_s_mean"] = float(np.mean([h["sr_fp64_s"] for h in hist]))
+    out["history"] = hist
+    print(json.dumps(out))
 
 
 def batch_run(args):
@@ -370,6 +436,9 @@ def main():
     ap.add_argument("--energy-nav", type=int, default=20)
     ap.add_argument("--energy-dir", default=None, help="run directory of --energy-run")
     ap.add_argument("--energy-seed", type=int, default=1, help="key of the walkers' Philox streams in --energy-run")
+    ap.add_argument("--step-compare", type=int, default=0, metavar="ITERS",
+                    help="an optimisation of ITERS iterations continuing with the FP64 step, each iteration's "
+                         "FP32 and split-BF16 x 3 steps compared with it on the same samples")
     ap.add_argument("--batch-run", type=int, default=0, metavar="ITERS",
                     help="the uneven multi-lattice workload (15 models in one sampler launch) for ITERS iterations")
     ap.add_argument("--batch-walkers", type=int, default=16384)
@@ -399,6 +468,9 @@ def main():
 
     if args.energy_run:
         energy_run(args)
+        return
+    if args.step_compare:
+        step_compare(args)
         return
     if args.batch_run:
         batch_run(args)
