@@ -134,9 +134,10 @@ def sweep_bytes(block, C, N, acc):
 
 
 def refine_sweeps(name):
-    """Refinement sweeps of the SR solve: one for FP64 S; two for a
-    mixed-precision S, refined against the FP64 operator of the O rows."""
-    return 1 if name == "fp64" else 2
+    """Refinement sweeps of the SR solve captured in its graph: one.  A
+    mixed-precision S is refined against the FP64 operator of the O rows
+    until the residual is below 1e-13, checked after every further sweep."""
+    return 1
 
 
 def sr_shape_bench(api, lt, L, alphar, J2, ns, nsl=None, seed=5):

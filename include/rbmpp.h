@@ -101,11 +101,13 @@ typedef struct {
 #define RBMPP_SR_NO_GRAPH 1
 /* With S assembled from FP32 or split-BF16 products, the system is solved by
    default with conjugate gradients on the FP64 operator of the stored O rows,
-   D (C^T C / norm) D + shift with C = O - kappa <O>^T (two passes over O per
+   D (C^T C / norm) D + shift with C = O - kappa <O>^T (one pass over O per
    iteration), preconditioned by the Cholesky factor of the assembled matrix:
-   refine_iters iterations inside the captured launch sequence, then more in
-   fours (up to 40) until the relative residual is below 1e-11, so that the
-   step is the FP64 system's solution.  The diagonal of a mixed S is formed in
+   refine_iters iterations inside the captured launch sequence, then more
+   (up to 40 per round) until the relative residual is below 1e-13;
+   each round ends with the true residual D g - A x, and a round that leaves
+   it above the tolerance restarts the recurrence from it (up to four
+   rounds), so that the step is the FP64 system's solution.  The diagonal of a mixed S is formed in
    FP64 from the O rows.  This flag refines against the assembled matrix
    instead (refine_iters sweeps). */
 #define RBMPP_SR_ASSEMBLED_REFINE 2

@@ -939,28 +939,33 @@ int rbmpp_sr_step(rbmpp_handle* h, const rbmpp_sr_config* c, double* gamma, rbmp
     if (fl[4] == 0 || !p.orows || fl[3] != 0 || attempt == 4) break;
     boost = (boost == 0.0 ? 10.0 : 10.0 * boost / sc[1]) * sc[1];
   }
-  // more conjugate-gradient iterations (in fours, up to 40 per round) until
+  // more conjugate-gradient iterations (up to 40 per round) until
   // the residual against the operator of the O rows is below 1e-13.  The
   // recurrence's residual can drift from D g - A x over many iterations (a
   // poor, boosted preconditioner of an ill-conditioned system), so each
-  // round ends by restarting from the true residual, which also gives lambda
-  // and the reported residual; up to four rounds.
+  // round ends with the true residual, which also gives lambda and the
+  // reported residual; a round that leaves it above the tolerance restarts
+  // the recurrence from it (up to four rounds).
   int pcg_iters = c->refine_iters;
   const double tol = 1e-13;
+  // iterations per residual check: one where a pass over the O rows (at
+  // least 256 MB) outweighs the host round trip of the check, else four
+  const int chunk = (double)p.rows * nv * sizeof(double) >= 256.0 * (1 << 20) ? 1 : 4;
   auto rel_res = [&]() { return std::sqrt(sc[5] / std::max(sc[6], 1e-300)); };
   if (p.orows && fl[4] == 0 && fl[3] == 0)
     for (int round = 0; round < 4; round++) {
+      if (round > 0) launch_pcg_direction(p, h->st);  // z = M^-1 r, p = z from the true residual
       int k = round == 0 ? c->refine_iters : 0, k_end = k + 40;
       while (rel_res() > tol && k < k_end) {
-        launch_pcg_iters(p, k, k + 4, h->st);
+        launch_pcg_iters(p, k, k + chunk, h->st);
         launch_pcg_final(p, h->st);
         HIPCHK(h, hipGetLastError());
-        k += 4;
-        pcg_iters += 4;
+        k += chunk;
+        pcg_iters += chunk;
         if ((rc = read_scalars())) return rc;
       }
-      launch_pcg_start(p, h->st);  // r = D g - A x
-      launch_pcg_final(p, h->st);  // A x = D g - r: lambda and the true residual
+      launch_true_residual(p, h->st);  // r = D g - A x
+      launch_pcg_final(p, h->st);      // A x = D g - r: lambda and the true residual
       HIPCHK(h, hipGetLastError());
       if ((rc = read_scalars())) return rc;
       if (rel_res() <= tol) break;

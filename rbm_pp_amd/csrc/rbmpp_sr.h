@@ -81,8 +81,12 @@ struct SrPipeline {
 void launch_sr_pipeline(const SrPipeline& p, hipStream_t st);
 void launch_pcg_iters(const SrPipeline& p, int k0, int k1, hipStream_t st);
 // (re)start: r = D g - A x with the operator of the O rows, z = M^-1 r, p = z;
-// launch_pcg_iters continues from k0 = 0
+// launch_pcg_iters continues from k0 = 0.  launch_pcg_start is the two
+// halves in order: launch_true_residual (r and A x = D g - r, one pass over
+// the O rows) and launch_pcg_direction (z, p and r.z from that r).
 void launch_pcg_start(const SrPipeline& p, hipStream_t st);
+void launch_true_residual(const SrPipeline& p, hipStream_t st);
+void launch_pcg_direction(const SrPipeline& p, hipStream_t st);
 void launch_pcg_final(const SrPipeline& p, hipStream_t st);
 // doubles of scratch of the O-row operator
 size_t orow_op_scratch(int rows, int nv);

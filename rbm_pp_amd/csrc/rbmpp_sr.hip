@@ -345,53 +345,115 @@ __global__ void k_sr_system(int nv, int nvr, const double* __restrict__ sc, cons
   }
 }
 
-// Forward substitution, block [k0, k0+nb): y_k = X_k b_k with X_k the inverse
-// of the diagonal block (every workgroup forms it from b, which no workgroup
-// of this launch writes), written to out by workgroup 0; then the rows below,
-// b_i -= sum_j L(i, k0 + j) y_j, each by the workgroup that owns it.
+// Substitutions with the Cholesky factor in steps of TS_NB = 4 CH_NB rows:
+// every workgroup of a step solves the step's diagonal block itself (four
+// CH_NB blocks through their inverses X_k and the L blocks between them, all
+// from L2), so each step is one launch and the launches over the rest of the
+// factor use many workgroups.  The diagonal entries read are the step's own,
+// which no workgroup of the launch writes.  Every inner product runs over a
+// fixed number of terms, fully unrolled and without per-term branches, so
+// that its loads are independent of each other: X_k is stored with zeros
+// above its diagonal and beyond the factor's end, the step's solution is
+// zero beyond its end, and a term beyond the end reads the last row or
+// column inside it and adds that times zero.
+#define TS_NB (4 * CH_NB)
+
+// Forward, step [k0, k0 + nb): y = L_kk^-1 b_k (written to out by workgroup
+// 0), then every row below, b_i -= sum_j L(i, k0 + j) y_j, one row per wave
+// and four rows in flight (lanes over j, 512-byte row segments).
 __global__ __launch_bounds__(256) void k_trsv_fwd(int n, int k0, const double* __restrict__ L,
                                                   const double* __restrict__ dinv, double* b, double* out) {
-  __shared__ double bk[CH_NB], yk[CH_NB];
-  const int nb = min(CH_NB, n - k0), t = threadIdx.x;
-  const double* D = dinv + (size_t)(k0 / CH_NB) * CH_NB * CH_NB;
-  if (t < CH_NB) bk[t] = t < nb ? b[k0 + t] : 0.0;
+  __shared__ double bb[TS_NB], yy[TS_NB];
+  const int nb = min(TS_NB, n - k0), t = threadIdx.x, lane = t & 63;
+  bb[t] = t < nb ? b[k0 + t] : 0.0;
+  yy[t] = 0.0;
   __syncthreads();
-  if (t < CH_NB) {
-    double y = 0.0;
-    for (int j = 0; j <= t; j++) y = fma(D[t * CH_NB + j], bk[j], y);
-    yk[t] = y;
-    if (blockIdx.x == 0 && t < nb) out[k0 + t] = y;
+  for (int s = 0; s < 4 && CH_NB * s < nb; s++) {
+    const int o = CH_NB * s;
+    if (t < CH_NB) {
+      const double* D = dinv + (size_t)(k0 / CH_NB + s) * CH_NB * CH_NB + t * CH_NB;
+      double y = 0.0;
+#pragma unroll
+      for (int j = 0; j < CH_NB; j++) y = fma(D[j], bb[o + j], y);
+      yy[o + t] = y;
+    }
+    __syncthreads();
+    if (t >= o + CH_NB && t < nb) {
+      const double* row = L + (size_t)(k0 + t) * n + k0 + o;
+      double acc = 0.0;
+#pragma unroll
+      for (int j = 0; j < CH_NB; j++) acc = fma(row[j], yy[o + j], acc);
+      bb[t] -= acc;
+    }
+    __syncthreads();
   }
-  __syncthreads();
-  const int i = k0 + nb + blockIdx.x * blockDim.x + t;
-  if (i >= n) return;
-  const double* row = L + (size_t)i * n + k0;
-  double acc = 0.0;
-  for (int j = 0; j < nb; j++) acc = fma(row[j], yk[j], acc);
-  b[i] -= acc;
+  if (blockIdx.x == 0 && t < nb) out[k0 + t] = yy[t];
+  int cj[4];
+#pragma unroll
+  for (int c = 0; c < 4; c++) cj[c] = min(lane + CH_NB * c, nb - 1);
+  const int nw = gridDim.x * 4;
+  for (int i0 = k0 + nb + blockIdx.x * 4 + (t >> 6); i0 < n; i0 += 4 * nw) {
+    double acc[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int i = min(i0 + r * nw, n - 1);  // rows past the end repeat the last row; not stored
+      const double* row = L + (size_t)i * n + k0;
+      acc[r] = 0.0;
+#pragma unroll
+      for (int c = 0; c < 4; c++) acc[r] = fma(row[cj[c]], yy[lane + CH_NB * c], acc[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const double v = wave_sum(acc[r]);
+      const int i = i0 + r * nw;
+      if (lane == 0 && i < n) b[i] -= v;
+    }
+  }
 }
 
-// Back substitution L^T x = y, block [k0, k0+nb): x_k = X_k^T y_k, written to
-// out by workgroup 0; the rows above, y_i -= sum_j L(k0 + j, i) x_j.
+// Back substitution L^T x = y, step [k0, k0 + nb): x = L_kk^-T y_k (written
+// to out by workgroup 0), then the rows above, y_i -= sum_j L(k0 + j, i) x_j:
+// workgroup g takes i = 64 g + lane, wave w the terms j in [64 w, 64 w + 64),
+// added in a fixed order.
 __global__ __launch_bounds__(256) void k_trsv_bwd(int n, int k0, const double* __restrict__ L,
                                                   const double* __restrict__ dinv, double* y, double* out) {
-  __shared__ double yb[CH_NB], xk[CH_NB];
-  const int nb = min(CH_NB, n - k0), t = threadIdx.x;
-  const double* D = dinv + (size_t)(k0 / CH_NB) * CH_NB * CH_NB;
-  if (t < CH_NB) yb[t] = t < nb ? y[k0 + t] : 0.0;
+  __shared__ double yb[TS_NB], xk[TS_NB], red[4][64];
+  const int nb = min(TS_NB, n - k0), t = threadIdx.x, lane = t & 63, w = t >> 6;
+  yb[t] = t < nb ? y[k0 + t] : 0.0;
+  xk[t] = 0.0;
   __syncthreads();
-  if (t < CH_NB) {
-    double x = 0.0;
-    for (int j = t; j < CH_NB; j++) x = fma(D[j * CH_NB + t], yb[j], x);
-    xk[t] = x;
-    if (blockIdx.x == 0 && t < nb) out[k0 + t] = x;
+  for (int s = (nb + CH_NB - 1) / CH_NB - 1; s >= 0; s--) {
+    const int o = CH_NB * s, ns = min(CH_NB, nb - o);
+    if (t < CH_NB) {
+      // X_k^T row t: the entries X_k(j, t), zero for j < t
+      const double* D = dinv + (size_t)(k0 / CH_NB + s) * CH_NB * CH_NB + t;
+      double x = 0.0;
+#pragma unroll
+      for (int j = 0; j < CH_NB; j++) x = fma(D[j * CH_NB], yb[o + j], x);
+      xk[o + t] = x;
+    }
+    __syncthreads();
+    if (t < o) {
+      const double* col = L + (size_t)(k0 + o) * n + k0 + t;
+      double acc = 0.0;
+#pragma unroll
+      for (int j = 0; j < CH_NB; j++) acc = fma(col[(size_t)min(j, ns - 1) * n], xk[o + j], acc);
+      yb[t] -= acc;
+    }
+    __syncthreads();
   }
-  __syncthreads();
-  const int i = blockIdx.x * blockDim.x + t;
-  if (i >= k0) return;
+  if (blockIdx.x == 0 && t < nb) out[k0 + t] = xk[t];
+  const int i = blockIdx.x * 64 + lane;
   double acc = 0.0;
-  for (int j = 0; j < nb; j++) acc = fma(L[(size_t)(k0 + j) * n + i], xk[j], acc);
-  y[i] -= acc;
+  if (i < k0 && CH_NB * w < nb) {
+    const double* col = L + (size_t)(k0 + CH_NB * w) * n + i;
+    const int last = min(CH_NB, nb - CH_NB * w) - 1;
+#pragma unroll
+    for (int j = 0; j < CH_NB; j++) acc = fma(col[(size_t)min(j, last) * n], xk[CH_NB * w + j], acc);
+  }
+  red[w][lane] = acc;
+  __syncthreads();
+  if (w == 0 && i < k0) y[i] -= ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
 }
 
 // r = D g - A x (one wave per row); with acc: x += 0 (plain residual)
@@ -468,16 +530,17 @@ void launch_cholesky(int n, double* A, double* dinv, int* info, hipStream_t st) 
 }
 
 void launch_chol_solve(int n, const double* L, const double* dinv, double* x, double* tmp, hipStream_t st) {
-  // forward: b = x (updated below each block) -> y in tmp
-  for (int k0 = 0; k0 < n; k0 += CH_NB) {
-    const int rest = std::max(n - k0 - CH_NB, 1);
-    hipLaunchKernelGGL(k_trsv_fwd, dim3((rest + 255) / 256), dim3(256), 0, st, n, k0, L, dinv, x, tmp);
+  // forward: b = x (updated below each step) -> y in tmp
+  for (int k0 = 0; k0 < n; k0 += TS_NB) {
+    const int rows = std::max(n - k0 - TS_NB, 1);
+    const int grid = std::min(256, (rows + 31) / 32);
+    hipLaunchKernelGGL(k_trsv_fwd, dim3(grid), dim3(256), 0, st, n, k0, L, dinv, x, tmp);
   }
-  // backward: y = tmp (updated above each block) -> x
-  const int nblk = (n + CH_NB - 1) / CH_NB;
-  for (int blk = nblk - 1; blk >= 0; blk--) {
-    const int k0 = blk * CH_NB;
-    hipLaunchKernelGGL(k_trsv_bwd, dim3((std::max(k0, 1) + 255) / 256), dim3(256), 0, st, n, k0, L, dinv, tmp, x);
+  // backward: y = tmp (updated above each step) -> x
+  const int nstep = (n + TS_NB - 1) / TS_NB;
+  for (int blk = nstep - 1; blk >= 0; blk--) {
+    const int k0 = blk * TS_NB;
+    hipLaunchKernelGGL(k_trsv_bwd, dim3(std::max(1, (k0 + 63) / 64)), dim3(256), 0, st, n, k0, L, dinv, tmp, x);
   }
 }
 
@@ -696,13 +759,21 @@ __global__ void k_b_minus(int n, int nvr, const double* __restrict__ sc, const d
   if (i < n) ax[i] = (i >= nvr ? g[i] * sc[0] : g[i]) - r[i];
 }
 
-void launch_pcg_start(const SrPipeline& p, hipStream_t st) {
+void launch_true_residual(const SrPipeline& p, hipStream_t st) {
+  launch_orow_op(p, p.x, p.ax, p.r, st);  // r = b - A x
+}
+
+void launch_pcg_direction(const SrPipeline& p, hipStream_t st) {
   const int n = p.nv, nb = (n + 255) / 256;
-  launch_orow_op(p, p.x, p.ax, p.r, st);  // r = b - A x0
   hipLaunchKernelGGL(k_copy, dim3(nb), dim3(256), 0, st, n, (const double*)p.r, p.z);
   launch_chol_solve(n, p.L, p.dinv, p.z, p.tmp, st);
   hipLaunchKernelGGL(k_copy, dim3(nb), dim3(256), 0, st, n, (const double*)p.z, p.pd);
   hipLaunchKernelGGL(k_dot, dim3(1), dim3(1024), 0, st, n, (const double*)p.r, (const double*)p.z, p.sc + 9);
+}
+
+void launch_pcg_start(const SrPipeline& p, hipStream_t st) {
+  launch_true_residual(p, st);
+  launch_pcg_direction(p, st);
 }
 
 void launch_pcg_iters(const SrPipeline& p, int k0, int k1, hipStream_t st) {

@@ -104,9 +104,10 @@ program rbmpp_vmc
 
   ! optimisation (main.f90:300-455)
   src%s_factor0 = s_factor0; src%delta_tau = delta_tau; src%max_change = 0.04d0
-  ! a mixed-precision S is refined twice against the FP64 operator of the O
-  ! rows, which brings the step to the FP64 system's solution
-  src%slater_abs_av = 0.25d0; src%refine_iters = merge(1, 2, prec == 0); src%flags = 0
+  ! one refinement sweep in the captured solve; a mixed-precision S is then
+  ! refined against the FP64 operator of the O rows until the residual is
+  ! below 1e-13, which brings the step to the FP64 system's solution
+  src%slater_abs_av = 0.25d0; src%refine_iters = 1; src%flags = 0
   shift = 0d0; shift_av = 0d0; wf_av_list = 0d0; gam_av = 0d0; nav_chk = 0
   open(unit=10, file='Energy_vs_Iteration.txt', status='unknown')
   do it = 1, Nstep
