@@ -31,6 +31,7 @@ struct rbmpp_handle {
   int real_only;
   int device = 0;  // HIP device current at rbmpp_create
   hipStream_t st = nullptr;
+  CholAhead ahead;  // the SR Cholesky's look-ahead stream and events (joined back into st)
   std::string err;
   std::vector<int> wmap, invmap, orb;
   std::vector<unsigned char> subB;
@@ -238,6 +239,10 @@ void rbmpp_destroy(rbmpp_handle* h) {
   if (h->e1) hipEventDestroy(h->e1);
   if (h->sr_exec) hipGraphExecDestroy(h->sr_exec);
   if (h->sr_graph) hipGraphDestroy(h->sr_graph);
+  if (h->ahead.st2) hipStreamSynchronize(h->ahead.st2);
+  if (h->ahead.col_done) hipEventDestroy(h->ahead.col_done);
+  if (h->ahead.diag_done) hipEventDestroy(h->ahead.diag_done);
+  if (h->ahead.st2) hipStreamDestroy(h->ahead.st2);
   if (h->st) hipStreamDestroy(h->st);
   delete h;
 }
@@ -302,6 +307,9 @@ int rbmpp_create(const rbmpp_config* cfg, const int32_t* w_map, const double* pa
   HIPCHK(h, hipGetDevice(&h->device));
   HIPCHK(h, hipDeviceGetAttribute(&h->n_cu, hipDeviceAttributeMultiprocessorCount, h->device));
   HIPCHK(h, hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
+  HIPCHK(h, hipStreamCreateWithFlags(&h->ahead.st2, hipStreamNonBlocking));
+  HIPCHK(h, hipEventCreateWithFlags(&h->ahead.col_done, hipEventDisableTiming));
+  HIPCHK(h, hipEventCreateWithFlags(&h->ahead.diag_done, hipEventDisableTiming));
   HIPCHK(h, hipEventCreate(&h->e0));
   HIPCHK(h, hipEventCreate(&h->e1));
   int rc = 0;
@@ -880,6 +888,7 @@ int rbmpp_sr_step(rbmpp_handle* h, const rbmpp_sr_config* c, double* gamma, rbmp
   double* d_g = h->d_vec + 3 * nv;
   p.g = d_g;
   p.sc = h->d_sc; p.fl = h->d_fl; p.dinv = h->d_dinv;
+  p.ahead = h->ahead;
   // a mixed-precision S is refined against the FP64 operator of the O rows
   if (h->cfg.s_precision != RBMPP_S_FP64 && !(c->flags & RBMPP_SR_ASSEMBLED_REFINE)) {
     const int rows = h->real_only ? (int)h->n_meas : 2 * (int)h->n_meas;

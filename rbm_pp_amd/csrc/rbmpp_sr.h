@@ -39,9 +39,18 @@ void launch_center_pack(const SyrkPlan& pl, int nrows, int nv, const double* oro
 void launch_syrk_packed(const SyrkPlan& pl, int nv, const unsigned char* packed, double* part, double inv_norm,
                         double* S, hipStream_t st);
 void launch_gemv(int n, const double* A, const double* x, double* y, hipStream_t st);
+// A second stream (default priority) and two events for the look-ahead of
+// launch_cholesky; the sequence joins it back into the calling stream, so it
+// can be captured into a graph from that stream.
+struct CholAhead {
+  hipStream_t st2 = nullptr;
+  hipEvent_t col_done = nullptr, diag_done = nullptr;
+};
 // blocked Cholesky of the lower triangle in place (64-wide blocks); dinv gets
-// the inverses of the 64 x 64 diagonal blocks of the factor
-void launch_cholesky(int n, double* A, double* dinv, int* info, hipStream_t st);
+// the inverses of the 64 x 64 diagonal blocks of the factor.  With ahead,
+// each diagonal block is factored on ahead->st2 under the previous block's
+// trailing update.
+void launch_cholesky(int n, double* A, double* dinv, int* info, hipStream_t st, const CholAhead* ahead = nullptr);
 // L L^T x = b in place in x (blocked substitutions through the diagonal-block
 // inverses; tmp is an n-vector of scratch)
 void launch_chol_solve(int n, const double* L, const double* dinv, double* x, double* tmp, hipStream_t st);
@@ -62,6 +71,7 @@ struct SrPipeline {
   double* sc;                // [16] scalars: k_sr_prepare's, sc[7] = 1/norm, sc[8..11] scratch of the
                              // O-row operator and the conjugate gradients, sc[12] extra diagonal of L
   int* fl;                   // [8] flags and Cholesky info
+  CholAhead ahead;           // st2 set: the Cholesky's look-ahead stream and events
   // orows != nullptr: the refinement residuals and lambda use the FP64
   // operator of the O rows, D (C^T C / norm) D + shift I with
   // C = O - kappa obar^T, instead of the assembled A (for S assembled from

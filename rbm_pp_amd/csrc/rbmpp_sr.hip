@@ -216,12 +216,15 @@ __global__ __launch_bounds__(256) void k_trsm_panel(int n, int k0, double* A, co
 // a 32 x 32 quadrant of 2 x 2 blocks.  MFMA lane l supplies row l % 16 at
 // k = 4 s + l / 16 of both operands; its output register r is element
 // (l / 16 + 4 r, l % 16) of the block.
-__global__ __launch_bounds__(256) void k_syrk_trailing(int n, int k0, int t0, int ntile, double* A) {
+// With col_only, workgroup p takes the tile (p, 0) of the first tile column
+// only (the look-ahead of launch_cholesky).
+__global__ __launch_bounds__(256) void k_syrk_trailing(int n, int k0, int t0, int ntile, int col_only, double* A) {
   constexpr int TS = 64;
   __shared__ double Li[TS][CH_NB + 1];
   __shared__ double Lj[TS][CH_NB + 1];
   int p = blockIdx.x, tj = 0;  // lower-triangle tile pairs ti >= tj
-  while (p >= ntile - tj) { p -= ntile - tj; tj++; }
+  if (!col_only)
+    while (p >= ntile - tj) { p -= ntile - tj; tj++; }
   const int ti = tj + p;
   const int nb = min(CH_NB, n - k0);
   for (int e = threadIdx.x; e < TS * CH_NB; e += 256) {
@@ -517,15 +520,35 @@ void launch_row_kappa(int ns, int cplx_rows, const double2* psi, const double* r
 void launch_gemv(int n, const double* A, const double* x, double* y, hipStream_t st) {
   hipLaunchKernelGGL(k_gemv, dim3((n + 3) / 4), dim3(256), 0, st, n, A, x, y);
 }
-void launch_cholesky(int n, double* A, double* dinv, int* info, hipStream_t st) {
-  for (int k0 = 0; k0 < n; k0 += CH_NB) {
-    hipLaunchKernelGGL(k_potrf_inv, dim3(1), dim3(64), 0, st, n, k0, A, dinv, info);
-    const int t0 = k0 + CH_NB;
-    if (t0 >= n) break;
-    const int rows = n - t0;
+void launch_cholesky(int n, double* A, double* dinv, int* info, hipStream_t st, const CholAhead* ahead) {
+  if (!ahead) {
+    for (int k0 = 0; k0 < n; k0 += CH_NB) {
+      hipLaunchKernelGGL(k_potrf_inv, dim3(1), dim3(64), 0, st, n, k0, A, dinv, info);
+      const int t0 = k0 + CH_NB;
+      if (t0 >= n) break;
+      const int rows = n - t0;
+      hipLaunchKernelGGL(k_trsm_panel, dim3((rows + 63) / 64), dim3(256), 0, st, n, k0, A, (const double*)dinv);
+      const int nt = (rows + 63) / 64;
+      hipLaunchKernelGGL(k_syrk_trailing, dim3(nt * (nt + 1) / 2), dim3(256), 0, st, n, k0, t0, nt, 0, A);
+    }
+    return;
+  }
+  // Look-ahead: after the panel of block k, the trailing update of block
+  // column k + 1 alone; block k + 1 is then factored on the second stream
+  // while the rest of the trailing update runs, and the next panel waits for
+  // both.  Every launch touches what it did in the one-stream order.
+  hipLaunchKernelGGL(k_potrf_inv, dim3(1), dim3(64), 0, st, n, 0, A, dinv, info);
+  for (int k0 = 0; k0 + CH_NB < n; k0 += CH_NB) {
+    const int t0 = k0 + CH_NB, rows = n - t0, nt = (rows + 63) / 64;
     hipLaunchKernelGGL(k_trsm_panel, dim3((rows + 63) / 64), dim3(256), 0, st, n, k0, A, (const double*)dinv);
-    const int nt = (rows + 63) / 64;
-    hipLaunchKernelGGL(k_syrk_trailing, dim3(nt * (nt + 1) / 2), dim3(256), 0, st, n, k0, t0, nt, A);
+    hipLaunchKernelGGL(k_syrk_trailing, dim3(nt), dim3(256), 0, st, n, k0, t0, nt, 1, A);
+    hipEventRecord(ahead->col_done, st);
+    hipStreamWaitEvent(ahead->st2, ahead->col_done, 0);
+    hipLaunchKernelGGL(k_potrf_inv, dim3(1), dim3(64), 0, ahead->st2, n, t0, A, dinv, info);
+    hipEventRecord(ahead->diag_done, ahead->st2);
+    if (nt > 1)
+      hipLaunchKernelGGL(k_syrk_trailing, dim3((nt - 1) * nt / 2), dim3(256), 0, st, n, k0, t0 + CH_NB, nt - 1, 0, A);
+    hipStreamWaitEvent(st, ahead->diag_done, 0);
   }
 }
 
@@ -824,7 +847,7 @@ void launch_sr_pipeline(const SrPipeline& p, hipStream_t st) {
   hipLaunchKernelGGL(k_sr_prepare, dim3(1), dim3(1024), 0, st, n, p.nvr, p.S, p.fii, p.sc, p.fl);
   hipLaunchKernelGGL(k_sr_system, dim3(1024), dim3(256), 0, st, n, p.nvr, (const double*)p.sc, p.S, p.A, p.L, p.g,
                      p.x);
-  launch_cholesky(n, p.L, p.dinv, p.fl + 4, st);
+  launch_cholesky(n, p.L, p.dinv, p.fl + 4, st, p.ahead.st2 ? &p.ahead : nullptr);
   launch_chol_solve(n, p.L, p.dinv, p.x, p.ax, st);
   if (p.orows) {
     // conjugate gradients on the FP64 operator of the O rows, preconditioned
