@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librbmpp.so")
 SOURCES = ["rbmpp_capi.hip", "rbmpp_sampler.hip", "rbmpp_local.hip", "rbmpp_meas.hip", "rbmpp_meas_blk.hip",
-           "rbmpp_corr.hip", "rbmpp_corr_blk.hip", "rbmpp_sr.hip", "rbmpp_syrk.hip"]
+           "rbmpp_meas_blk_big.hip", "rbmpp_corr.hip", "rbmpp_corr_blk.hip", "rbmpp_sr.hip", "rbmpp_syrk.hip"]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
          # packed FP32 instructions are kept out of every kernel
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

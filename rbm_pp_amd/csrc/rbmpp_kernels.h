@@ -112,6 +112,10 @@ void launch_meas(const LocalArgs& a, int grid, hipStream_t s);
 // block form (DevModel::block, C == RBMPP_CMAX, N <= RBMPP_NLDS): a.scratch
 // holds grid x C x (N/2)^2 doubles for the F^-1 of every copy
 void launch_meas_blk(const LocalArgs& a, int grid, hipStream_t s);
+// block form for RBMPP_NLDS < N <= 256 (C == RBMPP_CMAX): a.scratch holds
+// grid x meas_blk_big_scratch(N) doubles (at most grid x N x N)
+void launch_meas_blk_big(const LocalArgs& a, int grid, hipStream_t s);
+size_t meas_blk_big_scratch(int N);
 // the same estimator with the correlation measurement (N <= RBMPP_NLDS)
 void launch_meas_corr(const LocalArgs& a, int grid, hipStream_t s);
 // the block-form estimator with the correlation measurement; a.scratch as for

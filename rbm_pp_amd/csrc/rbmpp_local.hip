@@ -254,7 +254,9 @@ int launch_local(const DevModel& hm, const LocalArgs& a, int grid, hipStream_t s
   } else if (N <= RBMPP_NLDS) launch_meas(a, grid, st);
   else if (N <= 256) {
     if (!a.scratch) return -1;
-    launch_local_t<256, false>(a, grid, st);
+    // the callers' scratch holds N x N doubles per workgroup
+    if (hm.block && hm.C == RBMPP_CMAX && meas_blk_big_scratch(N) <= (size_t)N * N) launch_meas_blk_big(a, grid, st);
+    else launch_local_t<256, false>(a, grid, st);
   } else return -1;
   return 0;
 }

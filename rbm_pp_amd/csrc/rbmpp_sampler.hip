@@ -118,118 +118,6 @@ struct SweepSmem {
 };
 
 
-// det F and F^-1 of one copy in block form for N > 104 (np up to 128, where
-// a row per lane does not fit the registers), by the whole workgroup with F
-// in LDS: T(u, d) = F(u, d) = Slater[site_u][site_(np+d) + N], then
-// Gauss-Jordan exchange steps on the pivots (p_k, k), p_k the unused row with
-// the largest |T(., k)| (smallest index on ties), as the per-wave elimination
-// of rbmpp_block.h: pivot search by one wave, then the pivot row and column
-// staged and the whole matrix updated, three barriers a step.  Afterwards
-// F^-1(k, u) = T(perm[k], pinv[u]), written to Bout row-major.  det F is the
-// product of the pivots times the sign of the row order.  Returns 0 for an
-// exactly singular pivot (every thread).  With drift, drift[0] and drift[1]
-// get the largest |new - old| and |new| over Bout (every thread).
-// LDS: T np^2 doubles, rowb and colb np doubles each, used, perm, pinv np ints
-// each, red 32 doubles, s_p / s_pv / s_cnt one value each.
-__device__ __attribute__((noinline)) double wg_block_inverse_lds(const double* __restrict__ S, int N, int np,
-                                                                 const int* eidx, double* T, double* rowb,
-                                                                 double* colb, int* used, int* perm, int* pinv,
-                                                                 double* red, int* s_p, double* s_pv, int* s_cnt,
-                                                                 double* Bout, double* drift) {
-  const int tid = threadIdx.x, nth = blockDim.x, lane = tid & (RBMPP_WAVE - 1), wid = tid >> 6;
-  const int npp = np * np;
-  for (int e = tid; e < npp; e += nth) {
-    const int u = e / np, d = e - u * np;
-    T[e] = S[(size_t)eidx[u] * 2 * N + eidx[np + d] + N];
-  }
-  for (int r = tid; r < np; r += nth) used[r] = 0;
-  __syncthreads();
-  double det = 1.0;
-  int neg = 0;
-  for (int k = 0; k < np; k++) {
-    if (wid == 0) {
-      // rows lane and lane + 64: the unused one with the largest |T(., k)|
-      double bv = -1.0;
-      int bi = np;
-#pragma unroll
-      for (int h = 0; h < 2; h++) {
-        const int r = lane + RBMPP_WAVE * h;
-        if (r < np && !used[r]) {
-          const double v = fabs(T[r * np + k]);
-          if (v > bv) { bv = v; bi = r; }
-        }
-      }
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) {
-        const double ov = __shfl_xor(bv, o, RBMPP_WAVE);
-        const int oi = __shfl_xor(bi, o, RBMPP_WAVE);
-        if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
-      }
-      // unused rows below the pivot: the sign of the row order
-      const int r0 = lane, r1 = lane + RBMPP_WAVE;
-      const int below = __popcll(__ballot(r0 < np && r0 < bi && !used[r0])) +
-                        __popcll(__ballot(r1 < np && r1 < bi && !used[r1]));
-      if (lane == 0) {
-        if (bi < np) {
-          *s_p = bi;
-          *s_pv = T[bi * np + k];
-          *s_cnt = below;
-          used[bi] = 1;
-          perm[k] = bi;
-        } else {  // no finite candidate (NaN entries): reported as singular
-          *s_p = 0;
-          *s_pv = 0.0;
-          *s_cnt = 0;
-        }
-      }
-    }
-    __syncthreads();
-    const int p = *s_p;
-    const double pv = *s_pv;
-    if (pv == 0.0) return 0.0;  // the same value in every thread
-    det *= pv;
-    neg ^= *s_cnt & 1;
-    for (int c = tid; c < np; c += nth) rowb[c] = T[p * np + c];
-    for (int r = tid; r < np; r += nth) colb[r] = T[r * np + k];
-    __syncthreads();
-    const double inv = 1.0 / pv;
-    // other rows: T - (T(., k) / pv) * pivot row, column k -> -T(., k) / pv;
-    // the pivot row: / pv, column k -> 1 / pv
-    for (int e = tid; e < npp; e += nth) {
-      const int r = e / np, c = e - r * np;
-      double v;
-      if (r == p) v = c == k ? inv : rowb[c] * inv;
-      else v = c == k ? -(colb[r] * inv) : fma(-(colb[r] * inv), rowb[c], T[e]);
-      T[e] = v;
-    }
-    __syncthreads();
-  }
-  for (int k = tid; k < np; k += nth) pinv[perm[k]] = k;
-  __syncthreads();
-  double ldiff = 0.0, lmax = 0.0;
-  for (int e = tid; e < npp; e += nth) {
-    const int k = e / np, u = e - k * np;
-    const double v = T[perm[k] * np + pinv[u]];
-    if (drift) {
-      ldiff = fmax(ldiff, fabs(Bout[e] - v));
-      lmax = fmax(lmax, fabs(v));
-    }
-    Bout[e] = v;
-  }
-  if (drift) {
-    ldiff = wave_max(ldiff);
-    lmax = wave_max(lmax);
-    if (lane == 0) { red[2 * wid] = ldiff; red[2 * wid + 1] = lmax; }
-    __syncthreads();
-    double a = 0.0, b = 0.0;
-    for (int w = 0; w < (nth >> 6); w++) { a = fmax(a, red[2 * w]); b = fmax(b, red[2 * w + 1]); }
-    drift[0] = a;
-    drift[1] = b;
-    __syncthreads();  // red is reused by the caller
-  }
-  return neg ? -det : det;
-}
-
 template <int NMAX, bool LDSMAT, int RL>
 __device__ __attribute__((noinline)) void wg_recompute(const DevModel& m, SweepSmem<NMAX, LDSMAT, RL>& s, double* Mw,
                              double* gscratch, int check, double* drift_pf, double* drift_inv) {
@@ -288,7 +176,7 @@ __device__ __attribute__((noinline)) void wg_recompute(const DevModel& m, SweepS
     const double pfsign = ((np * (np - 1) / 2) & 1) ? -1.0 : 1.0;
     for (int q = 0; q < C; q++) {
       double dr[2] = {0.0, 0.0};
-      const double det = wg_block_inverse_lds(m.se + (size_t)q * 4 * N * N, N, np, s.eidx, s.u, s.gjbuf,
+      const double det = rbmpp_block::wg_block_inverse_lds(m.se + (size_t)q * 4 * N * N, N, np, s.eidx, s.u, s.gjbuf,
                                               s.gjbuf + HM, ib, ib + HM, ib + 2 * HM, s.red, &s.gsel[0],
                                               &s.gselv[0], &s.gsel[1], Mw + (size_t)q * fold_size(N),
                                               check ? dr : nullptr);
