@@ -447,6 +447,8 @@ def main():
     ap.add_argument("--batch-warmup", type=int, default=4)
     ap.add_argument("--sampler-only", action="store_true",
                     help="profiling aid: thermalise, run one make_sample launch and exit")
+    ap.add_argument("--with-measure", action="store_true",
+                    help="with --sampler-only: also time the measurement of the launch's samples")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="profiling aid: three device-to-device copies of MIB MiB and exit")
     args = ap.parse_args()
@@ -552,14 +554,21 @@ def main():
         block = sampler_form(args.projection, N, args.fused_measure)
         b_unit = sweep_bytes(block, Cc, N, acc)
         gbs = b_unit * args.walkers * sweeps_per_launch_walker / t["sample"] / 1e9
-        print(json.dumps({"kernel": "k_sweep",
-                          "workload": f"j1j2_{args.L}x{args.L}_J2_{args.J2}_alphar{args.alphar}_c4v_parity",
-                          "walkers": args.walkers, "sweeps_per_walker": sweeps_per_launch_walker,
-                          "launch_s": t["sample"], "acceptance": acc,
-                          "samples_per_s": args.walkers * nspw / t["sample"],
-                          "layout": "F^-1 per copy (block form)" if block else "folded skew X^-1 per copy",
-                          "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                       "frac": gbs / HBM_PEAK_GBS, "bytes_per_walker_sweep": b_unit}}))
+        out = {"kernel": "k_sweep",
+               "workload": f"j1j2_{args.L}x{args.L}_J2_{args.J2}_alphar{args.alphar}_c4v_parity",
+               "walkers": args.walkers, "sweeps_per_walker": sweeps_per_launch_walker,
+               "launch_s": t["sample"], "acceptance": acc,
+               "samples_per_s": args.walkers * nspw / t["sample"],
+               "layout": "F^-1 per copy (block form)" if block else "folded skew X^-1 per copy",
+               "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": gbs / HBM_PEAK_GBS, "bytes_per_walker_sweep": b_unit}}
+        if args.with_measure:
+            # calc_phys_quantity_opt on the launch's samples (E_loc, O_k)
+            eng.calc_phys_quantity_opt(shift=0.0)
+            tm = eng.timings()["measure"]
+            out["measure_s"] = tm
+            out["measure_samples_per_s"] = args.walkers * nspw / tm
+        print(json.dumps(out))
         return
 
     wf_list = [0.0] * 10

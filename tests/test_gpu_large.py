@@ -3,8 +3,10 @@ and the 16x16 workload (J2/J1 = 0.55, alphar = 16, C4v + parity, N = 256).
 Above N = 104 the normal projection's sampler keeps F^-1 of each copy with
 two columns per lane and inverts F in LDS (k_sweep<256, true>); the
 reference's exact weights keep X^-1 in folded skew storage with the recompute
-matrix in global scratch (k_sweep<256, false>, wg_skew_inverse); the
-estimator is the grid-strided k_local (rbmpp_local.hip).  Bounds as
+matrix in global scratch (k_sweep<256, false>, wg_skew_inverse).  The
+normal projection's estimator is the block-form k_meas_blk_big (F of each
+copy inverted in LDS by the workgroup); the exact weights' is the skew-form
+k_local (rbmpp_local.hip).  Bounds as
 in test_gpu_parity: FP64 values relative 1e-10 (normwise per matrix, scaled
 by the copy's condition where noted), trajectories bit-exact."""
 import numpy as np
@@ -84,12 +86,16 @@ def test_ratio_exchange_large(oracle_lib, L, ar, J2):
         assert abs(psi[s] - psin[0]) <= RTOL * np.abs(psin[1:]).sum()
 
 
-@pytest.mark.parametrize("L,ar,J2", LARGE)
-def test_local_energy_and_ok_large(oracle_lib, L, ar, J2):
+@pytest.mark.parametrize("L,ar,J2,proj", [(L, ar, J2, "block") for L, ar, J2 in LARGE] + [(12, 2, 0.5, "exact")])
+def test_local_energy_and_ok_large(oracle_lib, L, ar, J2, proj):
+    """proj "block": the block-form estimator (k_meas_blk_big); "exact": the
+    reference's weights, the skew-form estimator (k_local)."""
     api = _api()
     spec, p = _model(L, ar, J2)
     orc = oracle_lib.Oracle(spec, p)
     eng = api.Engine(spec, params=p, n_walkers=1, n_samples=1)
+    if proj == "exact":
+        eng.set_projection(api.PROJ_NORMAL_EXACT)
     xs = _configs(spec.N, 3, seed=200 + L)
     got = eng.local(xs, want_o=True)
     for s, x in enumerate(xs):
