@@ -109,8 +109,15 @@ __device__ __forceinline__ double blk_ratio(double yh, double xh, double w, doub
   return a1 * a2 - yc * b;
 }
 
+typedef double f64x4_t __attribute__((ext_vector_type(4)));
+
 // Gdd and Yh = Gdd F^-1 of copy S (F^-1 in s.binv), and with guu also Guu,
-// by the whole workgroup; ends in step.
+// by the whole workgroup; ends in step.  The product runs on the FP64 matrix
+// cores: np is padded to 64 with zeros, wave w computes the 16 x 16 tile
+// (w / 4, w % 4) of Yh by 16 v_mfma_f64_16x16x4 (lane l supplies row l % 16
+// of Gdd and column l % 16 of F^-1 at k = 4 s + l / 16; output register r
+// is element (l / 16 + 4 r, l % 16) of the tile), reading each operand once
+// from LDS instead of once per output entry.
 __device__ __attribute__((noinline)) void copy_yh(const DevModel& m, BlkSmem& s, const double* S, bool guu) {
   const int N = m.N, np = m.npair, two = 2 * N, tid = threadIdx.x;
   const int npp = np * np;
@@ -120,13 +127,24 @@ __device__ __attribute__((noinline)) void copy_yh(const DevModel& m, BlkSmem& s,
     if (guu) s.u.c.guu[e] = S[(size_t)s.eidx[r] * two + s.eidx[c] + N];
   }
   __syncthreads();
-  for (int e = tid; e < npp; e += BNT) {
-    const int d = e / np, u = e - d * np;
-    const double* gd = s.u.c.gdd + d * np;
-    double t = 0.0;
-#pragma unroll 10
-    for (int k = 0; k < np; k++) t = fma(gd[k], s.binv[k * np + u], t);
-    s.u.c.yh[e] = t;
+  static_assert(HMX <= 64 && BNW == 16, "Yh tiles: np <= 64, one 16 x 16 tile per wave");
+  const int lane = tid & (RBMPP_WAVE - 1), wid = tid >> 6;
+  const int r0 = 16 * (wid >> 2), c0 = 16 * (wid & 3);
+  if (r0 < np && c0 < np) {  // wave-uniform
+    const int mr = lane & 15, ks = lane >> 4;
+    const int ra = r0 + mr, cb = c0 + mr;
+    f64x4_t acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < np; k0 += 4) {
+      const int k = k0 + ks;
+      const double a = (ra < np && k < np) ? s.u.c.gdd[ra * np + k] : 0.0;
+      const double b = (cb < np && k < np) ? s.binv[k * np + cb] : 0.0;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int d = r0 + ks + 4 * r;
+      if (d < np && cb < np) s.u.c.yh[d * np + cb] = acc[r];
+    }
   }
   __syncthreads();
 }
