@@ -576,14 +576,17 @@ def main():
     gam = eng.gamma()
     gam_prev = gam.copy()
 
+    last_grad = None  # the energy gradient of the last measurement (SR_pdposv.f90:4-6)
+
     def one_step(rec):
-        nonlocal shift, gam, gam_prev
+        nonlocal shift, gam, gam_prev, last_grad
         t0 = time.perf_counter()
         nacc = eng.make_sample(False, shift=shift)
         tt = eng.timings()
         t1 = time.perf_counter()
         r = eng.calc_phys_quantity_opt(shift=shift)
         t2 = time.perf_counter()
+        last_grad = r["gvec"] - 2.0 * r["Etot"] * r["ovec"]
         gam_prev = gam.copy()
         gam, rep = eng.sr_step(gam, s_factor0=1e-4, delta_tau=0.05, refine_iters=refine_sweeps(args.s_precision))
         t3 = time.perf_counter()
@@ -644,13 +647,17 @@ def main():
                               s_rel_err=float(np.linalg.norm(r["S"] - base["S"]) / np.linalg.norm(base["S"])),
                               step_rel_err=float(np.linalg.norm(r["step"] - base["step"]) /
                                                  np.linalg.norm(base["step"])),
+                              # the energy change per site that the step's
+                              # difference from the FP64 step makes, to first
+                              # order, on the same samples
+                              dE_per_site=float(last_grad @ (r["step"] - base["step"])) / 4 / N,
                               refine_sweeps=refine_sweeps(name))
             if r["step_asm"] is not None:
                 fp64[name]["step_rel_err_assembled_refine"] = float(
                     np.linalg.norm(r["step_asm"] - base["step"]) / np.linalg.norm(base["step"]))
             log(f"S build {name}: {r['t_s'] * 1e3:.3f} ms ({ach:.1f} TFLOP/s, {100 * ach / peaks[name]:.1f}% of "
                 f"{peaks[name]:.0f}), SR step {r['t_sr'] * 1e3:.2f} ms, S err {fp64[name]['s_rel_err']:.2e}, "
-                f"step err {fp64[name]['step_rel_err']:.2e}")
+                f"step err {fp64[name]['step_rel_err']:.2e}, dE/N {fp64[name]['dE_per_site']:.1e}")
         eng.set_s_precision(S_MODES[args.s_precision])
 
     # The measurement of the correlation run (Calc_Correlation: Chi_sp over
