@@ -59,7 +59,6 @@ static_assert(sizeof(BigSmem) <= 160 * 1024, "the estimator's LDS fits a gfx950 
 // the members the shared helpers (wg_pp_oloc, wg_eloc, wg_orow) reach through
 // S: the shared-memory block plus the scratch-resident bins and sums
 struct BigView {
-  BigSmem& sm;
   double (*opp)[4 * GNMX];
   double2* psib;
   const int* x;
@@ -174,7 +173,7 @@ __global__ __launch_bounds__(GNT) void k_meas_blk_big(LocalArgs a) {
   double* rat = yh + (size_t)npp;          // [4N] ratios of the current copy
   double2* psib = (double2*)(rat + 4 * N);  // [4N] sums over copies of psi(x')
   double (*opp)[4 * GNMX] = (double (*)[4 * GNMX])(rat + 12 * N);  // [2][4N used] bins
-  BigView v{s, opp, psib, s.x, s.ecfg, s.thr, s.thc, s.psiq};
+  BigView v{opp, psib, s.x, s.ecfg, s.thr, s.thc, s.psiq};
 
   for (int cfg = blockIdx.x; cfg < a.nconf; cfg += gridDim.x) {
     __syncthreads();
