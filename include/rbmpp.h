@@ -224,8 +224,8 @@ int rbmpp_tune(rbmpp_handle* h, int32_t what, rbmpp_tune_report* rep);
  * one persistent launch over all their walkers: each workgroup takes the
  * model its walker belongs to.  Every handle then measures and takes its SR
  * step on its own (rbmpp_measure, rbmpp_sr_step), and may still sample alone.
- * The batch covers the normal projection's block form (16 copies, N <= 104)
- * and the skew form above N = 104; no fused measurement.  Walker w of handle
+ * The batch covers the normal projection's block form (16 copies, N <= 256)
+ * and the skew form above N = 104 (up to 256); no fused measurement.  Walker w of handle
  * i draws from the same Philox stream as in rbmpp_make_sample, so a batch
  * call reproduces the handles' separate calls exactly.  No counterpart in
  * the reference (one lattice per run, main.f90:75-141). */

@@ -1508,7 +1508,7 @@ int rbmpp_batch_create(rbmpp_handle* const* handles, int32_t n, rbmpp_batch** ou
     if (kind < 0)
       return bfail(b, RBMPP_E_ARG, "handle " + std::to_string(i) +
                                        ": the batch covers the block form (normal projection, 16 copies) up to "
-                                       "N = 104 and the skew form above");
+                                       "N = 256 and the skew form from N = 106 to 256");
     b->hs.push_back(h);
     b->kind.push_back(kind);
   }
