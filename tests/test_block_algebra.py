@@ -76,3 +76,36 @@ def test_block_exchange_ratio_is_one_for_the_identity_move():
     got = blk_ratio(Yh[d, u], B[d] @ Guu[:, u], Yh[d] @ Guu[:, u], B[d, u], Sl[sd, su] - Sl[sd, sd],
                     Sl[su, su] - Sl[su, sd], Gdd[d, d])
     assert got == pytest.approx(1.0, abs=1e-9)
+
+
+def test_sampler_rank2_update_matches_inverse():
+    """The sampler's accepted move in block form (rbmpp_sampler.hip, the
+    pending update applied by blk_upd1 / blk_upd4): F' = F + e_ua r^T + c e_db^T
+    with c(ua) = 0, y = r^T B, x = B c (old B), R = (1 + y_ua)(1 + B[db].c)
+    - (y.c) B[db][ua] and
+      B'[d][u] = B[d][u] + B[d][ua] al_u + x_d be_u,
+      al = -((1 + B[db].c) y - (y.c) B[db]) / R,
+      be = -((1 + y_ua) B[db] - B[db][ua] y) / R,
+    against det F'/det F and inv(F') on random matrices."""
+    rng = np.random.default_rng(11)
+    for np_ in (8, 25, 50, 128):
+        F = rng.standard_normal((np_, np_)) + 3.0 * np.eye(np_)
+        B = np.linalg.inv(F)  # B[d, u]
+        for _ in range(10):
+            ua, db = rng.integers(np_), rng.integers(np_)
+            r = rng.standard_normal(np_)  # change of row ua (over the down electrons d)
+            c = rng.standard_normal(np_)  # change of column db (over the up electrons u)
+            c[ua] = 0.0
+            F2 = F + np.outer(np.eye(np_)[ua], r) + np.outer(c, np.eye(np_)[db])
+            y = r @ B          # y_u = sum_d r_d B[d, u]
+            brow = B[db]       # B[db][u]
+            gam, bet = y[ua], brow[ua]
+            dlt, eps_ = brow @ c, y @ c
+            R = (1.0 + gam) * (1.0 + dlt) - eps_ * bet
+            assert R == pytest.approx(np.linalg.det(F2) / np.linalg.det(F), rel=1e-9)
+            al = -((1.0 + dlt) * y - eps_ * brow) / R
+            be = -((1.0 + gam) * brow - bet * y) / R
+            x = B @ c          # x_d = sum_u B[d, u] c_u
+            B2 = B + np.outer(B[:, ua], al) + np.outer(x, be)
+            want = np.linalg.inv(F2)
+            assert np.abs(B2 - want).max() <= 1e-9 * np.abs(want).max()
