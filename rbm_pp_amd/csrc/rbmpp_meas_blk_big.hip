@@ -85,21 +85,47 @@ __device__ __forceinline__ double big_ratio(double yh, double xh, double w, doub
   return a1 * a2 - yc * b;
 }
 
+typedef double f64x4_t __attribute__((ext_vector_type(4)));
+
 // Yh = Gdd F^-1 of copy S into yh (global scratch), Gdd(d, d') =
-// Slater[site_(np+d)][site_(np+d') + N]: thread e takes the entries
-// (d, u) = (e / np, e % np) + k 1024 / np rows; the k loop reads row d of the
-// Slater table (the same address across the lanes of a row) and column u of
-// F^-1 from LDS.  Ends in step.
+// Slater[site_(np+d)][site_(np+d') + N], on the FP64 matrix cores: np is
+// covered by 16 x 16 tiles (zero beyond np), wave w takes row strip w / 2
+// and the four column tiles 4 (w % 2) + j, so that one operand fragment of
+// Gdd (gathered from the Slater table) feeds four v_mfma_f64_16x16x4 (lane l
+// supplies row l % 16 of Gdd and column l % 16 of F^-1 at k = 4 s + l / 16;
+// output register r is element (l / 16 + 4 r, l % 16) of a tile).  Ends in
+// step.
 __device__ __attribute__((noinline)) void big_yh(const DevModel& m, const BigSmem& s, const double* S, double* yh) {
   const int N = m.N, np = m.npair, two = 2 * N, tid = threadIdx.x;
-  const int npp = np * np;
-  for (int e = tid; e < npp; e += GNT) {
-    const int d = e / np, u = e - d * np;
-    const double* srow = S + (size_t)s.eidx[np + d] * two + N;
-    double t = 0.0;
-#pragma unroll 8
-    for (int k = 0; k < np; k++) t = fma(srow[s.eidx[np + k]], s.T[k * np + u], t);
-    yh[e] = t;
+  const int lane = tid & (RBMPP_WAVE - 1), wid = tid >> 6;
+  const int r0 = 16 * (wid >> 1);
+  if (r0 < np) {  // wave-uniform
+    const int mr = lane & 15, ks = lane >> 4;
+    const int ra = r0 + mr;
+    const double* srow = S + (size_t)s.eidx[np + min(ra, np - 1)] * two + N;
+    f64x4_t acc[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[j] = f64x4_t{0.0, 0.0, 0.0, 0.0};
+    const int cbase = 16 * 4 * (wid & 1) + mr;
+    for (int k0 = 0; k0 < np; k0 += 4) {
+      const int k = k0 + ks;
+      const double a = (ra < np && k < np) ? srow[s.eidx[np + k]] : 0.0;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int cb = cbase + 16 * j;
+        const double b = (cb < np && k < np) ? s.T[k * np + cb] : 0.0;
+        acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[j], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int cb = cbase + 16 * j;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int d = r0 + ks + 4 * r;
+        if (d < np && cb < np) yh[(size_t)d * np + cb] = acc[j][r];
+      }
+    }
   }
   __syncthreads();
 }
